@@ -1,0 +1,209 @@
+"""Benchmark of the LDDMM registration hot path on one MI355X.
+
+Workload (default, `--config sphere_cube`): 3D surface matching, source an
+icosphere of radius 1 subdivided 5 times (10,242 vertices, 20,480
+triangles), target the surface of the cube [-0.8, 0.8]^3 with 58 x 58 cells
+per face (40,368 triangles), Matern nu = 3/2 deformation kernel of width
+0.3, varifold Gaussian width 0.15 x Binet, 20 Euler steps, matching weight
+100, all FP64.
+
+A step is one evaluation of the objective and its gradient (forward flow,
+varifold, adjoint sweep) with every input already resident in HBM.  The
+metric is kernel pair-interactions per second: (T-1) N^2 forward + (T-1) N^2
+fused backward + Ed^2 + Ed Et varifold pairs per evaluation (the constant
+target-target term is computed once per problem and excluded).
+
+Besides the timed steps the run reports one full L-BFGS registration
+(300 iterations, memory 5, weak Wolfe 0 / 0.5) as registrations/s, the
+roofline of the dominant kernel from HIP events on the stream it runs on,
+and a CPU baseline: the FP64 C restatement of the reference (oracle/) timed
+on a bounded sample of the same workload on one host core.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import lddmm_amd  # noqa: E402
+from lddmm_amd import geometry as G  # noqa: E402
+
+FP64_PEAK = 78.6e12   # MI355X FP64 vector FLOP/s
+
+# Algorithmic FLOPs per pair interaction (FMA = 2, add/sub/mul = 1;
+# transcendentals (exp, sqrt) counted separately), minimal algebraic form,
+# frozen before measuring.  Keys: (kernel, dim, knl_order or vf kernel pair).
+FLOPS_PER_PAIR = {
+    ("fwd_pairs", 3, -1): 15, ("fwd_pairs", 3, 1): 17, ("fwd_pairs", 2, -1): 10,
+    ("bwd_pairs", 3, -1): 33, ("bwd_pairs", 3, 1): 35, ("bwd_pairs", 2, -1): 22,
+    ("vf_pairs", 3, "GB"): 32, ("vf_pairs", 2, "GB"): 23,
+}
+TRANS_PER_PAIR = {("fwd_pairs", -1): 1, ("fwd_pairs", 1): 2, ("bwd_pairs", -1): 1, ("bwd_pairs", 1): 2,
+                  ("vf_pairs", "GB"): 1}
+
+
+def workload(name):
+    if name == "sphere_cube":
+        return G.sphere_cube(subdivisions=5, cells=58), dict(itr=300)
+    if name == "circle_square":
+        return G.circle_square(400), dict(itr=200)
+    raise SystemExit("unknown config " + name)
+
+
+def make_objective(w):
+    n = w["q0"].shape[0]
+    ifo = lddmm_amd.getElmIfo(w["elm"], n, one_based=False)
+    obj = lddmm_amd.Objective(w["q0"], w["elm"], ifo, w["tgt_cen"], w["tgt_vol"], w["tgt_dir"], w["cen_type"],
+                              w["cen_width"], w["dir_type"], w["dir_width"], w["dis_wgt"], w["knl_order"],
+                              w["knl_width"], w["time_stp"], w["time_num"])
+    return obj, ifo
+
+
+def cpu_baseline(w, ifo, pairs_per_eval):
+    """FP64 C restatement of the reference CPU path (oracle/), one core,
+    on a bounded sample: one forward K.alpha product and one adjoint step
+    (K.amp and dq) at the full N, and the varifold with gradient of the
+    full deformed surface against a 1/16 slice of the target; the per-pair
+    cost of each part is scaled to one full evaluation (the reference's
+    own evaluation does 3 (T-1) N^2 + Ed^2 + Ed Et + Et^2 pair evaluations;
+    the rate is quoted in the new path's pair count so the two are
+    comparable per evaluation)."""
+    import oracle
+    q = w["q0"]
+    n, d = q.shape
+    T1 = w["time_num"] - 1
+    a = 0.01 * G.normal(7, n * d).reshape(n, d)
+    t0 = time.perf_counter()
+    v = oracle.multiply_kernel(q, a, w["knl_order"], w["knl_width"])
+    t_mult = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.dq_kernel(q, a, v, w["knl_order"], w["knl_width"])
+    t_dq = time.perf_counter() - t0
+    et = len(w["tgt_vol"])
+    sl = slice(0, max(1, et // 16))
+    t0 = time.perf_counter()
+    oracle.varifold(q, w["elm"], ifo, w["tgt_cen"][sl], w["tgt_dir"][sl], w["tgt_vol"][sl], w["cen_type"],
+                    w["cen_width"], w["dir_type"], w["dir_width"])
+    t_vf = time.perf_counter() - t0
+    ed = w["elm"].shape[0]
+    ets = sl.stop
+    vf_pairs_sample = ed * ed + ed * ets + ets * ets
+    vf_pairs_full = ed * ed + ed * et + et * et     # the reference recomputes the target-target term
+    t_eval = T1 * (2 * t_mult + t_dq) + t_vf * vf_pairs_full / vf_pairs_sample
+    sample_s = t_mult + t_dq + t_vf
+    return dict(value=pairs_per_eval / t_eval, unit="pair-interactions/s", cores=1, kind="port",
+                sample=("oracle/lddmm_oracle.c, 1 core: K.alpha + dq at N=%d and the varifold with gradient vs "
+                        "%d of %d target triangles (%.1f s of CPU work), scaled to one full evaluation "
+                        "= %.1f s" % (n, ets, et, sample_s, t_eval)),
+                seconds_per_eval=t_eval)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="sphere_cube")
+    ap.add_argument("--no-registration", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (problems would shard by "
+              "target rows or by independent registrations; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    w, extra = workload(args.config)
+    obj, ifo = make_objective(w)
+    n, d = w["q0"].shape
+    T = w["time_num"]
+    nv = obj.varnum
+    # a seeded, non-trivial momentum in the device layout (T-1, D, N)
+    alp = (0.02 * G.normal(11, nv)).reshape(T - 1, d, n)
+    d_alp = lddmm_amd.DeviceArray(alp)
+    d_grd = lddmm_amd.DeviceArray(nbytes=nv * 8)
+    d_f = lddmm_amd.DeviceArray(nbytes=8)
+    pairs = obj.pairs_per_eval()
+
+    for _ in range(args.warmup):
+        obj.eval_device(d_f.ptr, d_grd.ptr, d_alp.ptr)
+    obj.synchronize()
+    lddmm_amd.profile_reset()
+    lddmm_amd.profile_enable(True)
+    lddmm_amd.lib().lddmm_device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        obj.eval_device(d_f.ptr, d_grd.ptr, d_alp.ptr)
+    obj.synchronize()
+    lddmm_amd.lib().lddmm_device_synchronize()
+    elapsed = time.perf_counter() - t0
+    lddmm_amd.profile_enable(False)
+    prof = lddmm_amd.profile_read()
+    ms_step = 1e3 * elapsed / args.steps
+    value = pairs * args.steps / elapsed
+
+    # dominant kernel and its roofline (algorithmic FLOPs / HIP-event time)
+    order = w["knl_order"]
+    vfkey = w["cen_type"] + w["dir_type"]
+    per_launch = {
+        "fwd_pairs": (n * n, FLOPS_PER_PAIR.get(("fwd_pairs", d, order))),
+        "bwd_pairs": (n * n, FLOPS_PER_PAIR.get(("bwd_pairs", d, order))),
+        "vf_pairs": (w["elm"].shape[0] * (w["elm"].shape[0] + len(w["tgt_vol"])),
+                     FLOPS_PER_PAIR.get(("vf_pairs", d, vfkey))),
+    }
+    kernels = {}
+    for k, (np_, fl) in per_launch.items():
+        ms, cnt = prof.get(k, (0.0, 0))
+        if cnt:
+            avg = ms / cnt * 1e-3
+            kernels[k] = dict(avg_ms=1e3 * avg, launches=cnt, pairs_per_launch=np_,
+                              pairs_per_s=np_ / avg, flops_per_pair=fl,
+                              tflops=(np_ * fl / avg / 1e12) if fl else None)
+    dom = max(kernels, key=lambda k: prof[k][0])
+    kd = kernels[dom]
+    roofline = dict(bound="valu", kernel=dom, achieved=kd["tflops"], peak=FP64_PEAK / 1e12, unit="TFLOP/s",
+                    frac=(kd["tflops"] / (FP64_PEAK / 1e12)) if kd["tflops"] else None, traffic=None,
+                    pairs_per_s=kd["pairs_per_s"], avg_launch_ms=kd["avg_ms"],
+                    flops_per_pair=kd["flops_per_pair"])
+
+    out = {
+        "metric": "kernel pair-interactions/s (objective+gradient evaluation, FP64)",
+        "value": value,
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded icosphere -> cube, seeded momentum)",
+        "config": {"workload": args.config + (": icosphere-5 (N=10242, Ed=20480) -> cube 58x58/face (Et=40368), "
+                                              "Matern nu=3/2 w=0.3, varifold G0.15xBinet, T-1=20"
+                                              if args.config == "sphere_cube" else ""),
+                   "pairs_per_eval": pairs, "evals_per_s": 1e3 / ms_step},
+        "roofline": roofline,
+        "kernels": kernels,
+    }
+
+    if not args.no_registration:
+        alp_opt, info = lddmm_amd.run(w["q0"], None, w["elm"], ifo, w["tgt_cen"], w["tgt_vol"], w["tgt_dir"],
+                                      w["cen_type"], w["cen_width"], w["dir_type"], w["dir_width"], w["dis_wgt"],
+                                      w["knl_order"], w["knl_width"], w["time_stp"], T, extra["itr"], 0.0, 0.0, 0.5, 5)
+        out["registration"] = dict(iterations=info["iterations"], evaluations=info["evaluations"],
+                                   status=info["status"], final_energy=info["fcn"], grad_norm=info["grad_norm"],
+                                   seconds=info["seconds"], registrations_per_s=1.0 / info["seconds"],
+                                   precision="fp64")
+
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(w, ifo, pairs)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

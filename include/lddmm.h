@@ -1,0 +1,197 @@
+/*
+ * lddmm.h -- C-ABI of the MI355X (gfx950) LDDMM landmark/varifold
+ * registration library (liblddmm_amd.so).
+ *
+ * Plain C: no HIP, torch or C++ types in any signature.  Every entry point
+ * returns an int status: 0 = ok; 1, 2, 3 = the reference's line-search
+ * codes (not a descent direction / no bracket / bracketed but weak Wolfe
+ * not met); negative = LDDMM_ERR_*.  The last error message of the calling
+ * thread is available from lddmm_last_error().
+ *
+ * Layouts.  "Host layout" is the reference's Cpp MEX layout
+ * (Cpp/src/LDDMM.m:7-14): a point set is D x N column-major (AoS
+ * x0 y0 [z0] x1 ...), a momentum stack is D x N x (T-1), element tables are
+ * int32 V x E 0-based, the adjacency table (1 + maxAdj*V) x N.  "Device
+ * layout" is the reference's CUDA MEX layout (CUDA/src/LDDMM.m:7-19): N x D
+ * column-major (SoA), stacks N x D x (T-1), elements E x V, adjacency
+ * N x (1 + maxAdj*V).
+ *
+ * Kernel orders: -1 Gaussian, 0..4 Matern (Cpp/src/computeKernel.cpp:23-24).
+ * 2D Matern orders return LDDMM_ERR_UNSUPPORTED on this GPU path.
+ */
+#ifndef LDDMM_AMD_H
+#define LDDMM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LDDMM_OK 0
+#define LDDMM_ERR_ARG (-1)
+#define LDDMM_ERR_ALLOC (-2)
+#define LDDMM_ERR_HIP (-3)
+#define LDDMM_ERR_UNSUPPORTED (-4)
+
+#define LDDMM_PRECISION_FP64 0
+
+/* Options beyond the reference's 22 arguments; NULL = defaults. */
+typedef struct lddmm_options {
+    int precision;            /* LDDMM_PRECISION_FP64 (reference arithmetic) */
+    double rel_energy_tol;    /* > 0: also stop once (f_prev - f)/|f| < rel_energy_tol */
+    int pair_target_blocks;   /* workgroups per all-pairs launch; 0 = default */
+    int reserved[5];
+} lddmm_options;
+
+/* Run summary; may be NULL. */
+typedef struct lddmm_result {
+    double fcn;               /* final objective value */
+    double grad_norm;         /* |grad f| at the final iterate */
+    double seconds;           /* wall time of the optimisation */
+    double pairs_per_eval;    /* kernel pair interactions per objective evaluation */
+    int iterations;           /* accepted L-BFGS iterations */
+    int evaluations;          /* objective evaluations */
+    int status;               /* 0 or line-search code 1..3 */
+    int reserved;
+} lddmm_result;
+
+/* ---------------------------------------------------------------- registration
+ * Replaces the LDDMM2D / LDDMM3D MEX entry of the CPU variant
+ * (Cpp/src/LDDMM.cpp:11-98, caller Cpp/src/LDDMM.m:16-36): the 22 inputs in
+ * their order and meaning (prhs[0..21]), then the sizes MATLAB reads from
+ * the arrays (mxGetN of prhs[0], prhs[2], prhs[4]; mxGetM of prhs[3]), then
+ * the output plhs[0] = optAlpVec (D*N*(T-1), caller-allocated), which always
+ * holds the last accepted iterate.  Host layout. */
+int lddmm_run_2d(const double *iniNdeMat, const double *iniAlpVec,
+                 const int32_t *dfmElmVtxMat, const int32_t *dfmElmIfoMat,
+                 const double *tgtCenPosMat, const double *tgtElmVolVec, const double *tgtUniDirMat,
+                 char cenKnlType, double cenKnlWidth, char dirKnlType, double dirKnlWidth, double disWgt,
+                 int knlOrder, double knlWidth, double timeStp, int timeNum,
+                 int itrMax, double tolVal, double wolfe1, double wolfe2, int memNum, int vbsFlg,
+                 int ndeNum, int dfmElmNum, int dfmElmIfoNum, int tgtElmNum,
+                 double *optAlpVec);
+int lddmm_run_3d(const double *iniNdeMat, const double *iniAlpVec,
+                 const int32_t *dfmElmVtxMat, const int32_t *dfmElmIfoMat,
+                 const double *tgtCenPosMat, const double *tgtElmVolVec, const double *tgtUniDirMat,
+                 char cenKnlType, double cenKnlWidth, char dirKnlType, double dirKnlWidth, double disWgt,
+                 int knlOrder, double knlWidth, double timeStp, int timeNum,
+                 int itrMax, double tolVal, double wolfe1, double wolfe2, int memNum, int vbsFlg,
+                 int ndeNum, int dfmElmNum, int dfmElmIfoNum, int tgtElmNum,
+                 double *optAlpVec);
+/* Same, with options and a run summary; dimNum is 2 or 3. */
+int lddmm_run_ex(int dimNum, const double *iniNdeMat, const double *iniAlpVec,
+                 const int32_t *dfmElmVtxMat, const int32_t *dfmElmIfoMat,
+                 const double *tgtCenPosMat, const double *tgtElmVolVec, const double *tgtUniDirMat,
+                 char cenKnlType, double cenKnlWidth, char dirKnlType, double dirKnlWidth, double disWgt,
+                 int knlOrder, double knlWidth, double timeStp, int timeNum,
+                 int itrMax, double tolVal, double wolfe1, double wolfe2, int memNum, int vbsFlg,
+                 int ndeNum, int dfmElmNum, int dfmElmIfoNum, int tgtElmNum,
+                 double *optAlpVec, const lddmm_options *opts, lddmm_result *result);
+
+/* Replaces the LDDMM2D / LDDMM3D MEX entry of the CUDA variant
+ * (CUDA/src/LDDMM.cu:13-151, caller CUDA/src/LDDMM.m:21-41): every array is
+ * a device pointer in the device layout; d_optAlpVec is caller-allocated on
+ * the device.  stream is a hipStream_t (NULL = the default stream); the call
+ * returns after the run has finished on it. */
+int lddmm_run_dev(int dimNum, const double *d_iniNdeMat, const double *d_iniAlpVec,
+                  const int32_t *d_dfmElmVtxMat, const int32_t *d_dfmElmIfoMat,
+                  const double *d_tgtCenPosMat, const double *d_tgtElmVolVec, const double *d_tgtUniDirMat,
+                  char cenKnlType, double cenKnlWidth, char dirKnlType, double dirKnlWidth, double disWgt,
+                  int knlOrder, double knlWidth, double timeStp, int timeNum,
+                  int itrMax, double tolVal, double wolfe1, double wolfe2, int memNum, int vbsFlg,
+                  int ndeNum, int dfmElmNum, int dfmElmIfoNum, int tgtElmNum,
+                  double *d_optAlpVec, const lddmm_options *opts, lddmm_result *result, void *stream);
+
+/* ---------------------------------------------------------------- objective
+ * The objective and gradient of the CPU variant, void objgrd(double *fcnVal,
+ * double *grdStk, double *alpStk, fcndata &) (Cpp/src/objgrd.cpp:26-90),
+ * with fcndata built once by lddmm_objective_create (the role of
+ * assignFcnStructMemory, Cpp/src/assignStructMemory.cpp:49-109).  Host
+ * layout for alpStk / grdStk (D x N x (T-1)) and for every create input. */
+typedef struct lddmm_objective lddmm_objective;
+int lddmm_objective_create(lddmm_objective **out, int dimNum, const double *iniNdeMat,
+                           const int32_t *dfmElmVtxMat, const int32_t *dfmElmIfoMat,
+                           const double *tgtCenPosMat, const double *tgtElmVolVec, const double *tgtUniDirMat,
+                           char cenKnlType, double cenKnlWidth, char dirKnlType, double dirKnlWidth,
+                           double disWgt, int knlOrder, double knlWidth, double timeStp, int timeNum,
+                           int ndeNum, int dfmElmNum, int dfmElmIfoNum, int tgtElmNum);
+int lddmm_objgrd(lddmm_objective *obj, double *fcnVal, double *grdStk, const double *alpStk);
+/* Device-layout form (the CUDA variant's objgrd, CUDA/src/objgrd.cu:13-76,
+ * works on device arrays): d_alpStk / d_grdStk are N x D x (T-1) device
+ * arrays, d_fcnVal one device double.  Enqueued on the handle's stream and
+ * asynchronous; lddmm_objective_synchronize waits for it. */
+int lddmm_objgrd_dev(lddmm_objective *obj, double *d_fcnVal, double *d_grdStk, const double *d_alpStk);
+int lddmm_objective_synchronize(lddmm_objective *obj);
+/* pair interactions per evaluation: (T-1) N^2 forward + (T-1) N^2 fused
+ * backward + Ed^2 + Ed Et varifold */
+int lddmm_objective_pairs(lddmm_objective *obj, double *pairsPerEval);
+/* D x N x T host-layout node trajectory of the last lddmm_objgrd call (ndeStk). */
+int lddmm_objective_trajectory(lddmm_objective *obj, double *ndeStk);
+/* The constant target-target varifold term, cached at create time. */
+int lddmm_objective_target_term(lddmm_objective *obj, double *value);
+void lddmm_objective_destroy(lddmm_objective *obj);
+
+/* ---------------------------------------------------------------- pair operators
+ * multiplyKernel MEX, 4- and 5-argument forms (Cpp/src/multiplyKernelITF.cpp:16-59;
+ * Cpp/src/multiplyKernel.cpp:22,294): v_i = sum_j k(q_i, q_j) alpha_j.  Host layout. */
+int lddmm_multiply_kernel(double *vlcMat, const double *lmkMat, const double *alpMat,
+                          int knlOrder, double knlWidth, int lmkNum, int dimNum);
+int lddmm_multiply_kernel_cross(double *vlcMat, const double *lmkiMat, const double *lmkjMat,
+                                const double *alpMat, int knlOrder, double knlWidth,
+                                int lmkiNum, int lmkjNum, int dimNum);
+/* dqKernel (Cpp/src/dqKernel.cpp:31, 307; MATLAB/src/dqKernelITF.cpp:16-72):
+ * gradient of lft^T K(q) rgt with respect to q; cross form gives both sides. */
+int lddmm_dq_kernel(double *dqKMat, const double *lmkMat, const double *lftMat, const double *rgtMat,
+                    int knlOrder, double knlWidth, int lmkNum, int dimNum);
+int lddmm_dq_kernel_cross(double *dqiKMat, double *dqjKMat, const double *lmkiMat, const double *lmkjMat,
+                          const double *lftMat, const double *rgtMat, int knlOrder, double knlWidth,
+                          int lmkiNum, int lmkjNum, int dimNum);
+/* computeKernel (Cpp/src/computeKernel.cpp:21, 246; MATLAB/src/computeKernelITF.cpp:15-59):
+ * dense matrix, self form knlMat[i*N + j], cross form knlMat[j*Ni + i]. */
+int lddmm_compute_kernel(double *knlMat, const double *lmkMat, int knlOrder, double knlWidth,
+                         int lmkNum, int dimNum);
+int lddmm_compute_kernel_cross(double *knlMat, const double *lmkiMat, const double *lmkjMat,
+                               int knlOrder, double knlWidth, int lmkiNum, int lmkjNum, int dimNum);
+/* varifold(...) value and (when dqVfdMat != NULL) vertex gradient
+ * (Cpp/src/varifold3D.cpp:393-435, Cpp/src/varifold2D.cpp:242-284; called by
+ * varifoldITF, Cpp/src/varifoldITF.cpp:7-19).  The reference's workspace
+ * arguments are owned by the library.  Host layout. */
+int lddmm_varifold(double *vfdVal, double *dqVfdMat, const double *dfmLmkPosMat,
+                   const int32_t *dfmElmVtxMat, const int32_t *dfmElmIfoMat,
+                   const double *tgtCenPosMat, const double *tgtUniDirMat, const double *tgtElmVolVec,
+                   char cenKnlType, double cenKnlWidth, char dirKnlType, double dirKnlWidth,
+                   int dfmLmkNum, int dfmElmNum, int dfmElmIfoNum, int tgtElmNum, int dimNum);
+
+/* ---------------------------------------------------------------- data formats
+ * getElmIfo (Cpp/src/getElmIfo.m:1-59) from a 0-based V x E table; writes
+ * the (1 + maxAdj*V) x N table when elmIfoMat != NULL and returns the row
+ * count (negative on error). */
+int lddmm_get_elm_ifo(int32_t *elmIfoMat, const int32_t *elmVtxMat, int vtxPerElm, int elmNum, int ndeNum);
+/* computeNodePosition (Cpp/src/computeNodePosition.m:1-20): ndeStk D x N x T,
+ * vlcStk D x N x (T-1) (either may be NULL).  Host layout. */
+int lddmm_compute_node_position(double *ndeStk, double *vlcStk, const double *iniNdeMat,
+                                const double *alpStk, int knlOrder, double knlWidth, double timeStp,
+                                int timeNum, int ndeNum, int dimNum);
+
+/* ---------------------------------------------------------------- utilities
+ * (no reference counterpart) */
+const char *lddmm_last_error(void);
+int lddmm_device_count(int *count);
+int lddmm_device_malloc(void **ptr, size_t bytes);
+int lddmm_device_free(void *ptr);
+int lddmm_memcpy_htod(void *dst, const void *src, size_t bytes);
+int lddmm_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int lddmm_device_synchronize(void);
+/* Kernel-time accounting: HIP events around every launch group on the
+ * stream it runs on.  names[k] / ms[k] / counts[k] for k < return value. */
+int lddmm_profile_enable(int on);
+int lddmm_profile_reset(void);
+int lddmm_profile_read(const char **names, double *ms, long long *counts, int maxEntries);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,375 @@
+"""lddmm_amd -- LDDMM landmark/curve/surface registration with a varifold
+matching term on AMD Instinct MI355X (gfx950).
+
+The compute path is liblddmm_amd.so (hand-written HIP kernels behind the
+C-ABI in include/lddmm.h).  This module is a thin ctypes mirror of the
+reference's MATLAB-facing interface (dnhsieh/LDDMM Cpp/src/LDDMM.m,
+multiplyKernel MEX, getElmIfo.m, computeNodePosition.m) so driver code and
+tests read like the reference's.  There is no CPU fallback: every call goes
+to the GPU library and raises when it is missing or fails.
+
+Array conventions: a MATLAB D x N matrix is a numpy array of shape (N, D)
+in C order (the same memory); a D x N x (T-1) stack is shape (T-1, N, D);
+element tables are (E, V).  `LDDMM()` and `getElmIfo()` take MATLAB's
+1-based element tables, like the reference's .m files; the lower-level
+functions take 0-based int32 tables, like the MEX entry points.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = ["lib", "LDDMM", "getElmIfo", "computeNodePosition", "multiplyKernel", "dqKernel", "computeKernel",
+           "varifold", "Objective", "run", "LddmmError", "profile_enable", "profile_read", "profile_reset"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblddmm_amd.so")
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class LddmmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("lddmm error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Options(C.Structure):
+    _fields_ = [("precision", C.c_int), ("rel_energy_tol", C.c_double), ("pair_target_blocks", C.c_int),
+                ("reserved", C.c_int * 5)]
+
+
+class Result(C.Structure):
+    _fields_ = [("fcn", C.c_double), ("grad_norm", C.c_double), ("seconds", C.c_double),
+                ("pairs_per_eval", C.c_double), ("iterations", C.c_int), ("evaluations", C.c_int),
+                ("status", C.c_int), ("reserved", C.c_int)]
+
+
+_RUN_ARGS = [_dp, _dp, _ip, _ip, _dp, _dp, _dp, C.c_char, C.c_double, C.c_char, C.c_double, C.c_double,
+             C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+             C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
+
+
+def lib():
+    """The HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("liblddmm_amd.so is not built (run `python -m lddmm_amd.build`); "
+                               "there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        L.lddmm_last_error.restype = C.c_char_p
+        L.lddmm_run_2d.argtypes = _RUN_ARGS
+        L.lddmm_run_3d.argtypes = _RUN_ARGS
+        L.lddmm_run_ex.argtypes = [C.c_int] + _RUN_ARGS + [C.POINTER(Options), C.POINTER(Result)]
+        L.lddmm_run_dev.argtypes = [C.c_int] + [C.c_void_p] * 7 + _RUN_ARGS[7:26] + [
+            C.c_void_p, C.POINTER(Options), C.POINTER(Result), C.c_void_p]
+        L.lddmm_objective_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, _dp, _ip, _ip, _dp, _dp, _dp,
+                                             C.c_char, C.c_double, C.c_char, C.c_double, C.c_double, C.c_int,
+                                             C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.lddmm_objgrd.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.lddmm_objective_trajectory.argtypes = [C.c_void_p, _dp]
+        L.lddmm_objgrd_dev.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.lddmm_objective_synchronize.argtypes = [C.c_void_p]
+        L.lddmm_objective_pairs.argtypes = [C.c_void_p, _dp]
+        L.lddmm_objective_target_term.argtypes = [C.c_void_p, _dp]
+        L.lddmm_objective_destroy.argtypes = [C.c_void_p]
+        L.lddmm_objective_destroy.restype = None
+        L.lddmm_multiply_kernel.argtypes = [_dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int]
+        L.lddmm_multiply_kernel_cross.argtypes = [_dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int,
+                                                  C.c_int]
+        L.lddmm_dq_kernel.argtypes = [_dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int]
+        L.lddmm_dq_kernel_cross.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int,
+                                            C.c_int]
+        L.lddmm_compute_kernel.argtypes = [_dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int]
+        L.lddmm_compute_kernel_cross.argtypes = [_dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int]
+        L.lddmm_varifold.argtypes = [_dp, _dp, _dp, _ip, _ip, _dp, _dp, _dp, C.c_char, C.c_double, C.c_char,
+                                     C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.lddmm_get_elm_ifo.argtypes = [_ip, _ip, C.c_int, C.c_int, C.c_int]
+        L.lddmm_compute_node_position.argtypes = [_dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_int,
+                                                  C.c_int, C.c_int]
+        L.lddmm_device_count.argtypes = [C.POINTER(C.c_int)]
+        L.lddmm_device_malloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        L.lddmm_device_free.argtypes = [C.c_void_p]
+        L.lddmm_memcpy_htod.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        L.lddmm_memcpy_dtoh.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        L.lddmm_profile_enable.argtypes = [C.c_int]
+        L.lddmm_profile_read.argtypes = [C.POINTER(C.c_char_p), _dp, C.POINTER(C.c_longlong), C.c_int]
+        _lib = L
+    return _lib
+
+
+def _check(rc, allow_positive=False):
+    if rc < 0 or (rc > 0 and not allow_positive):
+        raise LddmmError(rc, lib().lddmm_last_error().decode(errors="replace"))
+    return rc
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _p(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp if a.dtype == np.float64 else _ip)
+
+
+def _ch(c):
+    return c.encode() if isinstance(c, str) else bytes([c])
+
+
+# ---------------------------------------------------------------- data formats
+def getElmIfo(elmVtxMat, ndeNum, one_based=True):
+    """Adjacency table of getElmIfo.m (Cpp/src/getElmIfo.m:1-59): returns an
+    int32 array of shape (ndeNum, rows) (MATLAB's rows x ndeNum)."""
+    elm = _i32(np.asarray(elmVtxMat) - (1 if one_based else 0))
+    ne, vpe = elm.shape
+    rows = _check(lib().lddmm_get_elm_ifo(None, _p(elm), vpe, ne, ndeNum), allow_positive=True)
+    ifo = np.zeros((ndeNum, rows), dtype=np.int32)
+    _check(lib().lddmm_get_elm_ifo(_p(ifo), _p(elm), vpe, ne, ndeNum), allow_positive=True)
+    return ifo
+
+
+# ---------------------------------------------------------------- operators
+def multiplyKernel(*args):
+    """multiplyKernel(lmk, alp, order, width) or
+    multiplyKernel(lmki, lmkj, alp, order, width) (Cpp/src/multiplyKernelITF.cpp:16-61)."""
+    if len(args) == 4:
+        q, a, order, width = args
+        q, a = _f64(q), _f64(a)
+        v = np.empty_like(q)
+        _check(lib().lddmm_multiply_kernel(_p(v), _p(q), _p(a), order, width, q.shape[0], q.shape[1]))
+        return v
+    if len(args) == 5:
+        qi, qj, a, order, width = args
+        qi, qj, a = _f64(qi), _f64(qj), _f64(a)
+        v = np.empty_like(qi)
+        _check(lib().lddmm_multiply_kernel_cross(_p(v), _p(qi), _p(qj), _p(a), order, width, qi.shape[0],
+                                                 qj.shape[0], qi.shape[1]))
+        return v
+    raise TypeError("The number of inputs must be 4 or 5.")
+
+
+def dqKernel(*args):
+    """dqKernel(lmk, lft, rgt, order, width) -> dq, or
+    dqKernel(lmki, lmkj, lft, rgt, order, width) -> (dqi, dqj)
+    (MATLAB/src/dqKernelITF.cpp:16-72)."""
+    if len(args) == 5:
+        q, l, r, order, width = args
+        q, l, r = _f64(q), _f64(l), _f64(r)
+        g = np.empty_like(q)
+        _check(lib().lddmm_dq_kernel(_p(g), _p(q), _p(l), _p(r), order, width, q.shape[0], q.shape[1]))
+        return g
+    if len(args) == 6:
+        qi, qj, l, r, order, width = args
+        qi, qj, l, r = _f64(qi), _f64(qj), _f64(l), _f64(r)
+        gi, gj = np.empty_like(qi), np.empty_like(qj)
+        _check(lib().lddmm_dq_kernel_cross(_p(gi), _p(gj), _p(qi), _p(qj), _p(l), _p(r), order, width,
+                                           qi.shape[0], qj.shape[0], qi.shape[1]))
+        return gi, gj
+    raise TypeError("The number of inputs must be 5 or 6.")
+
+
+def computeKernel(*args):
+    """computeKernel(lmk, order, width) -> (N, N) with K[i, j];
+    computeKernel(lmki, lmkj, order, width) -> (Nj, Ni) holding the
+    reference's Ni x Nj column-major matrix (MATLAB/src/computeKernelITF.cpp:15-59)."""
+    if len(args) == 3:
+        q, order, width = args
+        q = _f64(q)
+        K = np.empty((q.shape[0], q.shape[0]))
+        _check(lib().lddmm_compute_kernel(_p(K), _p(q), order, width, q.shape[0], q.shape[1]))
+        return K
+    if len(args) == 4:
+        qi, qj, order, width = args
+        qi, qj = _f64(qi), _f64(qj)
+        K = np.empty((qj.shape[0], qi.shape[0]))
+        _check(lib().lddmm_compute_kernel_cross(_p(K), _p(qi), _p(qj), order, width, qi.shape[0], qj.shape[0],
+                                                qi.shape[1]))
+        return K
+    raise TypeError("The number of inputs must be 3 or 4.")
+
+
+def varifold(q, elm, ifo, tgt_cen, tgt_dir, tgt_vol, cen_type, cen_width, dir_type, dir_width, grad=True):
+    """Varifold distance (and vertex gradient) of the deformed set q with
+    0-based (E, V) elements against a target given by centres, unit
+    directions and volumes (Cpp/src/varifold3D.cpp:393-435)."""
+    q, elm, ifo = _f64(q), _i32(elm), _i32(ifo)
+    tc, tu, ta = _f64(tgt_cen), _f64(tgt_dir), _f64(tgt_vol)
+    val = C.c_double()
+    dq = np.empty_like(q) if grad else None
+    _check(lib().lddmm_varifold(C.byref(val), _p(dq), _p(q), _p(elm), _p(ifo), _p(tc), _p(tu), _p(ta),
+                                _ch(cen_type), cen_width, _ch(dir_type), dir_width, q.shape[0], elm.shape[0],
+                                ifo.shape[1], len(ta), q.shape[1]))
+    return (val.value, dq) if grad else val.value
+
+
+def computeNodePosition(iniNdeMat, alpStk, knlOrder, knlWidth, timeStp, timeNum):
+    """Trajectory and velocities of the Euler flow (Cpp/src/computeNodePosition.m:1-20):
+    returns ndeStk (T, N, D) and vlcStk (T-1, N, D)."""
+    q0, a = _f64(iniNdeMat), _f64(alpStk)
+    n, d = q0.shape
+    nde = np.empty((timeNum, n, d))
+    vlc = np.empty((max(timeNum - 1, 0), n, d))
+    _check(lib().lddmm_compute_node_position(_p(nde), _p(vlc), _p(q0), _p(a), knlOrder, knlWidth, timeStp,
+                                             timeNum, n, d))
+    return nde, vlc
+
+
+# ---------------------------------------------------------------- objective
+class Objective:
+    """objgrd (Cpp/src/objgrd.cpp:26-90) on the GPU for a fixed problem.
+    elm is a 0-based (E, V) table; ifo the (N, rows) adjacency table."""
+
+    def __init__(self, q0, elm, ifo, tgt_cen, tgt_vol, tgt_dir, cen_type, cen_width, dir_type, dir_width,
+                 dis_wgt, knl_order, knl_width, time_stp, time_num):
+        self.q0, self.elm, self.ifo = _f64(q0), _i32(elm), _i32(ifo)
+        self.tc, self.ta, self.tu = _f64(tgt_cen), _f64(tgt_vol), _f64(tgt_dir)
+        self.n, self.dim = self.q0.shape
+        self.T = int(time_num)
+        self.varnum = self.dim * self.n * (self.T - 1)
+        h = C.c_void_p()
+        _check(lib().lddmm_objective_create(C.byref(h), self.dim, _p(self.q0), _p(self.elm), _p(self.ifo),
+                                            _p(self.tc), _p(self.ta), _p(self.tu), _ch(cen_type), cen_width,
+                                            _ch(dir_type), dir_width, dis_wgt, knl_order, knl_width, time_stp,
+                                            self.T, self.n, self.elm.shape[0], self.ifo.shape[1], len(self.ta)))
+        self._h = h
+
+    def __call__(self, alp):
+        a = _f64(alp).reshape(-1)
+        if a.size != self.varnum:
+            raise ValueError("alpha must hold D*N*(T-1) = %d values" % self.varnum)
+        g = np.empty(self.varnum)
+        f = C.c_double()
+        _check(lib().lddmm_objgrd(self._h, C.byref(f), _p(g), _p(a)))
+        return f.value, g
+
+    def eval_device(self, d_fcn, d_grd, d_alp):
+        """Asynchronous evaluation on device arrays (device layout)."""
+        _check(lib().lddmm_objgrd_dev(self._h, d_fcn, d_grd, d_alp))
+
+    def synchronize(self):
+        _check(lib().lddmm_objective_synchronize(self._h))
+
+    def pairs_per_eval(self):
+        v = C.c_double()
+        _check(lib().lddmm_objective_pairs(self._h, C.byref(v)))
+        return v.value
+
+    def trajectory(self):
+        out = np.empty((self.T, self.n, self.dim))
+        _check(lib().lddmm_objective_trajectory(self._h, _p(out)))
+        return out
+
+    def target_term(self):
+        v = C.c_double()
+        _check(lib().lddmm_objective_target_term(self._h, C.byref(v)))
+        return v.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lddmm_objective_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- registration
+def run(q0, alp0, elm, ifo, tgt_cen, tgt_vol, tgt_dir, cen_type, cen_width, dir_type, dir_width, dis_wgt,
+        knl_order, knl_width, time_stp, time_num, itr_max, tol, wolfe1, wolfe2, mem, verbose=False,
+        rel_energy_tol=0.0, pair_target_blocks=0):
+    """The LDDMM2D/LDDMM3D MEX call (Cpp/src/LDDMM.cpp:11-98) with 0-based
+    tables.  Returns (optAlpVec reshaped (T-1, N, D), result dict)."""
+    q0, elm, ifo = _f64(q0), _i32(elm), _i32(ifo)
+    tc, ta, tu = _f64(tgt_cen), _f64(tgt_vol), _f64(tgt_dir)
+    n, d = q0.shape
+    T = int(time_num)
+    a0 = np.zeros(d * n * (T - 1)) if alp0 is None else _f64(alp0).reshape(-1)
+    out = np.empty(d * n * (T - 1))
+    opts = Options(0, rel_energy_tol, pair_target_blocks)
+    res = Result()
+    st = _check(lib().lddmm_run_ex(d, _p(q0), _p(a0), _p(elm), _p(ifo), _p(tc), _p(ta), _p(tu), _ch(cen_type),
+                                   cen_width, _ch(dir_type), dir_width, dis_wgt, knl_order, knl_width, time_stp, T,
+                                   itr_max, tol, wolfe1, wolfe2, mem, int(bool(verbose)), n, elm.shape[0],
+                                   ifo.shape[1], len(ta), _p(out), C.byref(opts), C.byref(res)),
+                allow_positive=True)
+    info = dict(fcn=res.fcn, grad_norm=res.grad_norm, seconds=res.seconds, pairs_per_eval=res.pairs_per_eval,
+                iterations=res.iterations, evaluations=res.evaluations, status=st)
+    return out.reshape(T - 1, n, d), info
+
+
+def LDDMM(iniNdeMat, iniAlpStk, disObj, knlOrder, knlWidth, timeEnd, timeStp, optObj):
+    """[ndeStk, vlcStk, alpStk] = LDDMM(...) of Cpp/src/LDDMM.m:1-46.
+    disObj / optObj are dicts with the reference's field names; the element
+    table disObj['dfmElmVtxMat'] is 1-based (E, V) as in MATLAB."""
+    q0 = _f64(iniNdeMat)
+    n, d = q0.shape
+    T = int(np.floor(timeEnd / timeStp)) + 1
+    elm1 = np.asarray(disObj["dfmElmVtxMat"])
+    ifo = getElmIfo(elm1, n)
+    elm = _i32(elm1 - 1)
+    alp0 = None if iniAlpStk is None or np.size(iniAlpStk) == 0 else iniAlpStk
+    alp, _ = run(q0, alp0, elm, ifo, disObj["tgtCenPosMat"], np.ravel(disObj["tgtElmVolVec"]),
+                 disObj["tgtUniDirMat"], disObj["cenKnlType"], disObj["cenKnlWidth"], disObj["dirKnlType"],
+                 disObj["dirKnlWidth"], disObj["disWgt"], knlOrder, knlWidth, timeStp, T,
+                 int(optObj["MaxIterations"]), optObj["OptimalityTolerance"], optObj["Wolfe1"], optObj["Wolfe2"],
+                 int(optObj["Columns"]), optObj.get("Verbose", False))
+    ndeStk, vlcStk = computeNodePosition(q0, alp, knlOrder, knlWidth, timeStp, T)
+    return ndeStk, vlcStk, alp
+
+
+# ---------------------------------------------------------------- profiling
+class DeviceArray:
+    """A device allocation made through the library (no torch involved)."""
+
+    def __init__(self, host=None, nbytes=None):
+        self.nbytes = int(host.nbytes if host is not None else nbytes)
+        self.ptr = C.c_void_p()
+        _check(lib().lddmm_device_malloc(C.byref(self.ptr), max(self.nbytes, 8)))
+        if host is not None:
+            h = np.ascontiguousarray(host)
+            _check(lib().lddmm_memcpy_htod(self.ptr, h.ctypes.data, self.nbytes))
+
+    def get(self, dtype=np.float64):
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype=dtype)
+        _check(lib().lddmm_memcpy_dtoh(out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().lddmm_device_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def profile_enable(on=True):
+    _check(lib().lddmm_profile_enable(int(bool(on))))
+
+
+def profile_reset():
+    _check(lib().lddmm_profile_reset())
+
+
+def profile_read():
+    """{group: (milliseconds, launches)} of the HIP-event kernel accounting."""
+    names = (C.c_char_p * 32)()
+    ms = (C.c_double * 32)()
+    cnt = (C.c_longlong * 32)()
+    k = _check(lib().lddmm_profile_read(names, ms, cnt, 32), allow_positive=True)
+    return {names[i].decode(): (ms[i], cnt[i]) for i in range(k)}

@@ -1,0 +1,787 @@
+// capi.cpp -- the extern "C" boundary declared in include/lddmm.h.
+//
+// Host-layout entry points transpose to the device (SoA) layout on upload
+// and back on download; the device-layout entry points take the caller's
+// device pointers as they are (CUDA/src/LDDMM.m:7-19 is the layout
+// contract).  Exceptions never cross the boundary: every entry point
+// returns a status and keeps the message for lddmm_last_error().
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/lddmm.h"
+#include "engine.hpp"
+
+using namespace lddmm;
+
+namespace {
+
+thread_local std::string g_err;
+
+template <typename F>
+int guarded(F &&f)
+{
+    try {
+        g_err.clear();
+        return f();
+    } catch (const Error &e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_err = "host allocation failed";
+        return kErrAlloc;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return kErrArg;
+    }
+}
+
+void require(bool ok, const char *msg)
+{
+    if (!ok) throw Error(kErrArg, msg);
+}
+
+// RAII device buffer
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    void alloc(size_t count)
+    {
+        n = count;
+        if (count == 0) return;
+        if (hipMalloc((void **)&p, count * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            throw Error(kErrAlloc, "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
+        }
+    }
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    void upload(const T *src, hipStream_t st)
+    {
+        if (n) LDDMM_HIP(hipMemcpyAsync(p, src, n * sizeof(T), hipMemcpyHostToDevice, st));
+    }
+    void download(T *dst, hipStream_t st) const
+    {
+        if (n) LDDMM_HIP(hipMemcpyAsync(dst, p, n * sizeof(T), hipMemcpyDeviceToHost, st));
+    }
+};
+
+struct Stream {
+    hipStream_t s = nullptr;
+    Stream() { LDDMM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+    ~Stream()
+    {
+        if (s) (void)hipStreamDestroy(s);
+    }
+    void sync() { LDDMM_HIP(hipStreamSynchronize(s)); }
+};
+
+// AoS (D x N column-major) -> SoA (N x D column-major), `slices` stacked sets
+template <typename T>
+std::vector<T> to_soa(const T *src, int n, int d, int slices = 1)
+{
+    std::vector<T> out((size_t)n * d * slices);
+    for (int t = 0; t < slices; ++t) {
+        const T *s = src + (size_t)t * n * d;
+        T *o = out.data() + (size_t)t * n * d;
+        for (int i = 0; i < n; ++i)
+            for (int c = 0; c < d; ++c) o[(size_t)c * n + i] = s[(size_t)i * d + c];
+    }
+    return out;
+}
+
+template <typename T>
+void from_soa(T *dst, const T *soa, int n, int d, int slices = 1)
+{
+    for (int t = 0; t < slices; ++t) {
+        const T *s = soa + (size_t)t * n * d;
+        T *o = dst + (size_t)t * n * d;
+        for (int i = 0; i < n; ++i)
+            for (int c = 0; c < d; ++c) o[(size_t)i * d + c] = s[(size_t)c * n + i];
+    }
+}
+
+// Host-side problem arrays converted to the device layout and uploaded.
+struct UploadedProblem {
+    DevBuf<double> q0, tc, tu, ta;
+    DevBuf<int> elm, ifo;
+    DeviceProblem p;
+};
+
+void check_common(int dim, int n, int ne, int ifo_rows, int nt, int T)
+{
+    require(dim == 2 || dim == 3, "dimNum must be 2 or 3");
+    require(n > 0, "ndeNum must be positive");
+    require(ne > 0, "dfmElmNum must be positive");
+    require(nt >= 0, "tgtElmNum must be non-negative");
+    require(ifo_rows >= 1, "dfmElmIfoNum must be at least 1");
+    require(T >= 2, "timeNum must be at least 2");
+}
+
+void fill_params(DeviceProblem &p, int dim, int n, int T, int order, double width, double h, double w, char ct,
+                 double cw, char dt, double dw, int ne, int nt, int rows)
+{
+    p.dim = dim;
+    p.n = n;
+    p.T = T;
+    p.order = order;
+    p.width = width;
+    p.h = h;
+    p.w = w;
+    p.vp = VfParams{ct, dt, cw, dw};
+    p.ne = ne;
+    p.nt = nt;
+    p.ifo_rows = rows;
+}
+
+void upload_host_problem(UploadedProblem &u, int dim, const double *q0, const int32_t *elm, const int32_t *ifo,
+                         const double *tc, const double *ta, const double *tu, int n, int ne, int rows, int nt,
+                         hipStream_t st)
+{
+    require(q0 && elm && ifo, "null input array");
+    require(nt == 0 || (tc && ta && tu), "null target array");
+    const int vpe = dim;   // segments in 2D, triangles in 3D (VFDVTXNUM, Cpp/src/include/constants.h)
+    auto hq = to_soa(q0, n, dim);
+    auto helm = to_soa(elm, ne, vpe);
+    auto hifo = to_soa(ifo, n, rows);
+    for (int v : helm) require(v >= 0 && v < n, "dfmElmVtxMat holds a vertex index outside [0, ndeNum)");
+    for (int k = 0; k < n; ++k) {
+        int adj = hifo[k];
+        require(adj >= 0 && 1 + (long long)adj * vpe <= rows, "dfmElmIfoMat count exceeds its row count");
+        for (int a = 0; a < adj; ++a) {
+            int e = hifo[(size_t)(1 + a * vpe) * n + k];
+            require(e >= 0 && e < ne, "dfmElmIfoMat holds an element index outside [0, dfmElmNum)");
+            if (vpe == 3)
+                for (int z = 1; z <= 2; ++z) {
+                    int v = hifo[(size_t)(1 + a * vpe + z) * n + k];
+                    require(v >= 0 && v < n, "dfmElmIfoMat holds a vertex index outside [0, ndeNum)");
+                }
+        }
+    }
+    u.q0.alloc(hq.size());
+    u.q0.upload(hq.data(), st);
+    u.elm.alloc(helm.size());
+    u.elm.upload(helm.data(), st);
+    u.ifo.alloc(hifo.size());
+    u.ifo.upload(hifo.data(), st);
+    if (nt > 0) {
+        auto htc = to_soa(tc, nt, dim), htu = to_soa(tu, nt, dim);
+        u.tc.alloc(htc.size());
+        u.tc.upload(htc.data(), st);
+        u.tu.alloc(htu.size());
+        u.tu.upload(htu.data(), st);
+        u.ta.alloc(nt);
+        u.ta.upload(ta, st);
+    }
+    LDDMM_HIP(hipStreamSynchronize(st));   // host vectors go out of scope
+    u.p.q0 = u.q0.p;
+    u.p.elm = u.elm.p;
+    u.p.ifo = u.ifo.p;
+    u.p.tc = u.tc.p;
+    u.p.tu = u.tu.p;
+    u.p.ta = u.ta.p;
+}
+
+LbfgsOptions make_opts(int itrMax, double tol, double w1, double w2, int mem, int vbs, const lddmm_options *o)
+{
+    LbfgsOptions lo;
+    lo.itr_max = itrMax;
+    lo.tol = tol;
+    lo.wolfe1 = w1;
+    lo.wolfe2 = w2;
+    lo.mem = mem;
+    lo.verbose = vbs != 0;
+    if (o) {
+        require(o->precision == LDDMM_PRECISION_FP64, "only LDDMM_PRECISION_FP64 is available");
+        lo.rel_tol = o->rel_energy_tol;
+    }
+    return lo;
+}
+
+void fill_result(lddmm_result *r, const LbfgsResult &lr, const Objective &obj)
+{
+    if (!r) return;
+    std::memset(r, 0, sizeof(*r));
+    r->fcn = lr.f;
+    r->grad_norm = lr.gnorm;
+    r->seconds = lr.seconds;
+    r->pairs_per_eval = obj.pairs_per_eval();
+    r->iterations = lr.iters;
+    r->evaluations = lr.evals;
+    r->status = lr.status;
+}
+
+int run_host(int dim, const double *iniNdeMat, const double *iniAlpVec, const int32_t *elm, const int32_t *ifo,
+             const double *tc, const double *ta, const double *tu, char ct, double cw, char dt, double dw,
+             double w, int order, double width, double h, int T, int itrMax, double tol, double w1, double w2,
+             int mem, int vbs, int n, int ne, int rows, int nt, double *optAlpVec, const lddmm_options *opts,
+             lddmm_result *res)
+{
+    check_common(dim, n, ne, rows, nt, T);
+    require(iniAlpVec && optAlpVec, "null momentum array");
+    LbfgsOptions lo = make_opts(itrMax, tol, w1, w2, mem, vbs, opts);
+    Stream st;
+    UploadedProblem u;
+    fill_params(u.p, dim, n, T, order, width, h, w, ct, cw, dt, dw, ne, nt, rows);
+    validate(u.p);
+    upload_host_problem(u, dim, iniNdeMat, elm, ifo, tc, ta, tu, n, ne, rows, nt, st.s);
+    Objective obj(u.p, st.s, opts ? opts->pair_target_blocks : 0);
+    auto hx = to_soa(iniAlpVec, n, dim, T - 1);
+    DevBuf<double> x(hx.size());
+    x.upload(hx.data(), st.s);
+    LbfgsResult lr = run_lbfgs(obj, x.p, lo, st.s);
+    x.download(hx.data(), st.s);
+    st.sync();
+    from_soa(optAlpVec, hx.data(), n, dim, T - 1);
+    fill_result(res, lr, obj);
+    return lr.status;
+}
+
+}  // namespace
+
+struct lddmm_objective {
+    Stream st;
+    UploadedProblem u;
+    std::unique_ptr<Objective> obj;
+    DevBuf<double> alp, grd, f;
+};
+
+extern "C" {
+
+const char *lddmm_last_error(void) { return g_err.c_str(); }
+
+int lddmm_run_ex(int dimNum, const double *iniNdeMat, const double *iniAlpVec, const int32_t *dfmElmVtxMat,
+                 const int32_t *dfmElmIfoMat, const double *tgtCenPosMat, const double *tgtElmVolVec,
+                 const double *tgtUniDirMat, char cenKnlType, double cenKnlWidth, char dirKnlType,
+                 double dirKnlWidth, double disWgt, int knlOrder, double knlWidth, double timeStp, int timeNum,
+                 int itrMax, double tolVal, double wolfe1, double wolfe2, int memNum, int vbsFlg, int ndeNum,
+                 int dfmElmNum, int dfmElmIfoNum, int tgtElmNum, double *optAlpVec, const lddmm_options *opts,
+                 lddmm_result *result)
+{
+    return guarded([&] {
+        return run_host(dimNum, iniNdeMat, iniAlpVec, dfmElmVtxMat, dfmElmIfoMat, tgtCenPosMat, tgtElmVolVec,
+                        tgtUniDirMat, cenKnlType, cenKnlWidth, dirKnlType, dirKnlWidth, disWgt, knlOrder, knlWidth,
+                        timeStp, timeNum, itrMax, tolVal, wolfe1, wolfe2, memNum, vbsFlg, ndeNum, dfmElmNum,
+                        dfmElmIfoNum, tgtElmNum, optAlpVec, opts, result);
+    });
+}
+
+int lddmm_run_2d(const double *iniNdeMat, const double *iniAlpVec, const int32_t *dfmElmVtxMat,
+                 const int32_t *dfmElmIfoMat, const double *tgtCenPosMat, const double *tgtElmVolVec,
+                 const double *tgtUniDirMat, char cenKnlType, double cenKnlWidth, char dirKnlType,
+                 double dirKnlWidth, double disWgt, int knlOrder, double knlWidth, double timeStp, int timeNum,
+                 int itrMax, double tolVal, double wolfe1, double wolfe2, int memNum, int vbsFlg, int ndeNum,
+                 int dfmElmNum, int dfmElmIfoNum, int tgtElmNum, double *optAlpVec)
+{
+    return lddmm_run_ex(2, iniNdeMat, iniAlpVec, dfmElmVtxMat, dfmElmIfoMat, tgtCenPosMat, tgtElmVolVec,
+                        tgtUniDirMat, cenKnlType, cenKnlWidth, dirKnlType, dirKnlWidth, disWgt, knlOrder, knlWidth,
+                        timeStp, timeNum, itrMax, tolVal, wolfe1, wolfe2, memNum, vbsFlg, ndeNum, dfmElmNum,
+                        dfmElmIfoNum, tgtElmNum, optAlpVec, nullptr, nullptr);
+}
+
+int lddmm_run_3d(const double *iniNdeMat, const double *iniAlpVec, const int32_t *dfmElmVtxMat,
+                 const int32_t *dfmElmIfoMat, const double *tgtCenPosMat, const double *tgtElmVolVec,
+                 const double *tgtUniDirMat, char cenKnlType, double cenKnlWidth, char dirKnlType,
+                 double dirKnlWidth, double disWgt, int knlOrder, double knlWidth, double timeStp, int timeNum,
+                 int itrMax, double tolVal, double wolfe1, double wolfe2, int memNum, int vbsFlg, int ndeNum,
+                 int dfmElmNum, int dfmElmIfoNum, int tgtElmNum, double *optAlpVec)
+{
+    return lddmm_run_ex(3, iniNdeMat, iniAlpVec, dfmElmVtxMat, dfmElmIfoMat, tgtCenPosMat, tgtElmVolVec,
+                        tgtUniDirMat, cenKnlType, cenKnlWidth, dirKnlType, dirKnlWidth, disWgt, knlOrder, knlWidth,
+                        timeStp, timeNum, itrMax, tolVal, wolfe1, wolfe2, memNum, vbsFlg, ndeNum, dfmElmNum,
+                        dfmElmIfoNum, tgtElmNum, optAlpVec, nullptr, nullptr);
+}
+
+int lddmm_run_dev(int dimNum, const double *d_iniNdeMat, const double *d_iniAlpVec, const int32_t *d_elm,
+                  const int32_t *d_ifo, const double *d_tc, const double *d_ta, const double *d_tu, char cenKnlType,
+                  double cenKnlWidth, char dirKnlType, double dirKnlWidth, double disWgt, int knlOrder,
+                  double knlWidth, double timeStp, int timeNum, int itrMax, double tolVal, double wolfe1,
+                  double wolfe2, int memNum, int vbsFlg, int ndeNum, int dfmElmNum, int dfmElmIfoNum,
+                  int tgtElmNum, double *d_optAlpVec, const lddmm_options *opts, lddmm_result *result,
+                  void *stream)
+{
+    return guarded([&] {
+        check_common(dimNum, ndeNum, dfmElmNum, dfmElmIfoNum, tgtElmNum, timeNum);
+        require(d_iniNdeMat && d_iniAlpVec && d_elm && d_ifo && d_optAlpVec, "null device array");
+        require(tgtElmNum == 0 || (d_tc && d_ta && d_tu), "null target array");
+        hipStream_t st = (hipStream_t)stream;
+        LbfgsOptions lo = make_opts(itrMax, tolVal, wolfe1, wolfe2, memNum, vbsFlg, opts);
+        DeviceProblem p;
+        fill_params(p, dimNum, ndeNum, timeNum, knlOrder, knlWidth, timeStp, disWgt, cenKnlType, cenKnlWidth,
+                    dirKnlType, dirKnlWidth, dfmElmNum, tgtElmNum, dfmElmIfoNum);
+        p.q0 = d_iniNdeMat;
+        p.elm = d_elm;
+        p.ifo = d_ifo;
+        p.tc = d_tc;
+        p.ta = d_ta;
+        p.tu = d_tu;
+        Objective obj(p, st, opts ? opts->pair_target_blocks : 0);
+        LDDMM_HIP(hipMemcpyAsync(d_optAlpVec, d_iniAlpVec, sizeof(double) * p.varnum(), hipMemcpyDeviceToDevice,
+                                 st));
+        LbfgsResult lr = run_lbfgs(obj, d_optAlpVec, lo, st);
+        fill_result(result, lr, obj);
+        return lr.status;
+    });
+}
+
+// ---------------------------------------------------------------- objective handle
+int lddmm_objective_create(lddmm_objective **out, int dimNum, const double *iniNdeMat, const int32_t *dfmElmVtxMat,
+                           const int32_t *dfmElmIfoMat, const double *tgtCenPosMat, const double *tgtElmVolVec,
+                           const double *tgtUniDirMat, char cenKnlType, double cenKnlWidth, char dirKnlType,
+                           double dirKnlWidth, double disWgt, int knlOrder, double knlWidth, double timeStp,
+                           int timeNum, int ndeNum, int dfmElmNum, int dfmElmIfoNum, int tgtElmNum)
+{
+    return guarded([&] {
+        require(out != nullptr, "null handle pointer");
+        *out = nullptr;
+        check_common(dimNum, ndeNum, dfmElmNum, dfmElmIfoNum, tgtElmNum, timeNum);
+        auto h = std::make_unique<lddmm_objective>();
+        fill_params(h->u.p, dimNum, ndeNum, timeNum, knlOrder, knlWidth, timeStp, disWgt, cenKnlType, cenKnlWidth,
+                    dirKnlType, dirKnlWidth, dfmElmNum, tgtElmNum, dfmElmIfoNum);
+        validate(h->u.p);
+        upload_host_problem(h->u, dimNum, iniNdeMat, dfmElmVtxMat, dfmElmIfoMat, tgtCenPosMat, tgtElmVolVec,
+                            tgtUniDirMat, ndeNum, dfmElmNum, dfmElmIfoNum, tgtElmNum, h->st.s);
+        h->obj = std::make_unique<Objective>(h->u.p, h->st.s);
+        h->alp.alloc(h->u.p.varnum());
+        h->grd.alloc(h->u.p.varnum());
+        h->f.alloc(1);
+        *out = h.release();
+        return 0;
+    });
+}
+
+int lddmm_objgrd(lddmm_objective *obj, double *fcnVal, double *grdStk, const double *alpStk)
+{
+    return guarded([&] {
+        require(obj && fcnVal && grdStk && alpStk, "null argument");
+        const DeviceProblem &p = obj->u.p;
+        auto ha = to_soa(alpStk, p.n, p.dim, p.T - 1);
+        obj->alp.upload(ha.data(), obj->st.s);
+        obj->obj->eval(obj->alp.p, obj->grd.p, obj->f.p);
+        obj->grd.download(ha.data(), obj->st.s);
+        obj->f.download(fcnVal, obj->st.s);
+        obj->st.sync();
+        from_soa(grdStk, ha.data(), p.n, p.dim, p.T - 1);
+        return 0;
+    });
+}
+
+int lddmm_objgrd_dev(lddmm_objective *obj, double *d_fcnVal, double *d_grdStk, const double *d_alpStk)
+{
+    return guarded([&] {
+        require(obj && d_fcnVal && d_grdStk && d_alpStk, "null argument");
+        obj->obj->eval(d_alpStk, d_grdStk, d_fcnVal);
+        return 0;
+    });
+}
+
+int lddmm_objective_synchronize(lddmm_objective *obj)
+{
+    return guarded([&] {
+        require(obj != nullptr, "null argument");
+        obj->st.sync();
+        return 0;
+    });
+}
+
+int lddmm_objective_pairs(lddmm_objective *obj, double *pairsPerEval)
+{
+    return guarded([&] {
+        require(obj && pairsPerEval, "null argument");
+        *pairsPerEval = obj->obj->pairs_per_eval();
+        return 0;
+    });
+}
+
+int lddmm_objective_trajectory(lddmm_objective *obj, double *ndeStk)
+{
+    return guarded([&] {
+        require(obj && ndeStk, "null argument");
+        const DeviceProblem &p = obj->u.p;
+        std::vector<double> h((size_t)p.dim * p.n * p.T);
+        LDDMM_HIP(hipMemcpyAsync(h.data(), obj->obj->trajectory(), h.size() * sizeof(double),
+                                 hipMemcpyDeviceToHost, obj->st.s));
+        obj->st.sync();
+        from_soa(ndeStk, h.data(), p.n, p.dim, p.T);
+        return 0;
+    });
+}
+
+int lddmm_objective_target_term(lddmm_objective *obj, double *value)
+{
+    return guarded([&] {
+        require(obj && value, "null argument");
+        *value = obj->obj->target_target();
+        return 0;
+    });
+}
+
+void lddmm_objective_destroy(lddmm_objective *obj) { delete obj; }
+
+// ---------------------------------------------------------------- operators
+int lddmm_multiply_kernel_cross(double *vlcMat, const double *lmkiMat, const double *lmkjMat, const double *alpMat,
+                                int knlOrder, double knlWidth, int lmkiNum, int lmkjNum, int dimNum)
+{
+    return guarded([&] {
+        require(vlcMat && lmkiMat && lmkjMat && alpMat, "null argument");
+        require(dimNum == 2 || dimNum == 3, "dimNum must be 2 or 3");
+        require(lmkiNum >= 0 && lmkjNum >= 0, "negative size");
+        require(knlWidth > 0.0, "knlWidth must be positive");
+        if (lmkiNum == 0) return 0;
+        if (lmkjNum == 0) {
+            std::memset(vlcMat, 0, sizeof(double) * dimNum * lmkiNum);
+            return 0;
+        }
+        Stream st;
+        auto hqi = to_soa(lmkiMat, lmkiNum, dimNum), hqj = to_soa(lmkjMat, lmkjNum, dimNum);
+        auto ha = to_soa(alpMat, lmkjNum, dimNum);
+        DevBuf<double> qi(hqi.size()), qj(hqj.size()), a(ha.size()), v(hqi.size());
+        qi.upload(hqi.data(), st.s);
+        qj.upload(hqj.data(), st.s);
+        a.upload(ha.data(), st.s);
+        PairSplit sp = make_split(lmkiNum, lmkjNum, 2048);
+        DevBuf<double> part((size_t)sp.splits * dimNum * lmkiNum);
+        {
+            ProfScope ps(kProfOperator, st.s);
+            launch_kv(dimNum, knlOrder, qi.p, lmkiNum, qj.p, a.p, lmkjNum, knlWidth, sp, part.p, st.s);
+            launch_sum_splits(dimNum, sp.splits, lmkiNum, part.p, 1.0, v.p, st.s);
+        }
+        v.download(hqi.data(), st.s);
+        st.sync();
+        from_soa(vlcMat, hqi.data(), lmkiNum, dimNum);
+        return 0;
+    });
+}
+
+int lddmm_multiply_kernel(double *vlcMat, const double *lmkMat, const double *alpMat, int knlOrder, double knlWidth,
+                          int lmkNum, int dimNum)
+{
+    return lddmm_multiply_kernel_cross(vlcMat, lmkMat, lmkMat, alpMat, knlOrder, knlWidth, lmkNum, lmkNum, dimNum);
+}
+
+int lddmm_dq_kernel(double *dqKMat, const double *lmkMat, const double *lftMat, const double *rgtMat, int knlOrder,
+                    double knlWidth, int lmkNum, int dimNum)
+{
+    return guarded([&] {
+        require(dqKMat && lmkMat && lftMat && rgtMat, "null argument");
+        require(dimNum == 2 || dimNum == 3, "dimNum must be 2 or 3");
+        require(lmkNum >= 0, "negative size");
+        require(knlWidth > 0.0, "knlWidth must be positive");
+        if (lmkNum == 0) return 0;
+        Stream st;
+        auto hq = to_soa(lmkMat, lmkNum, dimNum), hl = to_soa(lftMat, lmkNum, dimNum),
+             hr = to_soa(rgtMat, lmkNum, dimNum);
+        DevBuf<double> q(hq.size()), l(hl.size()), r(hr.size()), out(hq.size());
+        q.upload(hq.data(), st.s);
+        l.upload(hl.data(), st.s);
+        r.upload(hr.data(), st.s);
+        PairSplit sp = make_split(lmkNum, lmkNum, 2048);
+        DevBuf<double> part((size_t)sp.splits * dimNum * lmkNum);
+        {
+            ProfScope ps(kProfOperator, st.s);
+            launch_dq_self(dimNum, knlOrder, q.p, l.p, r.p, lmkNum, knlWidth, sp, part.p, st.s);
+            launch_sum_splits(dimNum, sp.splits, lmkNum, part.p, 1.0, out.p, st.s);
+        }
+        out.download(hq.data(), st.s);
+        st.sync();
+        from_soa(dqKMat, hq.data(), lmkNum, dimNum);
+        return 0;
+    });
+}
+
+int lddmm_dq_kernel_cross(double *dqiKMat, double *dqjKMat, const double *lmkiMat, const double *lmkjMat,
+                          const double *lftMat, const double *rgtMat, int knlOrder, double knlWidth, int lmkiNum,
+                          int lmkjNum, int dimNum)
+{
+    return guarded([&] {
+        require(dqiKMat && dqjKMat && lmkiMat && lmkjMat && lftMat && rgtMat, "null argument");
+        require(dimNum == 2 || dimNum == 3, "dimNum must be 2 or 3");
+        require(lmkiNum >= 0 && lmkjNum >= 0, "negative size");
+        require(knlWidth > 0.0, "knlWidth must be positive");
+        if (lmkiNum == 0 || lmkjNum == 0) {
+            std::memset(dqiKMat, 0, sizeof(double) * dimNum * lmkiNum);
+            std::memset(dqjKMat, 0, sizeof(double) * dimNum * lmkjNum);
+            return 0;
+        }
+        Stream st;
+        auto hqi = to_soa(lmkiMat, lmkiNum, dimNum), hqj = to_soa(lmkjMat, lmkjNum, dimNum);
+        auto hl = to_soa(lftMat, lmkiNum, dimNum), hr = to_soa(rgtMat, lmkjNum, dimNum);
+        DevBuf<double> qi(hqi.size()), qj(hqj.size()), l(hl.size()), r(hr.size());
+        DevBuf<double> oi(hqi.size()), oj(hqj.size());
+        qi.upload(hqi.data(), st.s);
+        qj.upload(hqj.data(), st.s);
+        l.upload(hl.data(), st.s);
+        r.upload(hr.data(), st.s);
+        PairSplit spi = make_split(lmkiNum, lmkjNum, 2048), spj = make_split(lmkjNum, lmkiNum, 2048);
+        DevBuf<double> part(std::max((size_t)spi.splits * dimNum * lmkiNum, (size_t)spj.splits * dimNum * lmkjNum));
+        {
+            ProfScope ps(kProfOperator, st.s);
+            launch_dq_cross(dimNum, knlOrder, qi.p, l.p, lmkiNum, qj.p, r.p, lmkjNum, knlWidth, spi, part.p, st.s);
+            launch_sum_splits(dimNum, spi.splits, lmkiNum, part.p, 1.0, oi.p, st.s);
+            launch_dq_cross(dimNum, knlOrder, qj.p, r.p, lmkjNum, qi.p, l.p, lmkiNum, knlWidth, spj, part.p, st.s);
+            launch_sum_splits(dimNum, spj.splits, lmkjNum, part.p, 1.0, oj.p, st.s);
+        }
+        oi.download(hqi.data(), st.s);
+        oj.download(hqj.data(), st.s);
+        st.sync();
+        from_soa(dqiKMat, hqi.data(), lmkiNum, dimNum);
+        from_soa(dqjKMat, hqj.data(), lmkjNum, dimNum);
+        return 0;
+    });
+}
+
+int lddmm_compute_kernel_cross(double *knlMat, const double *lmkiMat, const double *lmkjMat, int knlOrder,
+                               double knlWidth, int lmkiNum, int lmkjNum, int dimNum)
+{
+    return guarded([&] {
+        require(knlMat && lmkiMat && lmkjMat, "null argument");
+        require(dimNum == 2 || dimNum == 3, "dimNum must be 2 or 3");
+        require(lmkiNum >= 0 && lmkjNum >= 0, "negative size");
+        require(knlWidth > 0.0, "knlWidth must be positive");
+        if (lmkiNum == 0 || lmkjNum == 0) return 0;
+        Stream st;
+        auto hqi = to_soa(lmkiMat, lmkiNum, dimNum), hqj = to_soa(lmkjMat, lmkjNum, dimNum);
+        DevBuf<double> qi(hqi.size()), qj(hqj.size()), K((size_t)lmkiNum * lmkjNum);
+        qi.upload(hqi.data(), st.s);
+        qj.upload(hqj.data(), st.s);
+        {
+            ProfScope ps(kProfOperator, st.s);
+            launch_dense_kernel(dimNum, knlOrder, qi.p, lmkiNum, qj.p, lmkjNum, knlWidth, K.p, st.s);
+        }
+        K.download(knlMat, st.s);
+        st.sync();
+        return 0;
+    });
+}
+
+int lddmm_compute_kernel(double *knlMat, const double *lmkMat, int knlOrder, double knlWidth, int lmkNum,
+                         int dimNum)
+{
+    // symmetric: the cross layout K[j*N + i] equals the self layout K[i*N + j]
+    return lddmm_compute_kernel_cross(knlMat, lmkMat, lmkMat, knlOrder, knlWidth, lmkNum, lmkNum, dimNum);
+}
+
+int lddmm_varifold(double *vfdVal, double *dqVfdMat, const double *dfmLmkPosMat, const int32_t *dfmElmVtxMat,
+                   const int32_t *dfmElmIfoMat, const double *tgtCenPosMat, const double *tgtUniDirMat,
+                   const double *tgtElmVolVec, char cenKnlType, double cenKnlWidth, char dirKnlType,
+                   double dirKnlWidth, int dfmLmkNum, int dfmElmNum, int dfmElmIfoNum, int tgtElmNum, int dimNum)
+{
+    return guarded([&] {
+        require(vfdVal != nullptr, "null argument");
+        check_common(dimNum, dfmLmkNum, dfmElmNum, dfmElmIfoNum, tgtElmNum, 2);
+        Stream st;
+        UploadedProblem u;
+        fill_params(u.p, dimNum, dfmLmkNum, 2, -1, 1.0, 0.0, 1.0, cenKnlType, cenKnlWidth, dirKnlType, dirKnlWidth,
+                    dfmElmNum, tgtElmNum, dfmElmIfoNum);
+        validate(u.p);
+        upload_host_problem(u, dimNum, dfmLmkPosMat, dfmElmVtxMat, dfmElmIfoMat, tgtCenPosMat, tgtElmVolVec,
+                            tgtUniDirMat, dfmLmkNum, dfmElmNum, dfmElmIfoNum, tgtElmNum, st.s);
+        const int D = dimNum, ne = dfmElmNum, nt = tgtElmNum, nq = dfmLmkNum;
+        PairSplit spd = make_split(ne, ne, 1024), spt = make_split(ne, std::max(nt, 1), 1024);
+        PairSplit sptt = make_split(std::max(nt, 1), std::max(nt, 1), 2048);
+        DevBuf<double> cen((size_t)D * ne), dir((size_t)D * ne), vol(ne), dc((size_t)D * ne), dd((size_t)D * ne);
+        DevBuf<double> part(std::max((size_t)(spd.splits + spt.splits) * vf_comps(D) * ne,
+                                     (size_t)sptt.splits * std::max(nt, 1)));
+        DevBuf<double> ep(std::max(vf_finalize_blocks(ne), ceil_div(std::max(nt, 1), kBlock)));
+        DevBuf<double> scratch(kDotBlocks), vals(2), dq((size_t)D * nq);
+        {
+            ProfScope ps(kProfOperator, st.s);
+            launch_elm_geometry(D, u.p.q0, nq, u.p.elm, ne, cen.p, dir.p, vol.p, st.s);
+            launch_vf_pairs(D, u.p.vp, cen.p, dir.p, vol.p, ne, u.p.tc, u.p.tu, u.p.ta, nt, spd, spt, part.p, st.s);
+            launch_vf_finalize(D, u.p.vp, spd.splits + (nt > 0 ? spt.splits : 0), ne, part.p, dir.p, vol.p, dc.p,
+                               dd.p, ep.p, st.s);
+            launch_sum(ep.p, vf_finalize_blocks(ne), scratch.p, vals.p, st.s);
+            if (dqVfdMat) launch_vf_gather(D, u.p.q0, nq, u.p.ifo, dfmElmIfoNum, dc.p, dd.p, ne, 1.0, dq.p, st.s);
+            if (nt > 0) {
+                launch_vf_value_pairs(D, u.p.vp, u.p.tc, u.p.tu, u.p.ta, nt, sptt, part.p, st.s);
+                launch_vf_value_finalize(sptt.splits, nt, part.p, u.p.ta, ep.p, st.s);
+                launch_sum(ep.p, ceil_div(nt, kBlock), scratch.p, vals.p + 1, st.s);
+            } else {
+                LDDMM_HIP(hipMemsetAsync(vals.p + 1, 0, sizeof(double), st.s));
+            }
+        }
+        double hv[2];
+        vals.download(hv, st.s);
+        std::vector<double> hdq;
+        if (dqVfdMat) {
+            hdq.resize((size_t)D * nq);
+            dq.download(hdq.data(), st.s);
+        }
+        st.sync();
+        *vfdVal = hv[0] + hv[1];
+        if (dqVfdMat) from_soa(dqVfdMat, hdq.data(), nq, D);
+        return 0;
+    });
+}
+
+// ---------------------------------------------------------------- data formats
+int lddmm_get_elm_ifo(int32_t *elmIfoMat, const int32_t *elmVtxMat, int vtxPerElm, int elmNum, int ndeNum)
+{
+    return guarded([&] {
+        require(elmVtxMat != nullptr, "null element table");
+        require(vtxPerElm == 2 || vtxPerElm == 3, "vtxPerElm must be 2 or 3");
+        require(elmNum >= 0 && ndeNum >= 0, "negative size");
+        std::vector<int> cnt(ndeNum, 0);
+        for (long long k = 0; k < (long long)elmNum * vtxPerElm; ++k) {
+            int v = elmVtxMat[k];
+            require(v >= 0 && v < ndeNum, "element table holds a vertex outside [0, ndeNum)");
+            ++cnt[v];
+        }
+        int max_adj = 0;
+        for (int c : cnt) max_adj = std::max(max_adj, c);
+        const int rows = 1 + max_adj * vtxPerElm;
+        if (!elmIfoMat) return rows;
+        std::memset(elmIfoMat, 0, sizeof(int32_t) * (size_t)rows * ndeNum);
+        // elements in ascending order, as MATLAB's find() returns them
+        for (int e = 0; e < elmNum; ++e) {
+            const int32_t *ev = elmVtxMat + (size_t)e * vtxPerElm;
+            for (int k = 0; k < vtxPerElm; ++k) {
+                int32_t *col = elmIfoMat + (size_t)ev[k] * rows;
+                int32_t *slot = col + 1 + (size_t)col[0] * vtxPerElm;
+                ++col[0];
+                slot[0] = e;
+                if (vtxPerElm == 2) {
+                    slot[1] = k == 0 ? -1 : 1;
+                } else {
+                    // (qa, qb) so that dd x (qa - qb) is the normal's derivative
+                    static const int pa[3] = {2, 0, 1}, pb[3] = {1, 2, 0};
+                    slot[1] = ev[pa[k]];
+                    slot[2] = ev[pb[k]];
+                }
+            }
+        }
+        return rows;
+    });
+}
+
+int lddmm_compute_node_position(double *ndeStk, double *vlcStk, const double *iniNdeMat, const double *alpStk,
+                                int knlOrder, double knlWidth, double timeStp, int timeNum, int ndeNum, int dimNum)
+{
+    return guarded([&] {
+        require(iniNdeMat && alpStk, "null argument");
+        require(dimNum == 2 || dimNum == 3, "dimNum must be 2 or 3");
+        require(ndeNum > 0 && timeNum >= 1, "bad sizes");
+        require(knlWidth > 0.0, "knlWidth must be positive");
+        Stream st;
+        const int D = dimNum, n = ndeNum, T = timeNum;
+        const size_t dn = (size_t)D * n;
+        auto hq = to_soa(iniNdeMat, n, D);
+        auto ha = to_soa(alpStk, n, D, std::max(T - 1, 0));
+        DevBuf<double> q(dn * T), a(std::max(ha.size(), (size_t)1)), v(std::max(dn * (T - 1), (size_t)1));
+        LDDMM_HIP(hipMemcpyAsync(q.p, hq.data(), dn * sizeof(double), hipMemcpyHostToDevice, st.s));
+        if (!ha.empty()) LDDMM_HIP(hipMemcpyAsync(a.p, ha.data(), ha.size() * sizeof(double), hipMemcpyHostToDevice, st.s));
+        PairSplit sp = make_split(n, n, 2048);
+        DevBuf<double> part((size_t)sp.splits * D * n), ep(fwd_reduce_blocks(n));
+        for (int t = 0; t < T - 1; ++t) {
+            ProfScope ps(kProfOperator, st.s);
+            const double *qt = q.p + (size_t)t * dn, *at = a.p + (size_t)t * dn;
+            launch_kv(D, knlOrder, qt, n, qt, at, n, knlWidth, sp, part.p, st.s);
+            launch_fwd_reduce(D, sp.splits, n, part.p, qt, at, timeStp, q.p + (size_t)(t + 1) * dn,
+                              v.p + (size_t)t * dn, ep.p, st.s);
+        }
+        std::vector<double> hn(dn * T), hv(dn * std::max(T - 1, 0));
+        q.download(hn.data(), st.s);
+        if (!hv.empty()) LDDMM_HIP(hipMemcpyAsync(hv.data(), v.p, hv.size() * sizeof(double), hipMemcpyDeviceToHost, st.s));
+        st.sync();
+        if (ndeStk) from_soa(ndeStk, hn.data(), n, D, T);
+        if (vlcStk && T > 1) from_soa(vlcStk, hv.data(), n, D, T - 1);
+        return 0;
+    });
+}
+
+// ---------------------------------------------------------------- utilities
+int lddmm_device_count(int *count)
+{
+    return guarded([&] {
+        require(count != nullptr, "null argument");
+        int c = 0;
+        hipError_t e = hipGetDeviceCount(&c);
+        *count = e == hipSuccess ? c : 0;
+        return 0;
+    });
+}
+
+int lddmm_device_malloc(void **ptr, size_t bytes)
+{
+    return guarded([&] {
+        require(ptr != nullptr, "null argument");
+        if (hipMalloc(ptr, bytes) != hipSuccess) throw Error(kErrAlloc, "hipMalloc failed");
+        return 0;
+    });
+}
+
+int lddmm_device_free(void *ptr)
+{
+    return guarded([&] {
+        LDDMM_HIP(hipFree(ptr));
+        return 0;
+    });
+}
+
+int lddmm_memcpy_htod(void *dst, const void *src, size_t bytes)
+{
+    return guarded([&] {
+        LDDMM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+        return 0;
+    });
+}
+
+int lddmm_memcpy_dtoh(void *dst, const void *src, size_t bytes)
+{
+    return guarded([&] {
+        LDDMM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        return 0;
+    });
+}
+
+int lddmm_device_synchronize(void)
+{
+    return guarded([&] {
+        LDDMM_HIP(hipDeviceSynchronize());
+        return 0;
+    });
+}
+
+int lddmm_profile_enable(int on)
+{
+    return guarded([&] {
+        Profiler::get().enable(on != 0);
+        return 0;
+    });
+}
+
+int lddmm_profile_reset(void)
+{
+    return guarded([&] {
+        Profiler::get().reset();
+        return 0;
+    });
+}
+
+int lddmm_profile_read(const char **names, double *ms, long long *counts, int maxEntries)
+{
+    return guarded([&] {
+        double m[kProfCount];
+        long long c[kProfCount];
+        Profiler::get().read(m, c);
+        const int k = std::min(maxEntries, (int)kProfCount);
+        for (int i = 0; i < k; ++i) {
+            if (names) names[i] = prof_name(i);
+            if (ms) ms[i] = m[i];
+            if (counts) counts[i] = c[i];
+        }
+        return k;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,498 @@
+// engine.cpp -- device objective, device-resident L-BFGS driver, profiler.
+#include "engine.hpp"
+
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "vector_ops.hpp"
+
+namespace lddmm {
+
+// ======================================================================
+// Profiler
+// ======================================================================
+const char *prof_name(int id)
+{
+    static const char *names[kProfCount] = {
+        "fwd_pairs",  "fwd_reduce",  "bwd_pairs", "bwd_reduce",   "vf_geometry", "vf_pairs",  "vf_finalize",
+        "vf_gather",  "vf_target_target", "energy", "lbfgs_vector", "lbfgs_dots", "operator",
+    };
+    return (id >= 0 && id < kProfCount) ? names[id] : "?";
+}
+
+Profiler &Profiler::get()
+{
+    static Profiler p;
+    return p;
+}
+
+void Profiler::enable(bool on)
+{
+    std::lock_guard<std::mutex> g(mu_);
+    on_ = on;
+}
+
+hipEvent_t Profiler::take()
+{
+    if (!pool_.empty()) {
+        hipEvent_t e = pool_.back();
+        pool_.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    LDDMM_HIP(hipEventCreate(&e));
+    return e;
+}
+
+void Profiler::record(int id, hipStream_t st, bool start)
+{
+    std::lock_guard<std::mutex> g(mu_);
+    hipEvent_t e = take();
+    LDDMM_HIP(hipEventRecord(e, st));
+    if (start) {
+        open_.push_back({id, e});
+        return;
+    }
+    for (size_t k = open_.size(); k-- > 0;) {
+        if (open_[k].first == id) {
+            pend_.push_back({id, open_[k].second, e});
+            open_.erase(open_.begin() + (long)k);
+            break;
+        }
+    }
+    if (pend_.size() > 4096) collect_locked();
+}
+
+void Profiler::collect_locked()
+{
+    for (auto &p : pend_) {
+        LDDMM_HIP(hipEventSynchronize(p.b));
+        float ms = 0.f;
+        LDDMM_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+        ms_[p.id] += ms;
+        cnt_[p.id] += 1;
+        pool_.push_back(p.a);
+        pool_.push_back(p.b);
+    }
+    pend_.clear();
+}
+
+void Profiler::read(double *ms, long long *count)
+{
+    std::lock_guard<std::mutex> g(mu_);
+    collect_locked();
+    for (int k = 0; k < kProfCount; ++k) {
+        if (ms) ms[k] = ms_[k];
+        if (count) count[k] = cnt_[k];
+    }
+}
+
+void Profiler::reset()
+{
+    std::lock_guard<std::mutex> g(mu_);
+    collect_locked();
+    ms_.fill(0.0);
+    cnt_.fill(0);
+}
+
+// ======================================================================
+// Slab
+// ======================================================================
+Slab::~Slab()
+{
+    if (base_) (void)hipFree(base_);
+}
+
+void Slab::allocate()
+{
+    if (bytes_ == 0) return;
+    if (hipMalloc((void **)&base_, bytes_) != hipSuccess) {
+        base_ = nullptr;
+        throw Error(kErrAlloc, "device allocation of " + std::to_string(bytes_) + " bytes failed");
+    }
+    for (auto &r : reqs_) *r.slot = base_ + r.off;
+}
+
+// ======================================================================
+// Objective
+// ======================================================================
+void validate(const DeviceProblem &p)
+{
+    if (p.dim != 2 && p.dim != 3) throw Error(kErrArg, "only 2D and 3D are supported");
+    if (p.n <= 0) throw Error(kErrArg, "ndeNum must be positive");
+    if (p.T < 2) throw Error(kErrArg, "timeNum must be at least 2");
+    if (p.order < -1 || p.order > 4) throw Error(kErrArg, "knlOrder must be -1 or 0..4");
+    if (p.dim == 2 && p.order >= 0)
+        throw Error(kErrUnsupported, "2D Matern kernels are not implemented on the GPU path");
+    if (!(p.width > 0.0)) throw Error(kErrArg, "knlWidth must be positive");
+    if (p.ne <= 0 || p.nt < 0) throw Error(kErrArg, "element counts must be positive");
+    if (p.ifo_rows < 1) throw Error(kErrArg, "dfmElmIfoNum must be at least 1");
+    if (p.vp.cen_type != 'G' && p.vp.cen_type != 'C') throw Error(kErrArg, "cenKnlType must be 'G' or 'C'");
+    if (p.vp.dir_type != 'B' && p.vp.dir_type != 'L' && p.vp.dir_type != 'O' && p.vp.dir_type != 'U')
+        throw Error(kErrArg, "dirKnlType must be 'B', 'L', 'O' or 'U'");
+    if (!(p.vp.cen_width > 0.0)) throw Error(kErrArg, "cenKnlWidth must be positive");
+    if ((p.vp.dir_type == 'O' || p.vp.dir_type == 'U') && !(p.vp.dir_width > 0.0))
+        throw Error(kErrArg, "dirKnlWidth must be positive for 'O' and 'U'");
+    if ((long long)p.dim * p.n * (p.T - 1) >= (1LL << 40)) throw Error(kErrArg, "problem too large");
+}
+
+static int default_target_blocks()
+{
+    // ~8 resident workgroups of 256 threads per CU on the 256-CU chip
+    return 2048;
+}
+
+Objective::Objective(const DeviceProblem &p, hipStream_t st, int target_blocks) : p_(p), st_(st)
+{
+    validate(p_);
+    const int tb = target_blocks > 0 ? target_blocks : default_target_blocks();
+    const int D = p_.dim;
+    sp_pts_ = make_split(p_.n, p_.n, tb);
+    const double fd = (double)p_.ne / (double)(p_.ne + p_.nt);
+    int tbd = (int)std::max(1.0, tb * fd), tbt = std::max(1, tb - tbd);
+    sp_vfd_ = make_split(p_.ne, p_.ne, tbd);
+    sp_vft_ = make_split(p_.ne, std::max(p_.nt, 1), tbt);
+    PairSplit sp_tt = make_split(std::max(p_.nt, 1), std::max(p_.nt, 1), tb);
+
+    size_t part = (size_t)sp_pts_.splits * 2 * D * p_.n;
+    part = std::max(part, (size_t)(sp_vfd_.splits + sp_vft_.splits) * vf_comps(D) * p_.ne);
+    part = std::max(part, (size_t)sp_tt.splits * std::max(p_.nt, 1));
+    nb_fwd_ = fwd_reduce_blocks(p_.n);
+    nb_vf_ = vf_finalize_blocks(p_.ne);
+    const int nb_tt = ceil_div(std::max(p_.nt, 1), kBlock);
+
+    slab_.reserve(&nde_, (size_t)D * p_.n * p_.T);
+    slab_.reserve(&p_adj_, (size_t)D * p_.n);
+    slab_.reserve(&part_, part);
+    slab_.reserve(&ecen_, (size_t)D * p_.ne);
+    slab_.reserve(&edir_, (size_t)D * p_.ne);
+    slab_.reserve(&evol_, (size_t)p_.ne);
+    slab_.reserve(&dc_, (size_t)D * p_.ne);
+    slab_.reserve(&dd_, (size_t)D * p_.ne);
+    slab_.reserve(&ep_fwd_, (size_t)(p_.T - 1) * nb_fwd_);
+    slab_.reserve(&ep_vf_, (size_t)std::max(nb_vf_, nb_tt));
+    slab_.reserve(&tt_, 1);
+    slab_.reserve(&scratch_, (size_t)kDotBlocks);
+    slab_.allocate();
+    compute_target_target(tb);
+}
+
+Objective::~Objective() = default;
+
+void Objective::compute_target_target(int target_blocks)
+{
+    if (p_.nt <= 0) {
+        LDDMM_HIP(hipMemsetAsync(tt_, 0, sizeof(double), st_));
+        LDDMM_HIP(hipStreamSynchronize(st_));
+        tt_host_ = 0.0;
+        return;
+    }
+    PairSplit sp = make_split(p_.nt, p_.nt, target_blocks);
+    {
+        ProfScope ps(kProfVfTargetTarget, st_);
+        launch_vf_value_pairs(p_.dim, p_.vp, p_.tc, p_.tu, p_.ta, p_.nt, sp, part_, st_);
+        launch_vf_value_finalize(sp.splits, p_.nt, part_, p_.ta, ep_vf_, st_);
+        launch_sum(ep_vf_, ceil_div(p_.nt, kBlock), scratch_, tt_, st_);
+    }
+    LDDMM_HIP(hipMemcpyAsync(&tt_host_, tt_, sizeof(double), hipMemcpyDeviceToHost, st_));
+    LDDMM_HIP(hipStreamSynchronize(st_));
+}
+
+double Objective::pairs_per_eval() const
+{
+    const double n = p_.n, T1 = p_.T - 1;
+    return 2.0 * T1 * n * n + (double)p_.ne * p_.ne + (double)p_.ne * p_.nt;
+}
+
+void Objective::eval(const double *alp, double *grd, double *f_dev)
+{
+    const int D = p_.dim, n = p_.n, T = p_.T;
+    const size_t dn = (size_t)D * n;
+    const int S = sp_pts_.splits;
+
+    // forward flow q_{t+1} = q_t + h K(q_t) alpha_t, energy partials <alpha_t, v_t>
+    // (Cpp/src/objgrd.cpp:41-59)
+    LDDMM_HIP(hipMemcpyAsync(nde_, p_.q0, dn * sizeof(double), hipMemcpyDeviceToDevice, st_));
+    for (int t = 0; t < T - 1; ++t) {
+        const double *qt = nde_ + (size_t)t * dn;
+        const double *at = alp + (size_t)t * dn;
+        {
+            ProfScope ps(kProfFwdPairs, st_);
+            launch_kv(D, p_.order, qt, n, qt, at, n, p_.width, sp_pts_, part_, st_);
+        }
+        {
+            ProfScope ps(kProfFwdReduce, st_);
+            launch_fwd_reduce(D, S, n, part_, qt, at, p_.h, nde_ + (size_t)(t + 1) * dn, nullptr,
+                              ep_fwd_ + (size_t)t * nb_fwd_, st_);
+        }
+    }
+
+    // varifold at q_{T-1}; adjoint start p = -w grad V (Cpp/src/objgrd.cpp:61-67)
+    const double *qT = nde_ + (size_t)(T - 1) * dn;
+    {
+        ProfScope ps(kProfVfGeometry, st_);
+        launch_elm_geometry(D, qT, n, p_.elm, p_.ne, ecen_, edir_, evol_, st_);
+    }
+    {
+        ProfScope ps(kProfVfPairs, st_);
+        launch_vf_pairs(D, p_.vp, ecen_, edir_, evol_, p_.ne, p_.tc, p_.tu, p_.ta, p_.nt, sp_vfd_, sp_vft_, part_,
+                        st_);
+    }
+    {
+        ProfScope ps(kProfVfFinalize, st_);
+        launch_vf_finalize(D, p_.vp, sp_vfd_.splits + (p_.nt > 0 ? sp_vft_.splits : 0), p_.ne, part_, edir_,
+                           evol_, dc_, dd_, ep_vf_, st_);
+    }
+    {
+        ProfScope ps(kProfVfGather, st_);
+        launch_vf_gather(D, qT, n, p_.ifo, p_.ifo_rows, dc_, dd_, p_.ne, -p_.w, p_adj_, st_);
+    }
+    {
+        ProfScope ps(kProfEnergy, st_);
+        launch_energy(ep_fwd_, (T - 1) * nb_fwd_, ep_vf_, nb_vf_, tt_, p_.h, p_.w, f_dev, st_);
+    }
+
+    // backward adjoint sweep (Cpp/src/objgrd.cpp:69-87)
+    for (int t = T - 2; t >= 0; --t) {
+        const double *qt = nde_ + (size_t)t * dn;
+        const double *at = alp + (size_t)t * dn;
+        {
+            ProfScope ps(kProfBwdPairs, st_);
+            launch_bwd_pairs(D, p_.order, qt, at, p_adj_, n, p_.width, sp_pts_, part_, st_);
+        }
+        {
+            ProfScope ps(kProfBwdReduce, st_);
+            launch_bwd_reduce(D, S, n, part_, p_.h, grd + (size_t)t * dn, p_adj_, st_);
+        }
+    }
+}
+
+// ======================================================================
+// L-BFGS with device-resident vectors and scalars
+// ======================================================================
+namespace {
+
+// device scalar slots
+enum : int { kF = 0, kGG, kGD, kDD, kSD, kYD, kYY, kNumScal };
+
+struct LbfgsState {
+    Slab slab;
+    double *g = nullptr, *d = nullptr, *xt = nullptr, *gt = nullptr, *S = nullptr, *Y = nullptr;
+    double *scal = nullptr, *sy = nullptr, *rec = nullptr, *yy = nullptr, *scratch = nullptr;
+    double *host = nullptr;   // pinned mirror of scal
+    ~LbfgsState()
+    {
+        if (host) (void)hipHostFree(host);
+    }
+};
+
+void dots(LbfgsState &s, int nd, const double *const *xs, const double *const *ys, long long n, double *out,
+          hipStream_t st)
+{
+    ProfScope ps(kProfLbfgsDots, st);
+    launch_dots(nd, xs, ys, n, s.scratch, out, st);
+}
+
+void fetch(LbfgsState &s, int first, int count, hipStream_t st)
+{
+    LDDMM_HIP(hipMemcpyAsync(s.host + first, s.scal + first, sizeof(double) * count, hipMemcpyDeviceToHost, st));
+    LDDMM_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace
+
+LbfgsResult run_lbfgs(Objective &obj, double *x, const LbfgsOptions &o, hipStream_t st)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    if (o.mem < 1) throw Error(kErrArg, "memNum must be at least 1");
+    if (o.itr_max < 0) throw Error(kErrArg, "itrMax must be non-negative");
+    const long long nv = obj.problem().varnum();
+    const int m = o.mem;
+    LbfgsState s;
+    s.slab.reserve(&s.g, nv);
+    s.slab.reserve(&s.d, nv);
+    s.slab.reserve(&s.xt, nv);
+    s.slab.reserve(&s.gt, nv);
+    s.slab.reserve(&s.S, (size_t)nv * m);
+    s.slab.reserve(&s.Y, (size_t)nv * m);
+    s.slab.reserve(&s.scal, kNumScal);
+    s.slab.reserve(&s.sy, m);
+    s.slab.reserve(&s.rec, m);
+    s.slab.reserve(&s.yy, m);
+    s.slab.reserve(&s.scratch, (size_t)kMaxDots * kDotBlocks);
+    s.slab.allocate();
+    LDDMM_HIP(hipHostMalloc((void **)&s.host, sizeof(double) * kNumScal, hipHostMallocDefault));
+
+    LbfgsResult r;
+    auto push_trace = [&](double f, double g, double step, int cnt) {
+        r.trace_f.push_back(f);
+        r.trace_g.push_back(g);
+        r.trace_step.push_back(step);
+        r.trace_evals.push_back(cnt);
+    };
+
+    // f, g at the start (Cpp/src/LBFGS.cpp:50-52)
+    obj.eval(x, s.g, s.scal + kF);
+    {
+        const double *xs[1] = {s.g}, *ys[1] = {s.g};
+        dots(s, 1, xs, ys, nv, s.scal + kGG, st);
+    }
+    fetch(s, kF, 2, st);
+    double fnow = s.host[kF];
+    double glen = std::sqrt(s.host[kGG]);
+    int evals = 1;
+    if (o.verbose) {
+        printf("%5s   %13s  %13s  %13s  %9s\n", "iter", "f", "|grad f|", "step length", "fcn eval");
+        char sep[65] = {0};
+        memset(sep, '-', 62);
+        printf("%s\n", sep);
+        printf("%5d:  %13.6e  %13.6e\n", 0, fnow, glen);
+        fflush(stdout);
+    }
+    push_trace(fnow, glen, 0.0, 1);
+
+    int new_idx = -1, status = 0;
+    for (int it = 1; it <= o.itr_max; ++it) {
+        if (glen < o.tol) break;
+
+        // direction: two-loop recursion (Cpp/src/getDirection.cpp:9-66)
+        const int his = it <= m ? it - 1 : m;
+        {
+            ProfScope ps(kProfLbfgsVector, st);
+            if (his == 0) {
+                vec_copy_neg(s.d, s.g, nv, st);
+            } else {
+                LDDMM_HIP(hipMemcpyAsync(s.d, s.g, sizeof(double) * nv, hipMemcpyDeviceToDevice, st));
+            }
+        }
+        if (his > 0) {
+            for (int h = 0; h < his; ++h) {
+                int idx = new_idx - h;
+                if (idx < 0) idx += m;
+                const double *xs[1] = {s.S + (size_t)idx * nv}, *ys[1] = {s.d};
+                dots(s, 1, xs, ys, nv, s.scal + kSD, st);
+                ProfScope ps(kProfLbfgsVector, st);
+                vec_loop1(s.d, s.Y + (size_t)idx * nv, s.scal + kSD, s.sy + idx, s.rec + idx, nv, st);
+            }
+            {
+                ProfScope ps(kProfLbfgsVector, st);
+                vec_scale_h0(s.d, s.sy + new_idx, s.yy + new_idx, nv, st);
+            }
+            int old = his < m ? 0 : new_idx + 1;
+            if (old == m) old = 0;
+            for (int h = 0; h < his; ++h) {
+                int idx = old + h;
+                if (idx >= m) idx -= m;
+                const double *xs[1] = {s.Y + (size_t)idx * nv}, *ys[1] = {s.d};
+                dots(s, 1, xs, ys, nv, s.scal + kYD, st);
+                ProfScope ps(kProfLbfgsVector, st);
+                vec_loop2(s.d, s.S + (size_t)idx * nv, s.scal + kYD, s.sy + idx, s.rec + idx, h == his - 1, nv,
+                          st);
+            }
+        }
+
+        // weak-Wolfe line search (Cpp/src/lineSearch.cpp:26-110)
+        {
+            const double *xs[2] = {s.g, s.d}, *ys[2] = {s.d, s.d};
+            dots(s, 2, xs, ys, nv, s.scal + kGD, st);
+        }
+        fetch(s, kGD, 2, st);
+        const double slope = s.host[kGD];
+        if (slope > -DBL_MIN) {
+            printf("\nNot a descent direction.\nQuit LBFGS.\n\n");
+            fflush(stdout);
+            status = 1;
+            break;
+        }
+        const double dlen = std::sqrt(s.host[kDD]);
+        const int bis_max = (int)std::fmax(50.0, std::log2(1e5 * dlen));
+        const int epd_max = (int)std::fmax(10.0, std::log2(1e5 / dlen));
+        double lo = 0.0, hi = DBL_MAX, rel = 1.0, fnew = 0.0, gnew = 0.0, step = 0.0;
+        int bis = 0, epd = 0, cnt = 0, le = -1;
+        while (bis <= bis_max && epd <= epd_max) {
+            {
+                ProfScope ps(kProfLbfgsVector, st);
+                vec_try(s.xt, x, rel, s.d, nv, st);
+            }
+            obj.eval(s.xt, s.gt, s.scal + kF);
+            ++cnt;
+            {
+                const double *xs[2] = {s.gt, s.gt}, *ys[2] = {s.gt, s.d};
+                dots(s, 2, xs, ys, nv, s.scal + kGG, st);
+            }
+            fetch(s, kF, 3, st);
+            fnew = s.host[kF];
+            gnew = std::sqrt(s.host[kGG]);
+            const double slope_t = s.host[kGD];
+            if (gnew < o.tol) {
+                step = rel * dlen;
+                le = 0;
+                break;
+            }
+            if (fnew >= fnow + o.wolfe1 * rel * slope) hi = rel;
+            else if (slope_t <= o.wolfe2 * slope) lo = rel;
+            else {
+                step = rel * dlen;
+                le = 0;
+                break;
+            }
+            if (hi == DBL_MAX) {
+                rel *= 2.0;
+                ++epd;
+            } else {
+                rel = 0.5 * (lo + hi);
+                ++bis;
+            }
+        }
+        evals += cnt;
+        if (le != 0) {
+            if (hi == DBL_MAX) {
+                printf("\nLine search failed to bracket a point satisfying weak Wolfe conditions.\n"
+                       "Function may be unbounded below.\nQuit LBFGS.\n\n");
+                status = 2;
+            } else {
+                printf("\nLine search failed to satisfy weak Wolfe conditions, although\n"
+                       "a point satisfying conditions was bracketed in [%e, %e].\nQuit LBFGS.\n\n",
+                       lo, hi);
+                status = 3;
+            }
+            fflush(stdout);
+            break;
+        }
+
+        // accept: curvature pair into the ring buffer (Cpp/src/LBFGS.cpp:94-105)
+        new_idx = (new_idx == m - 1) ? 0 : new_idx + 1;
+        double *sv = s.S + (size_t)new_idx * nv, *yv = s.Y + (size_t)new_idx * nv;
+        {
+            ProfScope ps(kProfLbfgsVector, st);
+            vec_accept(x, s.g, s.xt, s.gt, sv, yv, nv, st);
+        }
+        {
+            const double *xs[2] = {sv, yv}, *ys[2] = {yv, yv};
+            dots(s, 2, xs, ys, nv, s.scal + kSD, st);
+            LDDMM_HIP(hipMemcpyAsync(s.sy + new_idx, s.scal + kSD, sizeof(double), hipMemcpyDeviceToDevice, st));
+            LDDMM_HIP(hipMemcpyAsync(s.yy + new_idx, s.scal + kYD, sizeof(double), hipMemcpyDeviceToDevice, st));
+        }
+        const double fprev = fnow;
+        fnow = fnew;
+        glen = gnew;
+        r.iters = it;
+        if (o.verbose) {
+            printf("%5d:  %13.6e  %13.6e  %13.6e  %9d\n", it, fnow, glen, step, cnt);
+            fflush(stdout);
+        }
+        push_trace(fnow, glen, step, cnt);
+        if (o.rel_tol > 0.0 && (fprev - fnow) < o.rel_tol * std::fabs(fnow)) break;
+    }
+    LDDMM_HIP(hipStreamSynchronize(st));
+    r.f = fnow;
+    r.gnorm = glen;
+    r.evals = evals;
+    r.status = status;
+    r.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return r;
+}
+
+}  // namespace lddmm

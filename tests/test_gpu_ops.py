@@ -1,0 +1,189 @@
+"""GPU parity of every operator and of the objective against the FP64 CPU
+oracle, through the C-ABI (lddmm_amd is a ctypes mirror of it).
+
+Tolerance for the FP64 kernels: 1e-12 relative (max-norm) for the pair
+operators and 1e-10 for objective energy and gradient, the bound the
+design sets for FP64; differences come only from summation order and FMA
+contraction."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_err
+from lddmm_amd import geometry as G
+
+pytestmark = pytest.mark.gpu
+
+TOL_OP = 1e-12
+TOL_OBJ = 1e-10
+
+
+def pts(seed, n, d, scale=0.5):
+    return (scale * G.normal(seed, n * d)).reshape(n, d)
+
+
+CASES = [(2, -1)] + [(3, o) for o in (-1, 0, 1, 2, 3, 4)]
+
+
+@pytest.mark.parametrize("dim,order", CASES)
+@pytest.mark.parametrize("n", [1, 7, 300, 1237])
+def test_multiply_kernel(gpu, dim, order, n):
+    q, a = pts(n + 1, n, dim), pts(n + 2, n, dim, 1.0)
+    w = 0.3
+    print("multiply_kernel dim=%d order=%d n=%d" % (dim, order, n), flush=True)
+    assert rel_err(gpu.multiplyKernel(q, a, order, w), oracle.multiply_kernel(q, a, order, w)) <= TOL_OP
+
+
+@pytest.mark.parametrize("dim,order", CASES)
+@pytest.mark.parametrize("ni,nj", [(5, 1), (300, 77), (64, 2000)])
+def test_multiply_kernel_cross(gpu, dim, order, ni, nj):
+    qi, qj, a = pts(3, ni, dim), pts(4, nj, dim), pts(5, nj, dim, 1.0)
+    print("multiply_kernel_cross dim=%d order=%d %d x %d" % (dim, order, ni, nj), flush=True)
+    assert rel_err(gpu.multiplyKernel(qi, qj, a, order, 0.4),
+                   oracle.multiply_kernel_cross(qi, qj, a, order, 0.4)) <= TOL_OP
+
+
+@pytest.mark.parametrize("dim,order", CASES)
+@pytest.mark.parametrize("n", [2, 300, 1237])
+def test_dq_kernel(gpu, dim, order, n):
+    q, l, r = pts(n + 6, n, dim), pts(n + 7, n, dim, 1.0), pts(n + 8, n, dim, 1.0)
+    print("dq_kernel dim=%d order=%d n=%d" % (dim, order, n), flush=True)
+    got = gpu.dqKernel(q, l, r, order, 0.3)
+    exp = oracle.dq_kernel(q, l, r, order, 0.3)
+    if order == 0:
+        assert not np.any(got)
+    else:
+        assert rel_err(got, exp) <= TOL_OP
+
+
+@pytest.mark.parametrize("dim,order", [(2, -1), (3, -1), (3, 1), (3, 4)])
+def test_dq_kernel_cross(gpu, dim, order):
+    qi, qj = pts(9, 250, dim), pts(10, 333, dim)
+    l, r = pts(11, 250, dim, 1.0), pts(12, 333, dim, 1.0)
+    print("dq_kernel_cross dim=%d order=%d" % (dim, order), flush=True)
+    gi, gj = gpu.dqKernel(qi, qj, l, r, order, 0.35)
+    ei, ej = oracle.dq_kernel_cross(qi, qj, l, r, order, 0.35)
+    assert rel_err(gi, ei) <= TOL_OP and rel_err(gj, ej) <= TOL_OP
+
+
+@pytest.mark.parametrize("dim,order", [(2, -1), (3, -1), (3, 2)])
+def test_compute_kernel(gpu, dim, order):
+    q, qj = pts(13, 90, dim), pts(14, 41, dim)
+    print("compute_kernel dim=%d order=%d" % (dim, order), flush=True)
+    assert rel_err(gpu.computeKernel(q, order, 0.3), oracle.compute_kernel(q, order, 0.3)) <= TOL_OP
+    assert rel_err(gpu.computeKernel(q, qj, order, 0.3), oracle.compute_kernel_cross(q, qj, order, 0.3)) <= TOL_OP
+
+
+def test_golden_kernels_on_gpu(gpu, golden_kernels):
+    """The GPU operators against the reference's own outputs directly."""
+    g = golden_kernels
+    for dim, order in CASES:
+        t = "d%d_o%d" % (dim, order if order >= 0 else 9)
+        q, qj, a, aj, l, w = g[t + "_q"], g[t + "_qj"], g[t + "_a"], g[t + "_aj"], g[t + "_l"], float(g[t + "_w"])
+        assert rel_err(gpu.multiplyKernel(q, a, order, w), g[t + "_mult"]) <= TOL_OP
+        assert rel_err(gpu.multiplyKernel(q, qj, aj, order, w), g[t + "_mult_cross"]) <= TOL_OP
+        if order != 0:
+            assert rel_err(gpu.dqKernel(q, l, a, order, w), g[t + "_dq"]) <= TOL_OP
+
+
+def test_unsupported_2d_matern_fails_loudly(gpu):
+    q = pts(1, 10, 2)
+    with pytest.raises(gpu.LddmmError) as ei:
+        gpu.multiplyKernel(q, q, 1, 0.3)
+    assert ei.value.code == -4
+
+
+@pytest.mark.parametrize("dim", (2, 3))
+@pytest.mark.parametrize("ct", "GC")
+@pytest.mark.parametrize("dt", "BLOU")
+def test_varifold_golden(gpu, golden_varifold, dim, ct, dt):
+    g = golden_varifold
+    d = "d%d" % dim
+    t = "%s_%s%s" % (d, ct, dt)
+    cw, dw = g[t + "_w"]
+    print("varifold", t, flush=True)
+    val, dq = gpu.varifold(g[d + "_q"], g[d + "_elm"], g[d + "_ifo"], g[d + "_tc"], g[d + "_tu"], g[d + "_ta"],
+                           ct, cw, dt, dw)
+    assert abs(val - float(g[t + "_val"])) <= TOL_OP * max(1.0, abs(float(g[t + "_val"])))
+    assert rel_err(dq, g[t + "_dq"]) <= 1e-11
+
+
+def test_varifold_larger_and_without_target(gpu):
+    q, tri = G.icosphere(3)
+    q = q + 0.01 * G.normal(3, q.size).reshape(q.shape)
+    ifo = oracle.get_elm_ifo(tri, q.shape[0])
+    cv, ct = G.cube_surface(12)
+    tc, tu, ta = G.triangle_varifold(cv, ct)
+    print("varifold ico3 vs cube12", flush=True)
+    v, dq = gpu.varifold(q, tri, ifo, tc, tu, ta, "G", 0.15, "B", 0.0)
+    ev, edq = oracle.varifold(q, tri, ifo, tc, tu, ta, "G", 0.15, "B", 0.0)
+    assert abs(v - ev) <= 1e-11 * abs(ev) and rel_err(dq, edq) <= 1e-11
+    empty = np.zeros((0, 3))
+    v, dq = gpu.varifold(q, tri, ifo, empty, empty, np.zeros(0), "C", 0.2, "U", 0.6)
+    ev, edq = oracle.varifold(q, tri, ifo, empty, empty, np.zeros(0), "C", 0.2, "U", 0.6)
+    assert abs(v - ev) <= 1e-11 * abs(ev) and rel_err(dq, edq) <= 1e-11
+
+
+def make_problem(kind):
+    if kind == "circle_square":
+        w = G.circle_square(200)
+        w["q0"] = w["q0"] * 0.95
+    elif kind == "sphere_cube":
+        w = G.sphere_cube(subdivisions=2, cells=10)
+    else:
+        w = G.superquadric_pair(4, 17)
+    w["time_num"] = 6
+    w["time_stp"] = 0.2
+    return w
+
+
+@pytest.mark.parametrize("kind", ["circle_square", "sphere_cube", "superquadric"])
+def test_objgrd_matches_oracle(gpu, kind):
+    w = make_problem(kind)
+    n, d = w["q0"].shape
+    ifo = oracle.get_elm_ifo(w["elm"], n)
+    args = (w["q0"], w["elm"], ifo, w["tgt_cen"], w["tgt_vol"], w["tgt_dir"], w["cen_type"], w["cen_width"],
+            w["dir_type"], w["dir_width"], w["dis_wgt"], w["knl_order"], w["knl_width"], w["time_stp"], w["time_num"])
+    obj = gpu.Objective(*args)
+    P = oracle.OracleProblem(w["q0"], w["elm"], w["tgt_cen"], w["tgt_dir"], w["tgt_vol"], w["cen_type"],
+                             w["cen_width"], w["dir_type"], w["dir_width"], w["dis_wgt"], w["knl_order"],
+                             w["knl_width"], w["time_stp"], w["time_num"], ifo=ifo)
+    print("objgrd", kind, "n=%d" % n, flush=True)
+    for seed, scale in ((0, 0.0), (5, 0.05), (6, 0.3)):
+        a = scale * G.normal(seed + 100, obj.varnum)
+        f, g = obj(a)
+        ef, eg, traj = P.objgrd(a, want_traj=True)
+        assert abs(f - ef) <= TOL_OBJ * abs(ef), (seed, f, ef)
+        assert rel_err(g, eg) <= TOL_OBJ, (seed, rel_err(g, eg))
+        assert rel_err(obj.trajectory(), traj) <= 1e-12
+    tt = oracle.varifold_terms(w["q0"], w["elm"], w["tgt_cen"], w["tgt_dir"], w["tgt_vol"], w["cen_type"],
+                               w["cen_width"], w["dir_type"], w["dir_width"])[2]
+    assert abs(obj.target_term() - tt) <= 1e-12 * abs(tt)
+    obj.close()
+
+
+def test_objgrd_is_deterministic(gpu):
+    w = make_problem("sphere_cube")
+    n = w["q0"].shape[0]
+    ifo = oracle.get_elm_ifo(w["elm"], n)
+    obj = gpu.Objective(w["q0"], w["elm"], ifo, w["tgt_cen"], w["tgt_vol"], w["tgt_dir"], w["cen_type"],
+                        w["cen_width"], w["dir_type"], w["dir_width"], w["dis_wgt"], w["knl_order"], w["knl_width"],
+                        w["time_stp"], w["time_num"])
+    a = 0.1 * G.normal(9, obj.varnum)
+    f1, g1 = obj(a)
+    f2, g2 = obj(a)
+    assert f1 == f2 and np.array_equal(g1, g2)
+
+
+def test_compute_node_position_matches_trajectory(gpu):
+    w = make_problem("circle_square")
+    n, d = w["q0"].shape
+    a = 0.2 * G.normal(44, d * n * (w["time_num"] - 1)).reshape(w["time_num"] - 1, n, d)
+    nde, vlc = gpu.computeNodePosition(w["q0"], a, w["knl_order"], w["knl_width"], w["time_stp"], w["time_num"])
+    q = w["q0"].copy()
+    assert np.array_equal(nde[0], q)
+    for t in range(w["time_num"] - 1):
+        v = oracle.multiply_kernel(q, a[t], w["knl_order"], w["knl_width"])
+        assert rel_err(vlc[t], v) <= 1e-12
+        q = q + w["time_stp"] * v
+        assert rel_err(nde[t + 1], q) <= 1e-12

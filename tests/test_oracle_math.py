@@ -1,0 +1,226 @@
+"""Pins of the CPU oracle by what the mathematics fixes: closed forms,
+finite differences (the discrete adjoint is the exact gradient of the
+discrete objective), invariants, a dense BFGS update and crafted line-search
+objectives.  These cover the parts of the reference that cannot be compiled
+here (Cpp/src/objgrd.cpp, LBFGS.cpp, getDirection.cpp, lineSearch.cpp need
+MATLAB's blas.h)."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_err
+from lddmm_amd import geometry as G
+
+
+def small_2d(n=16, seed=3):
+    q, e = G.circle_curve(n)
+    q = q + 0.05 * G.normal(seed, q.size).reshape(q.shape)
+    tc, tu, ta = G.polyline_varifold(G.square_polyline(12, 0.8))
+    return q, e, tc, tu, ta
+
+
+def small_3d(seed=4):
+    q, tri = G.icosphere(0)
+    q = q + 0.05 * G.normal(seed, q.size).reshape(q.shape)
+    sq, st = G.superquadric(1, 9)
+    tc, tu, ta = G.triangle_varifold(sq, st)
+    return q, tri, tc, tu, ta
+
+
+def problem(dim, order, ct="G", dt="B", T=4, w=10.0, width=0.4):
+    q, e, tc, tu, ta = small_2d() if dim == 2 else small_3d()
+    return oracle.OracleProblem(q, e, tc, tu, ta, ct, 0.3, dt, 0.7, w, order, width, 1.0 / (T - 1), T)
+
+
+def test_multiply_kernel_brute_force():
+    q = G.normal(1, 60).reshape(20, 3)
+    a = G.normal(2, 60).reshape(20, 3)
+    d2 = ((q[:, None, :] - q[None, :, :]) ** 2).sum(-1)
+    K = np.exp(-d2 / 0.7 ** 2)
+    assert rel_err(oracle.multiply_kernel(q, a, -1, 0.7), K @ a) <= 1e-14
+    d = np.sqrt(d2) / 0.7
+    K1 = (1 + d) * np.exp(-d)
+    assert rel_err(oracle.multiply_kernel(q, a, 1, 0.7), K1 @ a) <= 1e-14
+
+
+@pytest.mark.parametrize("dim,order", [(2, -1), (3, -1), (3, 1), (3, 2), (3, 4), (2, 3)])
+def test_dq_kernel_is_gradient_of_bilinear_form(dim, order):
+    n = 9
+    q = (0.3 * G.normal(5, n * dim)).reshape(n, dim)
+    l = G.normal(6, n * dim).reshape(n, dim)
+    r = G.normal(7, n * dim).reshape(n, dim)
+    w = 0.5
+
+    def form(x):
+        return float(np.sum(l * oracle.multiply_kernel(x, r, order, w)))
+
+    g = oracle.dq_kernel(q, l, r, order, w)
+    u = G.normal(8, n * dim).reshape(n, dim)
+    eps = 1e-6
+    fd = (form(q + eps * u) - form(q - eps * u)) / (2 * eps)
+    assert abs(np.sum(g * u) - fd) <= 1e-7 * max(1.0, abs(fd))
+
+
+@pytest.mark.parametrize("dim,order,ct,dt", [(2, -1, "G", "B"), (2, -1, "C", "L"), (3, -1, "G", "B"),
+                                             (3, 1, "C", "B"), (3, 3, "G", "O"), (3, 2, "G", "U")])
+def test_objgrd_matches_finite_differences(dim, order, ct, dt):
+    P = problem(dim, order, ct, dt)
+    a = 0.3 * G.normal(21, P.varnum)
+    f, g = P.objgrd(a)
+    for k in range(3):
+        u = G.normal(30 + k, P.varnum)
+        eps = 1e-6
+        fp, _ = P.objgrd(a + eps * u)
+        fm, _ = P.objgrd(a - eps * u)
+        fd = (fp - fm) / (2 * eps)
+        assert abs(np.dot(g, u) - fd) <= 1e-6 * max(1.0, abs(fd)), (k, np.dot(g, u), fd)
+
+
+def test_objgrd_zero_momentum_closed_form():
+    for dim in (2, 3):
+        P = problem(dim, -1)
+        f, g = P.objgrd(np.zeros(P.varnum))
+        v = oracle.varifold(P.q0, P.elm, P.ifo, P.tc, P.tu, P.ta, "G", 0.3, "B", 0.7, grad=False)
+        assert abs(f - 10.0 * v) <= 1e-13 * abs(f)
+        # at alpha = 0 the flow is the identity: g_t = h K(q0) (0 - p) with p = -w dV, the same for all t
+        _, dq = oracle.varifold(P.q0, P.elm, P.ifo, P.tc, P.tu, P.ta, "G", 0.3, "B", 0.7)
+        h = 1.0 / (P.T - 1)
+        g_last = g.reshape(P.T - 1, P.n, P.dim)[-1]
+        expect = h * oracle.multiply_kernel(P.q0, 10.0 * dq, -1, 0.4)
+        assert rel_err(g_last, expect) <= 1e-12
+
+
+@pytest.mark.parametrize("dim", (2, 3))
+@pytest.mark.parametrize("ct", "GC")
+@pytest.mark.parametrize("dt", "BLOU")
+def test_varifold_gradient_finite_differences(dim, ct, dt):
+    q, e, tc, tu, ta = small_2d() if dim == 2 else small_3d()
+    ifo = oracle.get_elm_ifo(e, q.shape[0])
+    args = (e, ifo, tc, tu, ta, ct, 0.3, dt, 0.7)
+    v, dq = oracle.varifold(q, *args)
+    u = G.normal(41, q.size).reshape(q.shape)
+    eps = 1e-6
+    fd = (oracle.varifold(q + eps * u, *args, grad=False) - oracle.varifold(q - eps * u, *args, grad=False)) / (2 * eps)
+    assert abs(np.sum(dq * u) - fd) <= 1e-7 * max(1.0, abs(fd))
+
+
+def test_varifold_vanishes_on_its_own_target():
+    q, tri, *_ = small_3d()
+    c, u, a = G.triangle_varifold(q, tri)
+    ifo = oracle.get_elm_ifo(tri, q.shape[0])
+    v, dq = oracle.varifold(q, tri, ifo, c, u, a, "G", 0.3, "B", 0.0)
+    assert abs(v) <= 1e-13 and np.max(np.abs(dq)) <= 1e-12
+
+
+def test_varifold_translation_invariance_without_target():
+    q, tri, *_ = small_3d()
+    ifo = oracle.get_elm_ifo(tri, q.shape[0])
+    empty = np.zeros((0, 3))
+    v, dq = oracle.varifold(q, tri, ifo, empty, empty, np.zeros(0), "C", 0.3, "U", 0.7)
+    assert np.max(np.abs(dq.sum(0))) <= 1e-12 * np.max(np.abs(dq))
+    v2 = oracle.varifold(q + 0.37, tri, ifo, empty, empty, np.zeros(0), "C", 0.3, "U", 0.7, grad=False)
+    assert abs(v - v2) <= 1e-12 * abs(v)
+
+
+def test_unoriented_kernels_ignore_target_orientation():
+    q, tri, tc, tu, ta = small_3d()
+    ifo = oracle.get_elm_ifo(tri, q.shape[0])
+    flip = np.where(np.arange(len(ta)) % 3 == 0, -1.0, 1.0)[:, None]
+    for dt in "BU":
+        v1 = oracle.varifold(q, tri, ifo, tc, tu, ta, "G", 0.3, dt, 0.7, grad=False)
+        v2 = oracle.varifold(q, tri, ifo, tc, tu * flip, ta, "G", 0.3, dt, 0.7, grad=False)
+        assert abs(v1 - v2) <= 1e-13 * abs(v1)
+    v1 = oracle.varifold(q, tri, ifo, tc, tu, ta, "G", 0.3, "L", 0.7, grad=False)
+    v2 = oracle.varifold(q, tri, ifo, tc, tu * flip, ta, "G", 0.3, "L", 0.7, grad=False)
+    assert abs(v1 - v2) > 1e-6
+
+
+def _get_elm_ifo_by_definition(elm0, nq):
+    """Direct reading of Cpp/src/getElmIfo.m: for each vertex, the elements
+    containing it in ascending order, with (elm, sign) in 2D and
+    (elm, v3, v2) / (elm, v1, v3) / (elm, v2, v1) in 3D."""
+    vpe = elm0.shape[1]
+    cols = []
+    for n in range(nq):
+        col = []
+        for e in range(elm0.shape[0]):
+            for k in range(vpe):
+                if elm0[e, k] == n:
+                    if vpe == 2:
+                        col += [e, -1 if k == 0 else 1]
+                    else:
+                        ev = elm0[e]
+                        col += [e] + [[ev[2], ev[1]], [ev[0], ev[2]], [ev[1], ev[0]]][k]
+        cols.append([len(col) // vpe] + col)
+    rows = max(len(c) for c in cols)
+    out = np.zeros((nq, rows), dtype=np.int32)
+    for n, c in enumerate(cols):
+        out[n, :len(c)] = c
+    return out
+
+
+def test_get_elm_ifo_matches_definition():
+    q, tri = G.icosphere(1)
+    assert np.array_equal(oracle.get_elm_ifo(tri, q.shape[0]), _get_elm_ifo_by_definition(tri, q.shape[0]))
+    q2, seg = G.circle_curve(11)
+    assert np.array_equal(oracle.get_elm_ifo(seg, 11), _get_elm_ifo_by_definition(seg, 11))
+
+
+def test_two_loop_equals_dense_bfgs_inverse():
+    n, m = 6, 5
+    rng_s = G.normal(61, n * 4).reshape(4, n)
+    A = np.diag(np.linspace(1, 5, n)) + 0.1
+    S = np.zeros((m, n))
+    Y = np.zeros((m, n))
+    for k in range(4):
+        S[k] = rng_s[k]
+        Y[k] = A @ rng_s[k]
+    g = G.normal(62, n)
+    new = 3
+    gamma = S[new] @ Y[new] / (Y[new] @ Y[new])
+    d = oracle.get_direction(gamma, g, S, Y, new, 4, m)
+    H = gamma * np.eye(n)
+    for k in range(4):
+        rho = 1.0 / (Y[k] @ S[k])
+        V = np.eye(n) - rho * np.outer(Y[k], S[k])
+        H = V.T @ H @ V + rho * np.outer(S[k], S[k])
+    assert rel_err(d, -H @ g) <= 1e-12
+
+
+def test_line_search_codes():
+    # 1: not a descent direction
+    quad = lambda x: (0.5 * float(x @ x), x.copy())
+    x = np.array([1.0, -2.0])
+    code, *_ = oracle.line_search(quad, x, x, x, 2.5, 0.0, 0.5)
+    assert code == 1
+    # 2: unbounded below along d -> the bracket never closes
+    c = np.array([1.0, 2.0])
+    lin = lambda x: (float(c @ x), c.copy())
+    code, *_ = oracle.line_search(lin, x, c, -c, float(c @ x), 0.0, 0.5)
+    assert code == 2
+    # 3: f and g inconsistent: g claims descent, f rises -> bracketed, never satisfied
+    bad = lambda x: (float(x @ x), np.array([-1.0, 0.0]))
+    z = np.zeros(2)
+    code, *_ = oracle.line_search(bad, z, np.array([-1.0, 0.0]), np.array([1.0, 0.0]), 0.0, 0.5, 0.9)
+    assert code == 3
+    # 0 on a quadratic: the unit step of the Newton direction satisfies weak Wolfe
+    code, xt, gt, ft, step, cnt = oracle.line_search(quad, x, x, -x, 2.5, 1e-4, 0.9)
+    assert code == 0 and cnt == 1 and np.allclose(xt, 0.0)
+
+
+def test_lbfgs_converges_on_rosenbrock():
+    def rosen(x):
+        f = 100.0 * (x[1] - x[0] ** 2) ** 2 + (1 - x[0]) ** 2
+        g = np.array([-400.0 * x[0] * (x[1] - x[0] ** 2) - 2 * (1 - x[0]), 200.0 * (x[1] - x[0] ** 2)])
+        return f, g
+    x, f, tr = oracle.lbfgs(rosen, np.array([-1.2, 1.0]), itr_max=500, tol=1e-9, wolfe1=1e-4, wolfe2=0.9, mem=5)
+    assert tr["status"] == 0 and np.allclose(x, 1.0, atol=1e-6)
+
+
+def test_lddmm_run_decreases_energy():
+    P = problem(2, -1, T=5)
+    alp, f, tr = P.run(itr_max=15, tol=0.0, wolfe1=0.0, wolfe2=0.5, mem=5)
+    assert tr["iters"] == 15 and tr["status"] == 0
+    assert np.all(np.diff(tr["f"]) < 0)
+    f2, _ = P.objgrd(alp)
+    assert f2 == f
