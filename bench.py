@@ -110,6 +110,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="sphere_cube")
     ap.add_argument("--no-registration", action="store_true")
+    ap.add_argument("--reg-iters", type=int, default=0, help="L-BFGS iterations of the registration (0 = config)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -194,7 +195,8 @@ def main():
     if not args.no_registration:
         alp_opt, info = lddmm_amd.run(w["q0"], None, w["elm"], ifo, w["tgt_cen"], w["tgt_vol"], w["tgt_dir"],
                                       w["cen_type"], w["cen_width"], w["dir_type"], w["dir_width"], w["dis_wgt"],
-                                      w["knl_order"], w["knl_width"], w["time_stp"], T, extra["itr"], 0.0, 0.0, 0.5, 5)
+                                      w["knl_order"], w["knl_width"], w["time_stp"], T,
+                                      args.reg_iters or extra["itr"], 0.0, 0.0, 0.5, 5)
         out["registration"] = dict(iterations=info["iterations"], evaluations=info["evaluations"],
                                    status=info["status"], final_energy=info["fcn"], grad_norm=info["grad_norm"],
                                    seconds=info["seconds"], registrations_per_s=1.0 / info["seconds"],
