@@ -35,14 +35,22 @@ extern "C" {
 #define LDDMM_ERR_HIP (-3)
 #define LDDMM_ERR_UNSUPPORTED (-4)
 
-#define LDDMM_PRECISION_FP64 0
+#define LDDMM_PRECISION_FP64 0   /* FP64 throughout: the reference's arithmetic */
+#define LDDMM_PRECISION_FP32 1   /* FP32 pair terms with FP64 coordinate differences, FP64
+                                    cross-tile sums, energies, reductions and L-BFGS state */
 
 /* Options beyond the reference's 22 arguments; NULL = defaults. */
 typedef struct lddmm_options {
-    int precision;            /* LDDMM_PRECISION_FP64 (reference arithmetic) */
+    int precision;            /* LDDMM_PRECISION_FP64 (default) or LDDMM_PRECISION_FP32 */
+    int no_fp64_refine;       /* FP32 runs: 0 (default) = once the FP32 phase stalls (line-search
+                                 failure or relative decrease below refine_rel_tol), re-evaluate in
+                                 FP64 and continue in FP64 with the L-BFGS memory kept; 1 = stop */
     double rel_energy_tol;    /* > 0: also stop once (f_prev - f)/|f| < rel_energy_tol */
+    double refine_rel_tol;    /* stall threshold of the FP32 phase; 0 = 1e-7 */
+    double target_energy;     /* used when has_target: stop once f <= target_energy */
+    int has_target;
     int pair_target_blocks;   /* workgroups per all-pairs launch; 0 = default */
-    int reserved[5];
+    int reserved[4];
 } lddmm_options;
 
 /* Run summary; may be NULL. */
@@ -54,7 +62,9 @@ typedef struct lddmm_result {
     int iterations;           /* accepted L-BFGS iterations */
     int evaluations;          /* objective evaluations */
     int status;               /* 0 or line-search code 1..3 */
-    int reserved;
+    int fp32_iterations;      /* iterations done with FP32 pair arithmetic */
+    int reached_target;       /* 1 if the run stopped on target_energy */
+    int reserved[3];
 } lddmm_result;
 
 /* ---------------------------------------------------------------- registration
@@ -129,6 +139,9 @@ int lddmm_objective_synchronize(lddmm_objective *obj);
 int lddmm_objective_pairs(lddmm_objective *obj, double *pairsPerEval);
 /* D x N x T host-layout node trajectory of the last lddmm_objgrd call (ndeStk). */
 int lddmm_objective_trajectory(lddmm_objective *obj, double *ndeStk);
+/* Pair arithmetic of later evaluations: LDDMM_PRECISION_FP64 (the default)
+ * or LDDMM_PRECISION_FP32. */
+int lddmm_objective_set_precision(lddmm_objective *obj, int precision);
 /* The constant target-target varifold term, cached at create time. */
 int lddmm_objective_target_term(lddmm_objective *obj, double *value);
 void lddmm_objective_destroy(lddmm_objective *obj);

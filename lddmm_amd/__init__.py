@@ -37,15 +37,19 @@ class LddmmError(RuntimeError):
 
 
 class Options(C.Structure):
-    _fields_ = [("precision", C.c_int), ("rel_energy_tol", C.c_double), ("pair_target_blocks", C.c_int),
-                ("reserved", C.c_int * 5)]
+    _fields_ = [("precision", C.c_int), ("no_fp64_refine", C.c_int), ("rel_energy_tol", C.c_double),
+                ("refine_rel_tol", C.c_double), ("target_energy", C.c_double), ("has_target", C.c_int),
+                ("pair_target_blocks", C.c_int), ("reserved", C.c_int * 4)]
 
 
 class Result(C.Structure):
     _fields_ = [("fcn", C.c_double), ("grad_norm", C.c_double), ("seconds", C.c_double),
                 ("pairs_per_eval", C.c_double), ("iterations", C.c_int), ("evaluations", C.c_int),
-                ("status", C.c_int), ("reserved", C.c_int)]
+                ("status", C.c_int), ("fp32_iterations", C.c_int), ("reached_target", C.c_int),
+                ("reserved", C.c_int * 3)]
 
+
+PRECISIONS = {"fp64": 0, "fp32": 1}
 
 _RUN_ARGS = [_dp, _dp, _ip, _ip, _dp, _dp, _dp, C.c_char, C.c_double, C.c_char, C.c_double, C.c_double,
              C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
@@ -75,6 +79,7 @@ def lib():
         L.lddmm_objective_synchronize.argtypes = [C.c_void_p]
         L.lddmm_objective_pairs.argtypes = [C.c_void_p, _dp]
         L.lddmm_objective_target_term.argtypes = [C.c_void_p, _dp]
+        L.lddmm_objective_set_precision.argtypes = [C.c_void_p, C.c_int]
         L.lddmm_objective_destroy.argtypes = [C.c_void_p]
         L.lddmm_objective_destroy.restype = None
         L.lddmm_multiply_kernel.argtypes = [_dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int]
@@ -229,7 +234,7 @@ class Objective:
     elm is a 0-based (E, V) table; ifo the (N, rows) adjacency table."""
 
     def __init__(self, q0, elm, ifo, tgt_cen, tgt_vol, tgt_dir, cen_type, cen_width, dir_type, dir_width,
-                 dis_wgt, knl_order, knl_width, time_stp, time_num):
+                 dis_wgt, knl_order, knl_width, time_stp, time_num, precision="fp64"):
         self.q0, self.elm, self.ifo = _f64(q0), _i32(elm), _i32(ifo)
         self.tc, self.ta, self.tu = _f64(tgt_cen), _f64(tgt_vol), _f64(tgt_dir)
         self.n, self.dim = self.q0.shape
@@ -241,6 +246,13 @@ class Objective:
                                             _ch(dir_type), dir_width, dis_wgt, knl_order, knl_width, time_stp,
                                             self.T, self.n, self.elm.shape[0], self.ifo.shape[1], len(self.ta)))
         self._h = h
+        if precision != "fp64":
+            self.set_precision(precision)
+
+    def set_precision(self, precision):
+        """'fp64' (the reference's arithmetic) or 'fp32' (FP32 pair terms,
+        FP64 differences, cross-tile sums, energies and reductions)."""
+        _check(lib().lddmm_objective_set_precision(self._h, PRECISIONS[precision]))
 
     def __call__(self, alp):
         a = _f64(alp).reshape(-1)
@@ -288,7 +300,8 @@ class Objective:
 # ---------------------------------------------------------------- registration
 def run(q0, alp0, elm, ifo, tgt_cen, tgt_vol, tgt_dir, cen_type, cen_width, dir_type, dir_width, dis_wgt,
         knl_order, knl_width, time_stp, time_num, itr_max, tol, wolfe1, wolfe2, mem, verbose=False,
-        rel_energy_tol=0.0, pair_target_blocks=0):
+        rel_energy_tol=0.0, pair_target_blocks=0, precision="fp64", fp64_refine=True, refine_rel_tol=0.0,
+        target_energy=None):
     """The LDDMM2D/LDDMM3D MEX call (Cpp/src/LDDMM.cpp:11-98) with 0-based
     tables.  Returns (optAlpVec reshaped (T-1, N, D), result dict)."""
     q0, elm, ifo = _f64(q0), _i32(elm), _i32(ifo)
@@ -297,7 +310,9 @@ def run(q0, alp0, elm, ifo, tgt_cen, tgt_vol, tgt_dir, cen_type, cen_width, dir_
     T = int(time_num)
     a0 = np.zeros(d * n * (T - 1)) if alp0 is None else _f64(alp0).reshape(-1)
     out = np.empty(d * n * (T - 1))
-    opts = Options(0, rel_energy_tol, pair_target_blocks)
+    opts = Options(PRECISIONS[precision], 0 if fp64_refine else 1, rel_energy_tol, refine_rel_tol,
+                   0.0 if target_energy is None else float(target_energy), int(target_energy is not None),
+                   pair_target_blocks)
     res = Result()
     st = _check(lib().lddmm_run_ex(d, _p(q0), _p(a0), _p(elm), _p(ifo), _p(tc), _p(ta), _p(tu), _ch(cen_type),
                                    cen_width, _ch(dir_type), dir_width, dis_wgt, knl_order, knl_width, time_stp, T,
@@ -305,7 +320,8 @@ def run(q0, alp0, elm, ifo, tgt_cen, tgt_vol, tgt_dir, cen_type, cen_width, dir_
                                    ifo.shape[1], len(ta), _p(out), C.byref(opts), C.byref(res)),
                 allow_positive=True)
     info = dict(fcn=res.fcn, grad_norm=res.grad_norm, seconds=res.seconds, pairs_per_eval=res.pairs_per_eval,
-                iterations=res.iterations, evaluations=res.evaluations, status=st)
+                iterations=res.iterations, evaluations=res.evaluations, status=st,
+                fp32_iterations=res.fp32_iterations, reached_target=bool(res.reached_target))
     return out.reshape(T - 1, n, d), info
 
 

@@ -202,10 +202,21 @@ LbfgsOptions make_opts(int itrMax, double tol, double w1, double w2, int mem, in
     lo.mem = mem;
     lo.verbose = vbs != 0;
     if (o) {
-        require(o->precision == LDDMM_PRECISION_FP64, "only LDDMM_PRECISION_FP64 is available");
         lo.rel_tol = o->rel_energy_tol;
+        lo.fp64_refine = o->no_fp64_refine == 0;
+        if (o->refine_rel_tol > 0.0) lo.refine_rel_tol = o->refine_rel_tol;
+        lo.has_target = o->has_target != 0;
+        lo.target = o->target_energy;
     }
     return lo;
+}
+
+int precision_of(const lddmm_options *o)
+{
+    if (!o) return kFP64;
+    require(o->precision == LDDMM_PRECISION_FP64 || o->precision == LDDMM_PRECISION_FP32,
+            "precision must be LDDMM_PRECISION_FP64 or LDDMM_PRECISION_FP32");
+    return o->precision == LDDMM_PRECISION_FP32 ? kFP32 : kFP64;
 }
 
 void fill_result(lddmm_result *r, const LbfgsResult &lr, const Objective &obj)
@@ -219,6 +230,8 @@ void fill_result(lddmm_result *r, const LbfgsResult &lr, const Objective &obj)
     r->iterations = lr.iters;
     r->evaluations = lr.evals;
     r->status = lr.status;
+    r->fp32_iterations = lr.fp32_iters;
+    r->reached_target = lr.reached_target;
 }
 
 int run_host(int dim, const double *iniNdeMat, const double *iniAlpVec, const int32_t *elm, const int32_t *ifo,
@@ -233,6 +246,7 @@ int run_host(int dim, const double *iniNdeMat, const double *iniAlpVec, const in
     Stream st;
     UploadedProblem u;
     fill_params(u.p, dim, n, T, order, width, h, w, ct, cw, dt, dw, ne, nt, rows);
+    u.p.prec = precision_of(opts);
     validate(u.p);
     upload_host_problem(u, dim, iniNdeMat, elm, ifo, tc, ta, tu, n, ne, rows, nt, st.s);
     Objective obj(u.p, st.s, opts ? opts->pair_target_blocks : 0);
@@ -319,6 +333,7 @@ int lddmm_run_dev(int dimNum, const double *d_iniNdeMat, const double *d_iniAlpV
         DeviceProblem p;
         fill_params(p, dimNum, ndeNum, timeNum, knlOrder, knlWidth, timeStp, disWgt, cenKnlType, cenKnlWidth,
                     dirKnlType, dirKnlWidth, dfmElmNum, tgtElmNum, dfmElmIfoNum);
+        p.prec = precision_of(opts);
         p.q0 = d_iniNdeMat;
         p.elm = d_elm;
         p.ifo = d_ifo;
@@ -417,6 +432,19 @@ int lddmm_objective_trajectory(lddmm_objective *obj, double *ndeStk)
     });
 }
 
+int lddmm_objective_set_precision(lddmm_objective *obj, int precision)
+{
+    return guarded([&] {
+        require(obj != nullptr, "null argument");
+        require(precision == LDDMM_PRECISION_FP64 || precision == LDDMM_PRECISION_FP32, "unknown precision");
+        obj->st.sync();
+        obj->u.p.prec = precision == LDDMM_PRECISION_FP32 ? kFP32 : kFP64;
+        obj->obj.reset();
+        obj->obj = std::make_unique<Objective>(obj->u.p, obj->st.s);
+        return 0;
+    });
+}
+
 int lddmm_objective_target_term(lddmm_objective *obj, double *value)
 {
     return guarded([&] {
@@ -449,11 +477,11 @@ int lddmm_multiply_kernel_cross(double *vlcMat, const double *lmkiMat, const dou
         qi.upload(hqi.data(), st.s);
         qj.upload(hqj.data(), st.s);
         a.upload(ha.data(), st.s);
-        PairSplit sp = make_split(lmkiNum, lmkjNum, 2048);
+        PairSplit sp = plan_kv(lmkiNum, lmkjNum, kFP64, 2048);
         DevBuf<double> part((size_t)sp.splits * dimNum * lmkiNum);
         {
             ProfScope ps(kProfOperator, st.s);
-            launch_kv(dimNum, knlOrder, qi.p, lmkiNum, qj.p, a.p, lmkjNum, knlWidth, sp, part.p, st.s);
+            launch_kv(dimNum, knlOrder, kFP64, qi.p, lmkiNum, qj.p, a.p, lmkjNum, knlWidth, sp, part.p, st.s);
             launch_sum_splits(dimNum, sp.splits, lmkiNum, part.p, 1.0, v.p, st.s);
         }
         v.download(hqi.data(), st.s);
@@ -485,11 +513,11 @@ int lddmm_dq_kernel(double *dqKMat, const double *lmkMat, const double *lftMat, 
         q.upload(hq.data(), st.s);
         l.upload(hl.data(), st.s);
         r.upload(hr.data(), st.s);
-        PairSplit sp = make_split(lmkNum, lmkNum, 2048);
+        PairSplit sp = plan_single(lmkNum, lmkNum, 2048);
         DevBuf<double> part((size_t)sp.splits * dimNum * lmkNum);
         {
             ProfScope ps(kProfOperator, st.s);
-            launch_dq_self(dimNum, knlOrder, q.p, l.p, r.p, lmkNum, knlWidth, sp, part.p, st.s);
+            launch_dq_self(dimNum, knlOrder, kFP64, q.p, l.p, r.p, lmkNum, knlWidth, sp, part.p, st.s);
             launch_sum_splits(dimNum, sp.splits, lmkNum, part.p, 1.0, out.p, st.s);
         }
         out.download(hq.data(), st.s);
@@ -522,13 +550,13 @@ int lddmm_dq_kernel_cross(double *dqiKMat, double *dqjKMat, const double *lmkiMa
         qj.upload(hqj.data(), st.s);
         l.upload(hl.data(), st.s);
         r.upload(hr.data(), st.s);
-        PairSplit spi = make_split(lmkiNum, lmkjNum, 2048), spj = make_split(lmkjNum, lmkiNum, 2048);
+        PairSplit spi = plan_single(lmkiNum, lmkjNum, 2048), spj = plan_single(lmkjNum, lmkiNum, 2048);
         DevBuf<double> part(std::max((size_t)spi.splits * dimNum * lmkiNum, (size_t)spj.splits * dimNum * lmkjNum));
         {
             ProfScope ps(kProfOperator, st.s);
-            launch_dq_cross(dimNum, knlOrder, qi.p, l.p, lmkiNum, qj.p, r.p, lmkjNum, knlWidth, spi, part.p, st.s);
+            launch_dq_cross(dimNum, knlOrder, kFP64, qi.p, l.p, lmkiNum, qj.p, r.p, lmkjNum, knlWidth, spi, part.p, st.s);
             launch_sum_splits(dimNum, spi.splits, lmkiNum, part.p, 1.0, oi.p, st.s);
-            launch_dq_cross(dimNum, knlOrder, qj.p, r.p, lmkjNum, qi.p, l.p, lmkiNum, knlWidth, spj, part.p, st.s);
+            launch_dq_cross(dimNum, knlOrder, kFP64, qj.p, r.p, lmkjNum, qi.p, l.p, lmkiNum, knlWidth, spj, part.p, st.s);
             launch_sum_splits(dimNum, spj.splits, lmkjNum, part.p, 1.0, oj.p, st.s);
         }
         oi.download(hqi.data(), st.s);
@@ -587,25 +615,25 @@ int lddmm_varifold(double *vfdVal, double *dqVfdMat, const double *dfmLmkPosMat,
         upload_host_problem(u, dimNum, dfmLmkPosMat, dfmElmVtxMat, dfmElmIfoMat, tgtCenPosMat, tgtElmVolVec,
                             tgtUniDirMat, dfmLmkNum, dfmElmNum, dfmElmIfoNum, tgtElmNum, st.s);
         const int D = dimNum, ne = dfmElmNum, nt = tgtElmNum, nq = dfmLmkNum;
-        PairSplit spd = make_split(ne, ne, 1024), spt = make_split(ne, std::max(nt, 1), 1024);
-        PairSplit sptt = make_split(std::max(nt, 1), std::max(nt, 1), 2048);
+        PairSplit spd = plan_vf(ne, ne, 1024), spt = plan_vf(ne, std::max(nt, 1), 1024);
+        PairSplit sptt = plan_vf(std::max(nt, 1), std::max(nt, 1), 2048);
         DevBuf<double> cen((size_t)D * ne), dir((size_t)D * ne), vol(ne), dc((size_t)D * ne), dd((size_t)D * ne);
         DevBuf<double> part(std::max((size_t)(spd.splits + spt.splits) * vf_comps(D) * ne,
                                      (size_t)sptt.splits * std::max(nt, 1)));
-        DevBuf<double> ep(std::max(vf_finalize_blocks(ne), ceil_div(std::max(nt, 1), kBlock)));
+        DevBuf<double> ep(std::max(vf_finalize_blocks(ne), vf_value_finalize_blocks(std::max(nt, 1))));
         DevBuf<double> scratch(kDotBlocks), vals(2), dq((size_t)D * nq);
         {
             ProfScope ps(kProfOperator, st.s);
             launch_elm_geometry(D, u.p.q0, nq, u.p.elm, ne, cen.p, dir.p, vol.p, st.s);
-            launch_vf_pairs(D, u.p.vp, cen.p, dir.p, vol.p, ne, u.p.tc, u.p.tu, u.p.ta, nt, spd, spt, part.p, st.s);
-            launch_vf_finalize(D, u.p.vp, spd.splits + (nt > 0 ? spt.splits : 0), ne, part.p, dir.p, vol.p, dc.p,
+            launch_vf_pairs(D, kFP64, u.p.vp, cen.p, dir.p, vol.p, ne, u.p.tc, u.p.tu, u.p.ta, nt, spd, spt, part.p, st.s);
+            launch_vf_finalize(D, u.p.vp, spd.splits, spd.splits + (nt > 0 ? spt.splits : 0), ne, part.p, dir.p, vol.p, dc.p,
                                dd.p, ep.p, st.s);
             launch_sum(ep.p, vf_finalize_blocks(ne), scratch.p, vals.p, st.s);
             if (dqVfdMat) launch_vf_gather(D, u.p.q0, nq, u.p.ifo, dfmElmIfoNum, dc.p, dd.p, ne, 1.0, dq.p, st.s);
             if (nt > 0) {
                 launch_vf_value_pairs(D, u.p.vp, u.p.tc, u.p.tu, u.p.ta, nt, sptt, part.p, st.s);
                 launch_vf_value_finalize(sptt.splits, nt, part.p, u.p.ta, ep.p, st.s);
-                launch_sum(ep.p, ceil_div(nt, kBlock), scratch.p, vals.p + 1, st.s);
+                launch_sum(ep.p, vf_value_finalize_blocks(nt), scratch.p, vals.p + 1, st.s);
             } else {
                 LDDMM_HIP(hipMemsetAsync(vals.p + 1, 0, sizeof(double), st.s));
             }
@@ -680,12 +708,12 @@ int lddmm_compute_node_position(double *ndeStk, double *vlcStk, const double *in
         DevBuf<double> q(dn * T), a(std::max(ha.size(), (size_t)1)), v(std::max(dn * (T - 1), (size_t)1));
         LDDMM_HIP(hipMemcpyAsync(q.p, hq.data(), dn * sizeof(double), hipMemcpyHostToDevice, st.s));
         if (!ha.empty()) LDDMM_HIP(hipMemcpyAsync(a.p, ha.data(), ha.size() * sizeof(double), hipMemcpyHostToDevice, st.s));
-        PairSplit sp = make_split(n, n, 2048);
+        PairSplit sp = plan_kv(n, n, kFP64, 2048);
         DevBuf<double> part((size_t)sp.splits * D * n), ep(fwd_reduce_blocks(n));
         for (int t = 0; t < T - 1; ++t) {
             ProfScope ps(kProfOperator, st.s);
             const double *qt = q.p + (size_t)t * dn, *at = a.p + (size_t)t * dn;
-            launch_kv(D, knlOrder, qt, n, qt, at, n, knlWidth, sp, part.p, st.s);
+            launch_kv(D, knlOrder, kFP64, qt, n, qt, at, n, knlWidth, sp, part.p, st.s);
             launch_fwd_reduce(D, sp.splits, n, part.p, qt, at, timeStp, q.p + (size_t)(t + 1) * dn,
                               v.p + (size_t)t * dn, ep.p, st.s);
         }

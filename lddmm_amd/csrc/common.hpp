@@ -40,6 +40,13 @@ struct Error : std::runtime_error {
 
 #define LDDMM_LAUNCH_CHECK() LDDMM_HIP(hipGetLastError())
 
+// Precision policies of the pair sums (the L-BFGS state, energies and all
+// reductions across tiles are FP64 in both).
+enum : int {
+    kFP64 = 0,   // FP64 throughout: the reference's arithmetic
+    kFP32 = 1,   // FP32 pair terms, FP64 coordinate differences and cross-tile sums
+};
+
 constexpr int kBlock = 256;       // threads per block of every pair kernel (4 wave64)
 constexpr int kTileJ = 256;       // j points staged in LDS per tile
 constexpr int kDotBlocks = 512;   // fixed grid of the deterministic dot reduction
@@ -111,6 +118,15 @@ inline void dispatch_order(int order, F &&f)
 }
 
 inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// Sum over the 64 lanes of a wave by an xor butterfly: every lane ends
+// with the same, order-fixed result.
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
 
 // j-split of an all-pairs sum over ni targets and nj sources: every block
 // owns kBlock targets and a chunk of sources; chunks are multiples of 64 so

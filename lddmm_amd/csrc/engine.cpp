@@ -148,20 +148,24 @@ Objective::Objective(const DeviceProblem &p, hipStream_t st, int target_blocks) 
 {
     validate(p_);
     const int tb = target_blocks > 0 ? target_blocks : default_target_blocks();
+    target_blocks_ = tb;
     const int D = p_.dim;
-    sp_pts_ = make_split(p_.n, p_.n, tb);
+    sp_kv_ = plan_kv(p_.n, p_.n, p_.prec, tb);
+    const PairSplit kv_other = plan_kv(p_.n, p_.n, p_.prec == kFP32 ? kFP64 : kFP32, tb);
+    sp_bwd_ = plan_single(p_.n, p_.n, tb);
     const double fd = (double)p_.ne / (double)(p_.ne + p_.nt);
     int tbd = (int)std::max(1.0, tb * fd), tbt = std::max(1, tb - tbd);
-    sp_vfd_ = make_split(p_.ne, p_.ne, tbd);
-    sp_vft_ = make_split(p_.ne, std::max(p_.nt, 1), tbt);
-    PairSplit sp_tt = make_split(std::max(p_.nt, 1), std::max(p_.nt, 1), tb);
+    sp_vfd_ = plan_vf(p_.ne, p_.ne, tbd);
+    sp_vft_ = plan_vf(p_.ne, std::max(p_.nt, 1), tbt);
+    PairSplit sp_tt = plan_vf(std::max(p_.nt, 1), std::max(p_.nt, 1), tb);
 
-    size_t part = (size_t)sp_pts_.splits * 2 * D * p_.n;
+    size_t part = (size_t)std::max(sp_kv_.splits, kv_other.splits) * D * p_.n;
+    part = std::max(part, (size_t)sp_bwd_.splits * 2 * D * p_.n);
     part = std::max(part, (size_t)(sp_vfd_.splits + sp_vft_.splits) * vf_comps(D) * p_.ne);
     part = std::max(part, (size_t)sp_tt.splits * std::max(p_.nt, 1));
     nb_fwd_ = fwd_reduce_blocks(p_.n);
     nb_vf_ = vf_finalize_blocks(p_.ne);
-    const int nb_tt = ceil_div(std::max(p_.nt, 1), kBlock);
+    const int nb_tt = vf_value_finalize_blocks(std::max(p_.nt, 1));
 
     slab_.reserve(&nde_, (size_t)D * p_.n * p_.T);
     slab_.reserve(&p_adj_, (size_t)D * p_.n);
@@ -181,6 +185,14 @@ Objective::Objective(const DeviceProblem &p, hipStream_t st, int target_blocks) 
 
 Objective::~Objective() = default;
 
+void Objective::set_precision(int prec)
+{
+    if (prec != kFP64 && prec != kFP32) throw Error(kErrArg, "unknown precision policy");
+    p_.prec = prec;
+    // the partial-sum buffer was sized for the plans of both policies
+    sp_kv_ = plan_kv(p_.n, p_.n, prec, target_blocks_);
+}
+
 void Objective::compute_target_target(int target_blocks)
 {
     if (p_.nt <= 0) {
@@ -189,12 +201,12 @@ void Objective::compute_target_target(int target_blocks)
         tt_host_ = 0.0;
         return;
     }
-    PairSplit sp = make_split(p_.nt, p_.nt, target_blocks);
+    PairSplit sp = plan_vf(p_.nt, p_.nt, target_blocks);
     {
         ProfScope ps(kProfVfTargetTarget, st_);
         launch_vf_value_pairs(p_.dim, p_.vp, p_.tc, p_.tu, p_.ta, p_.nt, sp, part_, st_);
         launch_vf_value_finalize(sp.splits, p_.nt, part_, p_.ta, ep_vf_, st_);
-        launch_sum(ep_vf_, ceil_div(p_.nt, kBlock), scratch_, tt_, st_);
+        launch_sum(ep_vf_, vf_value_finalize_blocks(p_.nt), scratch_, tt_, st_);
     }
     LDDMM_HIP(hipMemcpyAsync(&tt_host_, tt_, sizeof(double), hipMemcpyDeviceToHost, st_));
     LDDMM_HIP(hipStreamSynchronize(st_));
@@ -210,7 +222,6 @@ void Objective::eval(const double *alp, double *grd, double *f_dev)
 {
     const int D = p_.dim, n = p_.n, T = p_.T;
     const size_t dn = (size_t)D * n;
-    const int S = sp_pts_.splits;
 
     // forward flow q_{t+1} = q_t + h K(q_t) alpha_t, energy partials <alpha_t, v_t>
     // (Cpp/src/objgrd.cpp:41-59)
@@ -220,11 +231,11 @@ void Objective::eval(const double *alp, double *grd, double *f_dev)
         const double *at = alp + (size_t)t * dn;
         {
             ProfScope ps(kProfFwdPairs, st_);
-            launch_kv(D, p_.order, qt, n, qt, at, n, p_.width, sp_pts_, part_, st_);
+            launch_kv(D, p_.order, p_.prec, qt, n, qt, at, n, p_.width, sp_kv_, part_, st_);
         }
         {
             ProfScope ps(kProfFwdReduce, st_);
-            launch_fwd_reduce(D, S, n, part_, qt, at, p_.h, nde_ + (size_t)(t + 1) * dn, nullptr,
+            launch_fwd_reduce(D, sp_kv_.splits, n, part_, qt, at, p_.h, nde_ + (size_t)(t + 1) * dn, nullptr,
                               ep_fwd_ + (size_t)t * nb_fwd_, st_);
         }
     }
@@ -237,13 +248,13 @@ void Objective::eval(const double *alp, double *grd, double *f_dev)
     }
     {
         ProfScope ps(kProfVfPairs, st_);
-        launch_vf_pairs(D, p_.vp, ecen_, edir_, evol_, p_.ne, p_.tc, p_.tu, p_.ta, p_.nt, sp_vfd_, sp_vft_, part_,
-                        st_);
+        launch_vf_pairs(D, p_.prec, p_.vp, ecen_, edir_, evol_, p_.ne, p_.tc, p_.tu, p_.ta, p_.nt, sp_vfd_,
+                        sp_vft_, part_, st_);
     }
     {
         ProfScope ps(kProfVfFinalize, st_);
-        launch_vf_finalize(D, p_.vp, sp_vfd_.splits + (p_.nt > 0 ? sp_vft_.splits : 0), p_.ne, part_, edir_,
-                           evol_, dc_, dd_, ep_vf_, st_);
+        launch_vf_finalize(D, p_.vp, sp_vfd_.splits, sp_vfd_.splits + (p_.nt > 0 ? sp_vft_.splits : 0), p_.ne,
+                           part_, edir_, evol_, dc_, dd_, ep_vf_, st_);
     }
     {
         ProfScope ps(kProfVfGather, st_);
@@ -260,11 +271,11 @@ void Objective::eval(const double *alp, double *grd, double *f_dev)
         const double *at = alp + (size_t)t * dn;
         {
             ProfScope ps(kProfBwdPairs, st_);
-            launch_bwd_pairs(D, p_.order, qt, at, p_adj_, n, p_.width, sp_pts_, part_, st_);
+            launch_bwd_pairs(D, p_.order, p_.prec, qt, at, p_adj_, n, p_.width, sp_bwd_, part_, st_);
         }
         {
             ProfScope ps(kProfBwdReduce, st_);
-            launch_bwd_reduce(D, S, n, part_, p_.h, grd + (size_t)t * dn, p_adj_, st_);
+            launch_bwd_reduce(D, sp_bwd_.splits, n, part_, p_.h, grd + (size_t)t * dn, p_adj_, st_);
         }
     }
 }
@@ -353,9 +364,30 @@ LbfgsResult run_lbfgs(Objective &obj, double *x, const LbfgsOptions &o, hipStrea
     }
     push_trace(fnow, glen, 0.0, 1);
 
+    // FP32 objectives finish in FP64: re-evaluate f, g at the current
+    // iterate and keep the curvature pairs (no counterpart in the reference,
+    // whose arithmetic is FP64 throughout)
+    auto refine = [&](const char *why) {
+        obj.set_precision(kFP64);
+        obj.eval(x, s.g, s.scal + kF);
+        const double *xs[1] = {s.g}, *ys[1] = {s.g};
+        dots(s, 1, xs, ys, nv, s.scal + kGG, st);
+        fetch(s, kF, 2, st);
+        fnow = s.host[kF];
+        glen = std::sqrt(s.host[kGG]);
+        ++evals;
+        if (o.verbose) {
+            printf("       FP32 phase ended (%s): continuing in FP64, f = %13.6e  |grad f| = %13.6e\n", why, fnow,
+                   glen);
+            fflush(stdout);
+        }
+    };
+    const bool can_refine = o.fp64_refine && obj.precision() == kFP32;
+
     int new_idx = -1, status = 0;
     for (int it = 1; it <= o.itr_max; ++it) {
         if (glen < o.tol) break;
+        const bool in_fp32 = obj.precision() == kFP32;
 
         // direction: two-loop recursion (Cpp/src/getDirection.cpp:9-66)
         const int his = it <= m ? it - 1 : m;
@@ -400,6 +432,11 @@ LbfgsResult run_lbfgs(Objective &obj, double *x, const LbfgsOptions &o, hipStrea
         }
         fetch(s, kGD, 2, st);
         const double slope = s.host[kGD];
+        if (slope > -DBL_MIN && in_fp32 && can_refine) {
+            refine("not a descent direction");
+            --it;
+            continue;
+        }
         if (slope > -DBL_MIN) {
             printf("\nNot a descent direction.\nQuit LBFGS.\n\n");
             fflush(stdout);
@@ -447,6 +484,11 @@ LbfgsResult run_lbfgs(Objective &obj, double *x, const LbfgsOptions &o, hipStrea
             }
         }
         evals += cnt;
+        if (le != 0 && in_fp32 && can_refine) {
+            refine("line search failed");
+            --it;
+            continue;
+        }
         if (le != 0) {
             if (hi == DBL_MAX) {
                 printf("\nLine search failed to bracket a point satisfying weak Wolfe conditions.\n"
@@ -484,6 +526,16 @@ LbfgsResult run_lbfgs(Objective &obj, double *x, const LbfgsOptions &o, hipStrea
             fflush(stdout);
         }
         push_trace(fnow, glen, step, cnt);
+        if (in_fp32) ++r.fp32_iters;
+        if (o.has_target && fnow <= o.target) {
+            r.reached_target = 1;
+            break;
+        }
+        const bool small = (fprev - fnow) < o.refine_rel_tol * std::fabs(fnow);
+        if (in_fp32 && can_refine && small) {
+            refine("relative decrease below threshold");
+            continue;
+        }
         if (o.rel_tol > 0.0 && (fprev - fnow) < o.rel_tol * std::fabs(fnow)) break;
     }
     LDDMM_HIP(hipStreamSynchronize(st));

@@ -107,6 +107,7 @@ struct DeviceProblem {
     double width = 1.0, h = 0.0, w = 0.0;
     VfParams vp{'G', 'B', 1.0, 0.0};
     int ne = 0, nt = 0, ifo_rows = 1;
+    int prec = kFP64;             // pair arithmetic: kFP64 or kFP32
     const double *q0 = nullptr;   // D x N
     const int *elm = nullptr;     // Ed x V
     const int *ifo = nullptr;     // N x rows
@@ -128,6 +129,9 @@ public:
     const double *trajectory() const { return nde_; }   // T SoA slices of the last eval
     double target_target() const { return tt_host_; }
     const DeviceProblem &problem() const { return p_; }
+    // switch the pair arithmetic (kFP64 / kFP32) of later evaluations
+    void set_precision(int prec);
+    int precision() const { return p_.prec; }
     // pair interactions of one evaluation: forward, fused backward, varifold
     double pairs_per_eval() const;
     double pairs_target_target() const { return (double)p_.nt * p_.nt; }
@@ -135,12 +139,12 @@ public:
 private:
     DeviceProblem p_;
     hipStream_t st_;
-    PairSplit sp_pts_, sp_vfd_, sp_vft_;
+    PairSplit sp_kv_, sp_bwd_, sp_vfd_, sp_vft_;
     Slab slab_;
     double *nde_ = nullptr, *p_adj_ = nullptr, *part_ = nullptr;
     double *ecen_ = nullptr, *edir_ = nullptr, *evol_ = nullptr, *dc_ = nullptr, *dd_ = nullptr;
     double *ep_fwd_ = nullptr, *ep_vf_ = nullptr, *tt_ = nullptr, *scratch_ = nullptr;
-    int nb_fwd_ = 0, nb_vf_ = 0;
+    int nb_fwd_ = 0, nb_vf_ = 0, target_blocks_ = 0;
     double tt_host_ = 0.0;
     void compute_target_target(int target_blocks);
 };
@@ -151,11 +155,17 @@ struct LbfgsOptions {
     double tol = 0.0, wolfe1 = 0.0, wolfe2 = 0.5;
     bool verbose = false;
     double rel_tol = 0.0;   // > 0: also stop when (f_prev - f) / |f| < rel_tol
+    // FP32 objectives: on a stall (line-search failure, or relative decrease
+    // below refine_rel_tol) switch the objective to FP64, re-evaluate and go on
+    bool fp64_refine = true;
+    double refine_rel_tol = 1e-7;
+    bool has_target = false;   // stop once f <= target
+    double target = 0.0;
 };
 
 struct LbfgsResult {
     double f = 0.0, gnorm = 0.0, seconds = 0.0;
-    int iters = 0, evals = 0, status = 0;
+    int iters = 0, evals = 0, status = 0, fp32_iters = 0, reached_target = 0;
     std::vector<double> trace_f, trace_g, trace_step;
     std::vector<int> trace_evals;
 };

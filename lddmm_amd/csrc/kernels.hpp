@@ -1,6 +1,6 @@
 // kernels.hpp -- host launchers of the HIP kernels (all asynchronous on the
 // stream they are given).  Arrays are device pointers in the SoA layout of
-// common.hpp.
+// common.hpp.  `prec` selects kFP64 or kFP32 pair arithmetic.
 #pragma once
 
 #include "common.hpp"
@@ -8,8 +8,12 @@
 namespace lddmm {
 
 // ---- pair sums over points (pair_kernels.hip) ---------------------------
+// Split plans for the point kernels: plan_kv for K.a, plan_single for the
+// backward / dq kernels (one target per lane).
+PairSplit plan_kv(int ni, int nj, int prec, int target_blocks);
+PairSplit plan_single(int ni, int nj, int target_blocks);
 // part[(s*D + c)*ni + i] = sum_{j in chunk s} k(qi_i, qj_j) a_j[c]
-void launch_kv(int dim, int order, const double *qi, int ni, const double *qj, const double *a, int nj,
+void launch_kv(int dim, int order, int prec, const double *qi, int ni, const double *qj, const double *a, int nj,
                double width, const PairSplit &sp, double *part, hipStream_t st);
 // out[c*n + i] = scale * sum_s part[(s*comps + c)*n + i]
 void launch_sum_splits(int comps, int splits, int n, const double *part, double scale, double *out,
@@ -21,19 +25,19 @@ void launch_fwd_reduce(int dim, int splits, int n, const double *part, const dou
 int fwd_reduce_blocks(int n);
 // Fused backward pair pass: part[(s*2D + c)*n + i] for c < D holds
 // sum_j k_ij amp_j[c] (amp = a - p), for c >= D the dq sum of
-// l^T K r with l = amp - a/2, r = a.
-void launch_bwd_pairs(int dim, int order, const double *q, const double *a, const double *p, int n, double width,
-                      const PairSplit &sp, double *part, hipStream_t st);
+// l^T K r with l = amp - a/2, r = a.  Plan: plan_single.
+void launch_bwd_pairs(int dim, int order, int prec, const double *q, const double *a, const double *p, int n,
+                      double width, const PairSplit &sp, double *part, hipStream_t st);
 // g = h * sum_s gpart;  p -= h * sum_s dpart
 void launch_bwd_reduce(int dim, int splits, int n, const double *part, double h, double *g, double *p,
                        hipStream_t st);
-// Self dq: part = sum_j (l_i.r_j + l_j.r_i) c_ij (q_i - q_j)
-void launch_dq_self(int dim, int order, const double *q, const double *l, const double *r, int n, double width,
-                    const PairSplit &sp, double *part, hipStream_t st);
-// Cross dq: part = sum_j (u_i.v_j) c_ij (qi_i - qj_j)
-void launch_dq_cross(int dim, int order, const double *qi, const double *u, int ni, const double *qj,
+// Self dq: part = sum_j (l_i.r_j + l_j.r_i) c_ij (q_i - q_j).  Plan: plan_single.
+void launch_dq_self(int dim, int order, int prec, const double *q, const double *l, const double *r, int n,
+                    double width, const PairSplit &sp, double *part, hipStream_t st);
+// Cross dq: part = sum_j (u_i.v_j) c_ij (qi_i - qj_j).  Plan: plan_single.
+void launch_dq_cross(int dim, int order, int prec, const double *qi, const double *u, int ni, const double *qj,
                      const double *v, int nj, double width, const PairSplit &sp, double *part, hipStream_t st);
-// Dense K[j*ni + i] = k(qi_i, qj_j)
+// Dense K[j*ni + i] = k(qi_i, qj_j), FP64
 void launch_dense_kernel(int dim, int order, const double *qi, int ni, const double *qj, int nj, double width,
                          double *K, hipStream_t st);
 
@@ -45,22 +49,26 @@ struct VfParams {
 void launch_elm_geometry(int dim, const double *q, int nq, const int *elm, int ne, double *cen, double *dir,
                          double *vol, hipStream_t st);
 constexpr int vf_comps(int dim) { return 2 * dim + 2; }
-// Pair pass of the deformed elements i against (deformed, target) sources:
-// partial sums [(s*(2D+2) + c)*ne + i] for s over spd deformed splits then
-// spt target splits.
-void launch_vf_pairs(int dim, const VfParams &vp, const double *dcen, const double *ddir, const double *dvol, int ne,
-                     const double *tcen, const double *tdir, const double *tvol, int nt, const PairSplit &spd,
+PairSplit plan_vf(int ni, int nj, int target_blocks);
+// Pair pass of the deformed elements i against the deformed set (splits
+// [0, spd.splits)) and the target set (splits [spd.splits, +spt.splits)):
+// partial sums [(s*(2D+2) + c)*ne + i].
+void launch_vf_pairs(int dim, int prec, const VfParams &vp, const double *dcen, const double *ddir, const double *dvol,
+                     int ne, const double *tcen, const double *tdir, const double *tvol, int nt, const PairSplit &spd,
                      const PairSplit &spt, double *part, hipStream_t st);
-// Per element: energy partials per block into eparts, dc and dd.
-void launch_vf_finalize(int dim, const VfParams &vp, int splits, int ne, const double *part, const double *ddir,
-                        const double *dvol, double *dc, double *dd, double *eparts, hipStream_t st);
+// Per element: energy partials per block into eparts, dc and dd; the
+// first `sd` splits are deformed-deformed, the rest deformed-target.
+void launch_vf_finalize(int dim, const VfParams &vp, int sd, int splits, int ne, const double *part,
+                        const double *ddir, const double *dvol, double *dc, double *dd, double *eparts,
+                        hipStream_t st);
 int vf_finalize_blocks(int ne);
-// Value-only pair sum sum_ij kc ku a_i a_j of a set with itself (target-target)
+// Value-only pair sum sum_j kc ku a_j of a set with itself (target-target), FP64
 void launch_vf_value_pairs(int dim, const VfParams &vp, const double *cen, const double *dir, const double *vol,
                            int n, const PairSplit &sp, double *part, hipStream_t st);
 // per-block partials of sum_i a_i * sum_s part[s][i]
 void launch_vf_value_finalize(int splits, int n, const double *part, const double *vol, double *eparts,
                               hipStream_t st);
+int vf_value_finalize_blocks(int n);
 // Vertex gather: out = scale * dq with dq from dc, dd through the adjacency table.
 void launch_vf_gather(int dim, const double *q, int nq, const int *ifo, int ifo_rows, const double *dc,
                       const double *dd, int ne, double scale, double *out, hipStream_t st);

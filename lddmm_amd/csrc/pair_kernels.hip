@@ -1,282 +1,296 @@
 // pair_kernels.hip -- all-pairs kernel sums of the LDDMM flow on gfx950.
 //
-// Every kernel here is a tiled all-pairs sum: a block of kBlock (=256,
-// four wave64) threads owns kBlock target points i and one chunk of the
-// source points j.  Sources are staged through LDS in tiles of kTileJ SoA
-// coordinates; all 64 lanes of a wave read the same LDS word (broadcast,
-// no bank conflicts) while each lane keeps its own target in registers.
-// The source chunks (j-splits) give the chip enough workgroups at the
-// configs' sizes (N = 10,242 -> 41 target tiles only); the per-split
-// partial sums are reduced in a fixed order by a second, HBM-light kernel,
-// so every result is bitwise repeatable.
-//
-// Reference semantics: multiplyKernel (Cpp/src/multiplyKernel.cpp:22-564),
-// dqKernel (Cpp/src/dqKernel.cpp:31-591), computeKernel
-// (Cpp/src/computeKernel.cpp:21-470) and the Euler / adjoint steps of
-// objgrd (Cpp/src/objgrd.cpp:45-87).
+// Every sum here runs through the tiled driver of pair_driver.hpp; this
+// file supplies the per-pair operators and the split reductions that
+// finish them.  Reference semantics: multiplyKernel
+// (Cpp/src/multiplyKernel.cpp:22-564), dqKernel (Cpp/src/dqKernel.cpp:31-591),
+// computeKernel (Cpp/src/computeKernel.cpp:21-470) and the Euler / adjoint
+// steps of objgrd (Cpp/src/objgrd.cpp:45-87).
 #include "kernels.hpp"
+#include "pair_driver.hpp"
 
 namespace lddmm {
 
 namespace {
 
-template <int D>
-__device__ __forceinline__ void load_pt(double (&x)[D], const double *__restrict__ p, int n, int i)
-{
-#pragma unroll
-    for (int c = 0; c < D; ++c) x[c] = p[(size_t)c * n + i];
-}
-
 // ---------------------------------------------------------------------------
-// v_i = sum_j k(qi_i, qj_j) a_j      (forward flow and K.alpha operators)
+// v_i = sum_j k(qi_i, qj_j) a_j   (forward flow, multiplyKernel)
 // ---------------------------------------------------------------------------
-template <int D, int ORDER>
-__global__ __launch_bounds__(kBlock) void k_kv(const double *__restrict__ qi, int ni,
-                                               const double *__restrict__ qj, const double *__restrict__ a,
-                                               int nj, int chunk, int tiles, double inv_w, double inv_w2,
-                                               double *__restrict__ part)
-{
-    __shared__ double sq[D][kTileJ];
-    __shared__ double sa[D][kTileJ];
-    const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const int tile = L % tiles, s = L / tiles;
-    const int i = tile * kBlock + threadIdx.x;
-    const int j0 = s * chunk, j1 = min(nj, j0 + chunk);
-    double x[D], acc[D];
+template <int D_, int ORDER, class P, int R_>
+struct OpKV {
+    static constexpr int D = D_, F = D_, OUT = D_, R = R_;
+    using real = typename P::real;
+    using lds_t = real;
+    struct Args {
+        const double *qi, *qj, *a;
+        int ni, nj;
+        real iw, iw2;
+    };
+    struct Tgt {
+        double x[D];
+    };
+    struct Acc {
+        real v[D];
+    };
+    __device__ __forceinline__ static void load_src(const Args &A, int j, double (&x)[D], lds_t (&f)[F])
+    {
 #pragma unroll
-    for (int c = 0; c < D; ++c) {
-        x[c] = i < ni ? qi[(size_t)c * ni + i] : 0.0;
-        acc[c] = 0.0;
-    }
-    for (int jt = j0; jt < j1; jt += kTileJ) {
-        const int cnt = min(kTileJ, j1 - jt);
-        __syncthreads();
-        if ((int)threadIdx.x < cnt) {
-#pragma unroll
-            for (int c = 0; c < D; ++c) {
-                sq[c][threadIdx.x] = qj[(size_t)c * nj + jt + threadIdx.x];
-                sa[c][threadIdx.x] = a[(size_t)c * nj + jt + threadIdx.x];
-            }
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int t = 0; t < cnt; ++t) {
-            double r2 = 0.0;
-#pragma unroll
-            for (int c = 0; c < D; ++c) {
-                double df = x[c] - sq[c][t];
-                r2 += df * df;
-            }
-            double k = KernelFn<ORDER>::value(r2, inv_w, inv_w2);
-#pragma unroll
-            for (int c = 0; c < D; ++c) acc[c] += k * sa[c][t];
+        for (int c = 0; c < D; ++c) {
+            x[c] = A.qj[(size_t)c * A.nj + j];
+            f[c] = (real)A.a[(size_t)c * A.nj + j];
         }
     }
-    if (i < ni) {
+    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t)
+    {
 #pragma unroll
-        for (int c = 0; c < D; ++c) part[((size_t)s * D + c) * ni + i] = acc[c];
+        for (int c = 0; c < D; ++c) t.x[c] = i >= 0 ? A.qi[(size_t)c * A.ni + i] : 0.0;
     }
-}
+    __device__ __forceinline__ static void zero(Acc &a)
+    {
+#pragma unroll
+        for (int c = 0; c < D; ++c) a.v[c] = real(0);
+    }
+    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const double (&xj)[D],
+                                                const lds_t (&fj)[F], Acc &acc)
+    {
+        real r2 = real(0);
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            real df = (real)(t.x[c] - xj[c]);
+            r2 += df * df;
+        }
+        real k = KernelFnT<ORDER, real>::value(r2, A.iw, A.iw2);
+#pragma unroll
+        for (int c = 0; c < D; ++c) acc.v[c] += k * fj[c];
+    }
+    __device__ __forceinline__ static void flush(const Acc &a, double (&tot)[OUT])
+    {
+#pragma unroll
+        for (int c = 0; c < D; ++c) tot[c] += (double)a.v[c];
+    }
+};
 
 // ---------------------------------------------------------------------------
 // Fused backward pass of the adjoint (Cpp/src/objgrd.cpp:76-84): one kernel
-// evaluation per pair feeds both K(q) amp and the dq sum of l^T K(q) r
-// with amp = a - p, l = amp - a/2, r = a.
+// evaluation per pair feeds K(q) amp and the dq sum of l^T K(q) r with
+// amp = a - p, l = amp - a/2, r = a.  Outputs: [K amp (D), dq (D)].
 // ---------------------------------------------------------------------------
-template <int D, int ORDER>
-__global__ __launch_bounds__(kBlock) void k_bwd(const double *__restrict__ q, const double *__restrict__ a,
-                                                const double *__restrict__ p, int n, int chunk, int tiles,
-                                                double inv_w, double inv_w2, double *__restrict__ part)
-{
-    __shared__ double sq[D][kTileJ];
-    __shared__ double sa[D][kTileJ];   // r_j = a_j
-    __shared__ double sm[D][kTileJ];   // amp_j
-    __shared__ double sl[D][kTileJ];   // l_j
-    const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const int tile = L % tiles, s = L / tiles;
-    const int i = tile * kBlock + threadIdx.x;
-    const int j0 = s * chunk, j1 = min(n, j0 + chunk);
-    double x[D], ai[D], li[D], g[D], pd[D];
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-        const bool in = i < n;
-        x[c] = in ? q[(size_t)c * n + i] : 0.0;
-        ai[c] = in ? a[(size_t)c * n + i] : 0.0;
-        double amp = ai[c] - (in ? p[(size_t)c * n + i] : 0.0);
-        li[c] = amp - 0.5 * ai[c];
-        g[c] = 0.0;
-        pd[c] = 0.0;
-    }
-    for (int jt = j0; jt < j1; jt += kTileJ) {
-        const int cnt = min(kTileJ, j1 - jt);
-        __syncthreads();
-        if ((int)threadIdx.x < cnt) {
-            const size_t jj = jt + threadIdx.x;
-#pragma unroll
-            for (int c = 0; c < D; ++c) {
-                double aj = a[(size_t)c * n + jj];
-                double amp = aj - p[(size_t)c * n + jj];
-                sq[c][threadIdx.x] = q[(size_t)c * n + jj];
-                sa[c][threadIdx.x] = aj;
-                sm[c][threadIdx.x] = amp;
-                sl[c][threadIdx.x] = amp - 0.5 * aj;
-            }
-        }
-        __syncthreads();
-#pragma unroll 2
-        for (int t = 0; t < cnt; ++t) {
-            double df[D], r2 = 0.0;
-#pragma unroll
-            for (int c = 0; c < D; ++c) {
-                df[c] = x[c] - sq[c][t];
-                r2 += df[c] * df[c];
-            }
-            double k, cc;
-            KernelFn<ORDER>::eval(r2, inv_w, inv_w2, k, cc);
-            double lr = 0.0;
-#pragma unroll
-            for (int c = 0; c < D; ++c) lr += li[c] * sa[c][t];
-#pragma unroll
-            for (int c = 0; c < D; ++c) lr += sl[c][t] * ai[c];
-            double f = lr * cc;
-#pragma unroll
-            for (int c = 0; c < D; ++c) {
-                g[c] += k * sm[c][t];
-                pd[c] += f * df[c];
-            }
-        }
-    }
-    if (i < n) {
+template <int D_, int ORDER, class P>
+struct OpBwd {
+    static constexpr int D = D_, F = 3 * D_, OUT = 2 * D_, R = 1;
+    using real = typename P::real;
+    using lds_t = real;
+    struct Args {
+        const double *q, *a, *p;
+        int n;
+        real iw, iw2;
+    };
+    struct Tgt {
+        double x[D];
+        real a[D], l[D];
+    };
+    struct Acc {
+        real g[D], pd[D];
+    };
+    __device__ __forceinline__ static void load_src(const Args &A, int j, double (&x)[D], lds_t (&f)[F])
+    {
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-            part[((size_t)s * 2 * D + c) * n + i] = g[c];
-            part[((size_t)s * 2 * D + D + c) * n + i] = pd[c];
+            const size_t o = (size_t)c * A.n + j;
+            double aj = A.a[o], amp = aj - A.p[o];
+            x[c] = A.q[o];
+            f[c] = (real)aj;
+            f[D + c] = (real)amp;
+            f[2 * D + c] = (real)(amp - 0.5 * aj);
         }
     }
-}
+    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t)
+    {
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            const size_t o = (size_t)c * A.n + (i >= 0 ? i : 0);
+            double ai = i >= 0 ? A.a[o] : 0.0, amp = ai - (i >= 0 ? A.p[o] : 0.0);
+            t.x[c] = i >= 0 ? A.q[o] : 0.0;
+            t.a[c] = (real)ai;
+            t.l[c] = (real)(amp - 0.5 * ai);
+        }
+    }
+    __device__ __forceinline__ static void zero(Acc &a)
+    {
+#pragma unroll
+        for (int c = 0; c < D; ++c) a.g[c] = a.pd[c] = real(0);
+    }
+    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const double (&xj)[D],
+                                                const lds_t (&fj)[F], Acc &acc)
+    {
+        real df[D], r2 = real(0);
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            df[c] = (real)(t.x[c] - xj[c]);
+            r2 += df[c] * df[c];
+        }
+        real k, cc;
+        KernelFnT<ORDER, real>::eval(r2, A.iw, A.iw2, k, cc);
+        real lr = real(0);
+#pragma unroll
+        for (int c = 0; c < D; ++c) lr += t.l[c] * fj[c];
+#pragma unroll
+        for (int c = 0; c < D; ++c) lr += fj[2 * D + c] * t.a[c];
+        const real f = lr * cc;
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            acc.g[c] += k * fj[D + c];
+            acc.pd[c] += f * df[c];
+        }
+    }
+    __device__ __forceinline__ static void flush(const Acc &a, double (&tot)[OUT])
+    {
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            tot[c] += (double)a.g[c];
+            tot[D + c] += (double)a.pd[c];
+        }
+    }
+};
 
 // ---------------------------------------------------------------------------
 // dq of l^T K(q) r over one point set (Cpp/src/dqKernel.cpp:31-305)
 // ---------------------------------------------------------------------------
-template <int D, int ORDER>
-__global__ __launch_bounds__(kBlock) void k_dq_self(const double *__restrict__ q, const double *__restrict__ l,
-                                                    const double *__restrict__ r, int n, int chunk, int tiles,
-                                                    double inv_w, double inv_w2, double *__restrict__ part)
-{
-    __shared__ double sq[D][kTileJ];
-    __shared__ double sl[D][kTileJ];
-    __shared__ double sr[D][kTileJ];
-    const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const int tile = L % tiles, s = L / tiles;
-    const int i = tile * kBlock + threadIdx.x;
-    const int j0 = s * chunk, j1 = min(n, j0 + chunk);
-    double x[D], li[D], ri[D], pd[D];
+template <int D_, int ORDER, class P>
+struct OpDqSelf {
+    static constexpr int D = D_, F = 2 * D_, OUT = D_, R = 1;
+    using real = typename P::real;
+    using lds_t = real;
+    struct Args {
+        const double *q, *l, *r;
+        int n;
+        real iw, iw2;
+    };
+    struct Tgt {
+        double x[D];
+        real l[D], r[D];
+    };
+    struct Acc {
+        real pd[D];
+    };
+    __device__ __forceinline__ static void load_src(const Args &A, int j, double (&x)[D], lds_t (&f)[F])
+    {
 #pragma unroll
-    for (int c = 0; c < D; ++c) {
-        const bool in = i < n;
-        x[c] = in ? q[(size_t)c * n + i] : 0.0;
-        li[c] = in ? l[(size_t)c * n + i] : 0.0;
-        ri[c] = in ? r[(size_t)c * n + i] : 0.0;
-        pd[c] = 0.0;
-    }
-    for (int jt = j0; jt < j1; jt += kTileJ) {
-        const int cnt = min(kTileJ, j1 - jt);
-        __syncthreads();
-        if ((int)threadIdx.x < cnt) {
-            const size_t jj = jt + threadIdx.x;
-#pragma unroll
-            for (int c = 0; c < D; ++c) {
-                sq[c][threadIdx.x] = q[(size_t)c * n + jj];
-                sl[c][threadIdx.x] = l[(size_t)c * n + jj];
-                sr[c][threadIdx.x] = r[(size_t)c * n + jj];
-            }
-        }
-        __syncthreads();
-#pragma unroll 2
-        for (int t = 0; t < cnt; ++t) {
-            double df[D], r2 = 0.0;
-#pragma unroll
-            for (int c = 0; c < D; ++c) {
-                df[c] = x[c] - sq[c][t];
-                r2 += df[c] * df[c];
-            }
-            double k, cc;
-            KernelFn<ORDER>::eval(r2, inv_w, inv_w2, k, cc);
-            double lr = 0.0;
-#pragma unroll
-            for (int c = 0; c < D; ++c) lr += li[c] * sr[c][t];
-#pragma unroll
-            for (int c = 0; c < D; ++c) lr += sl[c][t] * ri[c];
-            double f = lr * cc;
-#pragma unroll
-            for (int c = 0; c < D; ++c) pd[c] += f * df[c];
+        for (int c = 0; c < D; ++c) {
+            const size_t o = (size_t)c * A.n + j;
+            x[c] = A.q[o];
+            f[c] = (real)A.l[o];
+            f[D + c] = (real)A.r[o];
         }
     }
-    if (i < n) {
+    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t)
+    {
 #pragma unroll
-        for (int c = 0; c < D; ++c) part[((size_t)s * D + c) * n + i] = pd[c];
+        for (int c = 0; c < D; ++c) {
+            const size_t o = (size_t)c * A.n + (i >= 0 ? i : 0);
+            t.x[c] = i >= 0 ? A.q[o] : 0.0;
+            t.l[c] = i >= 0 ? (real)A.l[o] : real(0);
+            t.r[c] = i >= 0 ? (real)A.r[o] : real(0);
+        }
     }
-}
+    __device__ __forceinline__ static void zero(Acc &a)
+    {
+#pragma unroll
+        for (int c = 0; c < D; ++c) a.pd[c] = real(0);
+    }
+    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const double (&xj)[D],
+                                                const lds_t (&fj)[F], Acc &acc)
+    {
+        real df[D], r2 = real(0);
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            df[c] = (real)(t.x[c] - xj[c]);
+            r2 += df[c] * df[c];
+        }
+        real k, cc;
+        KernelFnT<ORDER, real>::eval(r2, A.iw, A.iw2, k, cc);
+        real lr = real(0);
+#pragma unroll
+        for (int c = 0; c < D; ++c) lr += t.l[c] * fj[D + c];
+#pragma unroll
+        for (int c = 0; c < D; ++c) lr += fj[c] * t.r[c];
+        const real f = lr * cc;
+#pragma unroll
+        for (int c = 0; c < D; ++c) acc.pd[c] += f * df[c];
+    }
+    __device__ __forceinline__ static void flush(const Acc &a, double (&tot)[OUT])
+    {
+#pragma unroll
+        for (int c = 0; c < D; ++c) tot[c] += (double)a.pd[c];
+    }
+};
 
 // ---------------------------------------------------------------------------
 // Cross dq: sum_j (u_i.v_j) c_ij (qi_i - qj_j) (Cpp/src/dqKernel.cpp:307-591;
 // the qj-side gradient is the same sum with the roles of the sets swapped)
 // ---------------------------------------------------------------------------
-template <int D, int ORDER>
-__global__ __launch_bounds__(kBlock) void k_dq_cross(const double *__restrict__ qi, const double *__restrict__ u,
-                                                     int ni, const double *__restrict__ qj,
-                                                     const double *__restrict__ v, int nj, int chunk, int tiles,
-                                                     double inv_w, double inv_w2, double *__restrict__ part)
-{
-    __shared__ double sq[D][kTileJ];
-    __shared__ double sv[D][kTileJ];
-    const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const int tile = L % tiles, s = L / tiles;
-    const int i = tile * kBlock + threadIdx.x;
-    const int j0 = s * chunk, j1 = min(nj, j0 + chunk);
-    double x[D], ui[D], pd[D];
+template <int D_, int ORDER, class P>
+struct OpDqCross {
+    static constexpr int D = D_, F = D_, OUT = D_, R = 1;
+    using real = typename P::real;
+    using lds_t = real;
+    struct Args {
+        const double *qi, *u, *qj, *v;
+        int ni, nj;
+        real iw, iw2;
+    };
+    struct Tgt {
+        double x[D];
+        real u[D];
+    };
+    struct Acc {
+        real pd[D];
+    };
+    __device__ __forceinline__ static void load_src(const Args &A, int j, double (&x)[D], lds_t (&f)[F])
+    {
 #pragma unroll
-    for (int c = 0; c < D; ++c) {
-        const bool in = i < ni;
-        x[c] = in ? qi[(size_t)c * ni + i] : 0.0;
-        ui[c] = in ? u[(size_t)c * ni + i] : 0.0;
-        pd[c] = 0.0;
-    }
-    for (int jt = j0; jt < j1; jt += kTileJ) {
-        const int cnt = min(kTileJ, j1 - jt);
-        __syncthreads();
-        if ((int)threadIdx.x < cnt) {
-            const size_t jj = jt + threadIdx.x;
-#pragma unroll
-            for (int c = 0; c < D; ++c) {
-                sq[c][threadIdx.x] = qj[(size_t)c * nj + jj];
-                sv[c][threadIdx.x] = v[(size_t)c * nj + jj];
-            }
-        }
-        __syncthreads();
-#pragma unroll 2
-        for (int t = 0; t < cnt; ++t) {
-            double df[D], r2 = 0.0;
-#pragma unroll
-            for (int c = 0; c < D; ++c) {
-                df[c] = x[c] - sq[c][t];
-                r2 += df[c] * df[c];
-            }
-            double k, cc;
-            KernelFn<ORDER>::eval(r2, inv_w, inv_w2, k, cc);
-            double lr = 0.0;
-#pragma unroll
-            for (int c = 0; c < D; ++c) lr += ui[c] * sv[c][t];
-            double f = lr * cc;
-#pragma unroll
-            for (int c = 0; c < D; ++c) pd[c] += f * df[c];
+        for (int c = 0; c < D; ++c) {
+            x[c] = A.qj[(size_t)c * A.nj + j];
+            f[c] = (real)A.v[(size_t)c * A.nj + j];
         }
     }
-    if (i < ni) {
+    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t)
+    {
 #pragma unroll
-        for (int c = 0; c < D; ++c) part[((size_t)s * D + c) * ni + i] = pd[c];
+        for (int c = 0; c < D; ++c) {
+            t.x[c] = i >= 0 ? A.qi[(size_t)c * A.ni + i] : 0.0;
+            t.u[c] = i >= 0 ? (real)A.u[(size_t)c * A.ni + i] : real(0);
+        }
     }
-}
+    __device__ __forceinline__ static void zero(Acc &a)
+    {
+#pragma unroll
+        for (int c = 0; c < D; ++c) a.pd[c] = real(0);
+    }
+    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const double (&xj)[D],
+                                                const lds_t (&fj)[F], Acc &acc)
+    {
+        real df[D], r2 = real(0);
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            df[c] = (real)(t.x[c] - xj[c]);
+            r2 += df[c] * df[c];
+        }
+        real k, cc;
+        KernelFnT<ORDER, real>::eval(r2, A.iw, A.iw2, k, cc);
+        real lr = real(0);
+#pragma unroll
+        for (int c = 0; c < D; ++c) lr += t.u[c] * fj[c];
+        const real f = lr * cc;
+#pragma unroll
+        for (int c = 0; c < D; ++c) acc.pd[c] += f * df[c];
+    }
+    __device__ __forceinline__ static void flush(const Acc &a, double (&tot)[OUT])
+    {
+#pragma unroll
+        for (int c = 0; c < D; ++c) tot[c] += (double)a.pd[c];
+    }
+};
 
 // ---------------------------------------------------------------------------
 // Dense kernel matrix (Cpp/src/computeKernel.cpp:246-470 layout K[j*ni + i])
@@ -289,7 +303,8 @@ __global__ __launch_bounds__(kBlock) void k_dense(const double *__restrict__ qi,
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= ni) return;
     double x[D];
-    load_pt<D>(x, qi, ni, i);
+#pragma unroll
+    for (int c = 0; c < D; ++c) x[c] = qi[(size_t)c * ni + i];
     for (int j = blockIdx.y; j < nj; j += gridDim.y) {
         double r2 = 0.0;
 #pragma unroll
@@ -302,8 +317,34 @@ __global__ __launch_bounds__(kBlock) void k_dense(const double *__restrict__ qi,
 }
 
 // ---------------------------------------------------------------------------
-// split reductions
+// split reductions (64-thread blocks: one target per lane, the split loop
+// unrolled so several loads are in flight; the order of the sum is fixed)
 // ---------------------------------------------------------------------------
+constexpr int kRedBlock = 64;
+
+template <int NC>
+__device__ __forceinline__ void sum_splits(const double *__restrict__ part, int splits, int n, int i,
+                                           double (&v)[NC])
+{
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] = 0.0;
+    int k = 0;
+    for (; k + 4 <= splits; k += 4) {
+        double t[4][NC];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) t[u][c] = part[((size_t)(k + u) * NC + c) * n + i];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) v[c] += t[u][c];
+    }
+    for (; k < splits; ++k)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) v[c] += part[((size_t)k * NC + c) * n + i];
+}
+
 __global__ __launch_bounds__(kBlock) void k_sum_splits(int comps, int splits, int n,
                                                        const double *__restrict__ part, double scale,
                                                        double *__restrict__ out)
@@ -316,60 +357,44 @@ __global__ __launch_bounds__(kBlock) void k_sum_splits(int comps, int splits, in
     out[(size_t)c * n + i] = scale * s;
 }
 
-// fixed-shape tree over the 256 threads of a block; result valid in thread 0
-__device__ __forceinline__ double block_sum(double v, double *red)
-{
-    red[threadIdx.x] = v;
-    __syncthreads();
-#pragma unroll
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    return red[0];
-}
-
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_fwd_reduce(int splits, int n, const double *__restrict__ part,
-                                                       const double *__restrict__ q,
-                                                       const double *__restrict__ a, double h,
-                                                       double *__restrict__ qnext, double *__restrict__ vout,
-                                                       double *__restrict__ eparts)
+__global__ __launch_bounds__(kRedBlock) void k_fwd_reduce(int splits, int n, const double *__restrict__ part,
+                                                          const double *__restrict__ q,
+                                                          const double *__restrict__ a, double h,
+                                                          double *__restrict__ qnext, double *__restrict__ vout,
+                                                          double *__restrict__ eparts)
 {
-    __shared__ double red[kBlock];
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = blockIdx.x * kRedBlock + threadIdx.x;
     double e = 0.0;
     if (i < n) {
+        double v[D];
+        sum_splits<D>(part, splits, n, i, v);
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-            double v = 0.0;
-            for (int k = 0; k < splits; ++k) v += part[((size_t)k * D + c) * n + i];
             const size_t o = (size_t)c * n + i;
-            e += a[o] * v;
-            qnext[o] = q[o] + h * v;
-            if (vout) vout[o] = v;
+            e += a[o] * v[c];
+            qnext[o] = q[o] + h * v[c];
+            if (vout) vout[o] = v[c];
         }
     }
-    double t = block_sum(e, red);
-    if (threadIdx.x == 0) eparts[blockIdx.x] = t;
+    e = wave_sum(e);
+    if (threadIdx.x == 0) eparts[blockIdx.x] = e;
 }
 
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_bwd_reduce(int splits, int n, const double *__restrict__ part,
-                                                       double h, double *__restrict__ g, double *__restrict__ p)
+__global__ __launch_bounds__(kRedBlock) void k_bwd_reduce(int splits, int n, const double *__restrict__ part,
+                                                          double h, double *__restrict__ g,
+                                                          double *__restrict__ p)
 {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = blockIdx.x * kRedBlock + threadIdx.x;
     if (i >= n) return;
+    double v[2 * D];
+    sum_splits<2 * D>(part, splits, n, i, v);
 #pragma unroll
     for (int c = 0; c < D; ++c) {
-        double gs = 0.0, ps = 0.0;
-        for (int k = 0; k < splits; ++k) {
-            gs += part[((size_t)k * 2 * D + c) * n + i];
-            ps += part[((size_t)k * 2 * D + D + c) * n + i];
-        }
         const size_t o = (size_t)c * n + i;
-        g[o] = h * gs;
-        p[o] = p[o] - h * ps;
+        g[o] = h * v[c];
+        p[o] = p[o] - h * v[D + c];
     }
 }
 
@@ -381,27 +406,44 @@ void check_dim_order(int dim, int order)
         throw Error(kErrUnsupported, "2D Matern kernels (Bessel K0/K1) are not implemented on the GPU path");
 }
 
+// f(Dim, Order, Prec)
 template <typename F>
-void dispatch_dim_order(int dim, int order, F &&f)
+void dispatch_all(int dim, int order, int prec, F &&f)
 {
     check_dim_order(dim, order);
+    if (prec != kFP64 && prec != kFP32) throw Error(kErrArg, "unknown precision policy");
     dispatch_order(order, [&](auto o) {
-        if (dim == 2) f(std::integral_constant<int, 2>{}, o);
-        else f(std::integral_constant<int, 3>{}, o);
+        auto with_prec = [&](auto d) {
+            if (prec == kFP32) f(d, o, PrecF32{});
+            else f(d, o, PrecF64{});
+        };
+        if (dim == 2) with_prec(std::integral_constant<int, 2>{});
+        else with_prec(std::integral_constant<int, 3>{});
     });
 }
 
 }  // namespace
 
-void launch_kv(int dim, int order, const double *qi, int ni, const double *qj, const double *a, int nj,
+int kv_targets_per_block(int prec) { return prec == kFP32 ? 128 : 64; }
+
+PairSplit plan_kv(int ni, int nj, int prec, int target_blocks)
+{
+    return make_split_ti(ni, nj, kv_targets_per_block(prec), target_blocks);
+}
+
+PairSplit plan_single(int ni, int nj, int target_blocks) { return make_split_ti(ni, nj, 64, target_blocks); }
+
+void launch_kv(int dim, int order, int prec, const double *qi, int ni, const double *qj, const double *a, int nj,
                double width, const PairSplit &sp, double *part, hipStream_t st)
 {
     if (ni <= 0) return;
-    dispatch_dim_order(dim, order, [&](auto D, auto O) {
-        k_kv<D(), O()><<<sp.blocks(), kBlock, 0, st>>>(qi, ni, qj, a, nj, sp.chunk, sp.tiles, 1.0 / width,
-                                                       1.0 / (width * width), part);
+    dispatch_all(dim, order, prec, [&](auto Dc, auto Oc, auto Pc) {
+        constexpr int R = decltype(Pc)::fp32 ? 2 : 1;
+        using Op = OpKV<Dc(), Oc(), decltype(Pc), R>;
+        using real = typename Op::real;
+        typename Op::Args A{qi, qj, a, ni, nj, (real)(1.0 / width), (real)(1.0 / (width * width))};
+        launch_pairs<Op>(A, ni, nj, sp, 0, part, st);
     });
-    LDDMM_LAUNCH_CHECK();
 }
 
 void launch_sum_splits(int comps, int splits, int n, const double *part, double scale, double *out,
@@ -413,64 +455,67 @@ void launch_sum_splits(int comps, int splits, int n, const double *part, double 
     LDDMM_LAUNCH_CHECK();
 }
 
-int fwd_reduce_blocks(int n) { return ceil_div(n, kBlock); }
+int fwd_reduce_blocks(int n) { return ceil_div(n, kRedBlock); }
 
 void launch_fwd_reduce(int dim, int splits, int n, const double *part, const double *q, const double *a,
                        double h, double *qnext, double *vout, double *eparts, hipStream_t st)
 {
     int nb = fwd_reduce_blocks(n);
-    if (dim == 2) k_fwd_reduce<2><<<nb, kBlock, 0, st>>>(splits, n, part, q, a, h, qnext, vout, eparts);
-    else k_fwd_reduce<3><<<nb, kBlock, 0, st>>>(splits, n, part, q, a, h, qnext, vout, eparts);
+    if (dim == 2) k_fwd_reduce<2><<<nb, kRedBlock, 0, st>>>(splits, n, part, q, a, h, qnext, vout, eparts);
+    else k_fwd_reduce<3><<<nb, kRedBlock, 0, st>>>(splits, n, part, q, a, h, qnext, vout, eparts);
     LDDMM_LAUNCH_CHECK();
 }
 
-void launch_bwd_pairs(int dim, int order, const double *q, const double *a, const double *p, int n, double width,
-                      const PairSplit &sp, double *part, hipStream_t st)
+void launch_bwd_pairs(int dim, int order, int prec, const double *q, const double *a, const double *p, int n,
+                      double width, const PairSplit &sp, double *part, hipStream_t st)
 {
-    dispatch_dim_order(dim, order, [&](auto D, auto O) {
-        k_bwd<D(), O()><<<sp.blocks(), kBlock, 0, st>>>(q, a, p, n, sp.chunk, sp.tiles, 1.0 / width,
-                                                        1.0 / (width * width), part);
+    dispatch_all(dim, order, prec, [&](auto Dc, auto Oc, auto Pc) {
+        using Op = OpBwd<Dc(), Oc(), decltype(Pc)>;
+        using real = typename Op::real;
+        typename Op::Args A{q, a, p, n, (real)(1.0 / width), (real)(1.0 / (width * width))};
+        launch_pairs<Op>(A, n, n, sp, 0, part, st);
     });
-    LDDMM_LAUNCH_CHECK();
 }
 
 void launch_bwd_reduce(int dim, int splits, int n, const double *part, double h, double *g, double *p,
                        hipStream_t st)
 {
-    int nb = ceil_div(n, kBlock);
-    if (dim == 2) k_bwd_reduce<2><<<nb, kBlock, 0, st>>>(splits, n, part, h, g, p);
-    else k_bwd_reduce<3><<<nb, kBlock, 0, st>>>(splits, n, part, h, g, p);
+    int nb = ceil_div(n, kRedBlock);
+    if (dim == 2) k_bwd_reduce<2><<<nb, kRedBlock, 0, st>>>(splits, n, part, h, g, p);
+    else k_bwd_reduce<3><<<nb, kRedBlock, 0, st>>>(splits, n, part, h, g, p);
     LDDMM_LAUNCH_CHECK();
 }
 
-void launch_dq_self(int dim, int order, const double *q, const double *l, const double *r, int n, double width,
-                    const PairSplit &sp, double *part, hipStream_t st)
+void launch_dq_self(int dim, int order, int prec, const double *q, const double *l, const double *r, int n,
+                    double width, const PairSplit &sp, double *part, hipStream_t st)
 {
-    dispatch_dim_order(dim, order, [&](auto D, auto O) {
-        k_dq_self<D(), O()><<<sp.blocks(), kBlock, 0, st>>>(q, l, r, n, sp.chunk, sp.tiles, 1.0 / width,
-                                                            1.0 / (width * width), part);
+    dispatch_all(dim, order, prec, [&](auto Dc, auto Oc, auto Pc) {
+        using Op = OpDqSelf<Dc(), Oc(), decltype(Pc)>;
+        using real = typename Op::real;
+        typename Op::Args A{q, l, r, n, (real)(1.0 / width), (real)(1.0 / (width * width))};
+        launch_pairs<Op>(A, n, n, sp, 0, part, st);
     });
-    LDDMM_LAUNCH_CHECK();
 }
 
-void launch_dq_cross(int dim, int order, const double *qi, const double *u, int ni, const double *qj,
+void launch_dq_cross(int dim, int order, int prec, const double *qi, const double *u, int ni, const double *qj,
                      const double *v, int nj, double width, const PairSplit &sp, double *part, hipStream_t st)
 {
     if (ni <= 0) return;
-    dispatch_dim_order(dim, order, [&](auto D, auto O) {
-        k_dq_cross<D(), O()><<<sp.blocks(), kBlock, 0, st>>>(qi, u, ni, qj, v, nj, sp.chunk, sp.tiles,
-                                                             1.0 / width, 1.0 / (width * width), part);
+    dispatch_all(dim, order, prec, [&](auto Dc, auto Oc, auto Pc) {
+        using Op = OpDqCross<Dc(), Oc(), decltype(Pc)>;
+        using real = typename Op::real;
+        typename Op::Args A{qi, u, qj, v, ni, nj, (real)(1.0 / width), (real)(1.0 / (width * width))};
+        launch_pairs<Op>(A, ni, nj, sp, 0, part, st);
     });
-    LDDMM_LAUNCH_CHECK();
 }
 
 void launch_dense_kernel(int dim, int order, const double *qi, int ni, const double *qj, int nj, double width,
                          double *K, hipStream_t st)
 {
     if (ni <= 0 || nj <= 0) return;
-    dispatch_dim_order(dim, order, [&](auto D, auto O) {
+    dispatch_all(dim, order, kFP64, [&](auto Dc, auto Oc, auto) {
         dim3 grid(ceil_div(ni, kBlock), nj < 65535 ? nj : 65535);
-        k_dense<D(), O()><<<grid, kBlock, 0, st>>>(qi, ni, qj, nj, 1.0 / width, 1.0 / (width * width), K);
+        k_dense<Dc(), Oc()><<<grid, kBlock, 0, st>>>(qi, ni, qj, nj, 1.0 / width, 1.0 / (width * width), K);
     });
     LDDMM_LAUNCH_CHECK();
 }

@@ -8,18 +8,21 @@
 // kernels ku(ang = u_i.u_j): 'B' ang^2, 'L' ang, 'O' exp(2(ang-1)/s^2),
 // 'U' exp(2(ang^2-1)/s^2) (Cpp/src/varifold3D.cpp:41-188).
 //
-// The pair pass accumulates, per deformed element i and in the same tiled
-// j-split scheme as the point kernels, five sums from which the
-// reference's per-pair gradient terms factor out exactly:
+// The pair pass (the tiled driver of pair_driver.hpp) accumulates, per
+// deformed element i, four sums from which the reference's per-pair
+// gradient terms factor out exactly:
 //   E   = sum_j kc ku a_j                  -> energy a_i E
 //   G   = sum_j kc^p ku a_j (c_i - c_j)    -> dc_i = -4 a_i G / s^2 (p = 1 'G', 2 'C')
 //   U   = sum_j kc a_j ku'(ang) u_j
 //   S   = sum_j kc a_j (ku - ku'(ang) ang) -> dd_i = 2 dv (U + S u_i)
 // with dv = d a / d |n| = 1/2 in 3D (|n| = 2a) and 1 in 2D, which is the
 // reference's (ku' / |n|)(u_j - ang u_i) a_i + ku dv u_i summed over j.
-// The target-target term does not depend on q; it is evaluated once per
-// problem and cached.
+// The deformed-deformed and deformed-target sources are two launches
+// writing disjoint split ranges; the finalize kernel applies the -2 / -1
+// weights of the cross term.  The target-target term does not depend on q;
+// it is evaluated once per problem, in FP64, and cached.
 #include "kernels.hpp"
+#include "pair_driver.hpp"
 
 namespace lddmm {
 
@@ -28,32 +31,173 @@ namespace {
 enum { kCenG = 0, kCenC = 1 };
 enum { kDirB = 0, kDirL = 1, kDirO = 2, kDirU = 3 };
 
-template <int CEN>
-__device__ __forceinline__ double cen_value(double r2, double inv_cs2)
+template <int CEN, typename real>
+__device__ __forceinline__ real cen_value(real r2, real inv_cs2)
 {
-    if constexpr (CEN == kCenG) return exp(-r2 * inv_cs2);
-    else return 1.0 / (1.0 + r2 * inv_cs2);
+    if constexpr (CEN == kCenG) {
+        if constexpr (sizeof(real) == 4) return __builtin_amdgcn_exp2f(-r2 * inv_cs2 * 1.4426950408889634f);
+        else return exp(-r2 * inv_cs2);
+    } else {
+        return real(1) / (real(1) + r2 * inv_cs2);
+    }
 }
 
-template <int DIR>
-__device__ __forceinline__ double dir_value(double ang, double inv_ds2, double &dku)
+template <typename real>
+__device__ __forceinline__ real ex(real x)
+{
+    if constexpr (sizeof(real) == 4) return __builtin_amdgcn_exp2f(x * 1.4426950408889634f);
+    else return exp(x);
+}
+
+template <int DIR, typename real>
+__device__ __forceinline__ real dir_value(real ang, real inv_ds2, real &dku)
 {
     if constexpr (DIR == kDirB) {
-        dku = 2.0 * ang;
+        dku = real(2) * ang;
         return ang * ang;
     } else if constexpr (DIR == kDirL) {
-        dku = 1.0;
+        dku = real(1);
         return ang;
     } else if constexpr (DIR == kDirO) {
-        double f = exp(2.0 * (ang - 1.0) * inv_ds2);
-        dku = 2.0 * f * inv_ds2;
+        real f = ex<real>(real(2) * (ang - real(1)) * inv_ds2);
+        dku = real(2) * f * inv_ds2;
         return f;
     } else {
-        double f = exp(2.0 * (ang * ang - 1.0) * inv_ds2);
-        dku = 4.0 * ang * f * inv_ds2;
+        real f = ex<real>(real(2) * (ang * ang - real(1)) * inv_ds2);
+        dku = real(4) * ang * f * inv_ds2;
         return f;
     }
 }
+
+template <int D_, int CEN, int DIR, class P>
+struct OpVf {
+    static constexpr int D = D_, F = D_ + 1, OUT = 2 * D_ + 2, R = 1;
+    using real = typename P::real;
+    using lds_t = real;
+    struct Args {
+        const double *ci, *ui;   // targets: deformed element centres / directions
+        int ni;
+        const double *cj, *uj, *aj;   // sources
+        int nj;
+        real ics2, ids2;
+    };
+    struct Tgt {
+        double c[D];
+        real u[D];
+    };
+    struct Acc {
+        real e, s, g[D], u[D];
+    };
+    __device__ __forceinline__ static void load_src(const Args &A, int j, double (&x)[D], lds_t (&f)[F])
+    {
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            x[c] = A.cj[(size_t)c * A.nj + j];
+            f[c] = (real)A.uj[(size_t)c * A.nj + j];
+        }
+        f[D] = (real)A.aj[j];
+    }
+    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t)
+    {
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            t.c[c] = i >= 0 ? A.ci[(size_t)c * A.ni + i] : 0.0;
+            t.u[c] = i >= 0 ? (real)A.ui[(size_t)c * A.ni + i] : real(0);
+        }
+    }
+    __device__ __forceinline__ static void zero(Acc &a)
+    {
+        a.e = a.s = real(0);
+#pragma unroll
+        for (int c = 0; c < D; ++c) a.g[c] = a.u[c] = real(0);
+    }
+    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const double (&xj)[D],
+                                                const lds_t (&fj)[F], Acc &acc)
+    {
+        real df[D], r2 = real(0), ang = real(0);
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            df[c] = (real)(t.c[c] - xj[c]);
+            r2 += df[c] * df[c];
+            ang += t.u[c] * fj[c];
+        }
+        const real kc = cen_value<CEN, real>(r2, A.ics2);
+        real dku;
+        const real ku = dir_value<DIR, real>(ang, A.ids2, dku);
+        const real w = kc * fj[D];
+        const real wk = w * ku;
+        const real gw = (CEN == kCenG) ? wk : wk * kc;
+        const real wd = w * dku;
+        acc.e += wk;
+        acc.s += wk - wd * ang;
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            acc.g[c] += gw * df[c];
+            acc.u[c] += wd * fj[c];
+        }
+    }
+    __device__ __forceinline__ static void flush(const Acc &a, double (&tot)[OUT])
+    {
+        tot[0] += (double)a.e;
+        tot[1] += (double)a.s;
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            tot[2 + c] += (double)a.g[c];
+            tot[2 + D + c] += (double)a.u[c];
+        }
+    }
+};
+
+// value-only: sum_j kc ku a_j (target-target term)
+template <int D_, int CEN, int DIR>
+struct OpVfValue {
+    static constexpr int D = D_, F = D_ + 1, OUT = 1, R = 1;
+    using real = double;
+    using lds_t = double;
+    struct Args {
+        const double *c, *u, *a;
+        int n;
+        double ics2, ids2;
+    };
+    struct Tgt {
+        double c[D], u[D];
+    };
+    struct Acc {
+        double e;
+    };
+    __device__ __forceinline__ static void load_src(const Args &A, int j, double (&x)[D], lds_t (&f)[F])
+    {
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            x[c] = A.c[(size_t)c * A.n + j];
+            f[c] = A.u[(size_t)c * A.n + j];
+        }
+        f[D] = A.a[j];
+    }
+    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t)
+    {
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            t.c[c] = i >= 0 ? A.c[(size_t)c * A.n + i] : 0.0;
+            t.u[c] = i >= 0 ? A.u[(size_t)c * A.n + i] : 0.0;
+        }
+    }
+    __device__ __forceinline__ static void zero(Acc &a) { a.e = 0.0; }
+    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const double (&xj)[D],
+                                                const lds_t (&fj)[F], Acc &acc)
+    {
+        double r2 = 0.0, ang = 0.0;
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            double df = t.c[c] - xj[c];
+            r2 += df * df;
+            ang += t.u[c] * fj[c];
+        }
+        double dku;
+        acc.e += cen_value<CEN, double>(r2, A.ics2) * dir_value<DIR, double>(ang, A.ids2, dku) * fj[D];
+    }
+    __device__ __forceinline__ static void flush(const Acc &a, double (&tot)[OUT]) { tot[0] += a.e; }
+};
 
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_elm_geometry(const double *__restrict__ q, int nq,
@@ -95,115 +239,18 @@ __global__ __launch_bounds__(kBlock) void k_elm_geometry(const double *__restric
     }
 }
 
-// Pair pass.  Blocks with s < sd take deformed source chunk s (weight +1),
-// the others target chunk s - sd (energy weight -2, gradient sign -1).
-template <int D, int CEN, int DIR>
-__global__ __launch_bounds__(kBlock) void k_vf_pairs(const double *__restrict__ dcen,
-                                                     const double *__restrict__ ddir,
-                                                     const double *__restrict__ dvol, int ne,
-                                                     const double *__restrict__ tcen,
-                                                     const double *__restrict__ tdir,
-                                                     const double *__restrict__ tvol, int nt, int tiles, int sd,
-                                                     int chunkd, int chunkt, double inv_cs2, double inv_ds2,
-                                                     double *__restrict__ part)
-{
-    constexpr int NC = 2 * D + 2;
-    __shared__ double sc[D][kTileJ];
-    __shared__ double su[D][kTileJ];
-    __shared__ double sa[kTileJ];
-    const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const int tile = L % tiles, s = L / tiles;
-    const bool tgt = s >= sd;
-    const double *__restrict__ c_src = tgt ? tcen : dcen;
-    const double *__restrict__ u_src = tgt ? tdir : ddir;
-    const double *__restrict__ a_src = tgt ? tvol : dvol;
-    const int nj = tgt ? nt : ne;
-    const int j0 = tgt ? (s - sd) * chunkt : s * chunkd;
-    const int j1 = min(nj, j0 + (tgt ? chunkt : chunkd));
-    const int i = tile * kBlock + threadIdx.x;
-    double ci[D], ui[D];
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-        ci[c] = i < ne ? dcen[(size_t)c * ne + i] : 0.0;
-        ui[c] = i < ne ? ddir[(size_t)c * ne + i] : 0.0;
-    }
-    double accE = 0.0, accS = 0.0, accG[D], accU[D];
-#pragma unroll
-    for (int c = 0; c < D; ++c) accG[c] = accU[c] = 0.0;
-
-    for (int jt = j0; jt < j1; jt += kTileJ) {
-        const int cnt = min(kTileJ, j1 - jt);
-        __syncthreads();
-        if ((int)threadIdx.x < cnt) {
-            const size_t jj = jt + threadIdx.x;
-#pragma unroll
-            for (int c = 0; c < D; ++c) {
-                sc[c][threadIdx.x] = c_src[(size_t)c * nj + jj];
-                su[c][threadIdx.x] = u_src[(size_t)c * nj + jj];
-            }
-            sa[threadIdx.x] = a_src[jj];
-        }
-        __syncthreads();
-#pragma unroll 2
-        for (int t = 0; t < cnt; ++t) {
-            double df[D], r2 = 0.0, ang = 0.0;
-#pragma unroll
-            for (int c = 0; c < D; ++c) {
-                df[c] = ci[c] - sc[c][t];
-                r2 += df[c] * df[c];
-                ang += ui[c] * su[c][t];
-            }
-            double kc = cen_value<CEN>(r2, inv_cs2);
-            double dku;
-            double ku = dir_value<DIR>(ang, inv_ds2, dku);
-            double w = kc * sa[t];
-            double wk = w * ku;
-            accE += wk;
-            double gw = (CEN == kCenG) ? wk : wk * kc;
-            double wd = w * dku;
-            accS += w * ku - wd * ang;
-#pragma unroll
-            for (int c = 0; c < D; ++c) {
-                accG[c] += gw * df[c];
-                accU[c] += wd * su[c][t];
-            }
-        }
-    }
-    if (i < ne) {
-        const double se = tgt ? -2.0 : 1.0, sg = tgt ? -1.0 : 1.0;
-        double *o = part + (size_t)s * NC * ne + i;
-        o[0] = se * accE;
-        o[(size_t)1 * ne] = sg * accS;
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            o[(size_t)(2 + c) * ne] = sg * accG[c];
-            o[(size_t)(2 + D + c) * ne] = sg * accU[c];
-        }
-    }
-}
-
-__device__ __forceinline__ double block_sum(double v, double *red)
-{
-    red[threadIdx.x] = v;
-    __syncthreads();
-#pragma unroll
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    return red[0];
-}
+constexpr int kRedBlock = 64;
 
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_vf_finalize(int splits, int ne, const double *__restrict__ part,
-                                                        const double *__restrict__ ddir,
-                                                        const double *__restrict__ dvol, double gscale,
-                                                        double *__restrict__ dc, double *__restrict__ dd,
-                                                        double *__restrict__ eparts)
+__global__ __launch_bounds__(kRedBlock) void k_vf_finalize(int sd, int splits, int ne,
+                                                           const double *__restrict__ part,
+                                                           const double *__restrict__ ddir,
+                                                           const double *__restrict__ dvol, double gscale,
+                                                           double *__restrict__ dc, double *__restrict__ dd,
+                                                           double *__restrict__ eparts)
 {
     constexpr int NC = 2 * D + 2;
-    __shared__ double red[kBlock];
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = blockIdx.x * kRedBlock + threadIdx.x;
     double e = 0.0;
     if (i < ne) {
         double acc[NC];
@@ -211,8 +258,14 @@ __global__ __launch_bounds__(kBlock) void k_vf_finalize(int splits, int ne, cons
         for (int c = 0; c < NC; ++c) acc[c] = 0.0;
         for (int s = 0; s < splits; ++s) {
             const double *o = part + (size_t)s * NC * ne + i;
+            double t[NC];
 #pragma unroll
-            for (int c = 0; c < NC; ++c) acc[c] += o[(size_t)c * ne];
+            for (int c = 0; c < NC; ++c) t[c] = o[(size_t)c * ne];
+            // cross term: energy weight -2, gradient sign -1
+            const double we = s < sd ? 1.0 : -2.0, wg = s < sd ? 1.0 : -1.0;
+            acc[0] += we * t[0];
+#pragma unroll
+            for (int c = 1; c < NC; ++c) acc[c] += wg * t[c];
         }
         const double a = dvol[i];
         const double dv = (D == 3) ? 0.5 : 1.0;
@@ -223,72 +276,23 @@ __global__ __launch_bounds__(kBlock) void k_vf_finalize(int splits, int ne, cons
             dd[(size_t)c * ne + i] = 2.0 * dv * (acc[2 + D + c] + acc[1] * ddir[(size_t)c * ne + i]);
         }
     }
-    double t = block_sum(e, red);
-    if (threadIdx.x == 0) eparts[blockIdx.x] = t;
+    e = wave_sum(e);
+    if (threadIdx.x == 0) eparts[blockIdx.x] = e;
 }
 
-template <int D, int CEN, int DIR>
-__global__ __launch_bounds__(kBlock) void k_vf_value(const double *__restrict__ cen, const double *__restrict__ dir,
-                                                     const double *__restrict__ vol, int n, int chunk, int tiles,
-                                                     double inv_cs2, double inv_ds2, double *__restrict__ part)
+__global__ __launch_bounds__(kRedBlock) void k_vf_value_finalize(int splits, int n, const double *__restrict__ part,
+                                                                 const double *__restrict__ vol,
+                                                                 double *__restrict__ eparts)
 {
-    __shared__ double sc[D][kTileJ];
-    __shared__ double su[D][kTileJ];
-    __shared__ double sa[kTileJ];
-    const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const int tile = L % tiles, s = L / tiles;
-    const int i = tile * kBlock + threadIdx.x;
-    const int j0 = s * chunk, j1 = min(n, j0 + chunk);
-    double ci[D], ui[D];
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-        ci[c] = i < n ? cen[(size_t)c * n + i] : 0.0;
-        ui[c] = i < n ? dir[(size_t)c * n + i] : 0.0;
-    }
-    double acc = 0.0;
-    for (int jt = j0; jt < j1; jt += kTileJ) {
-        const int cnt = min(kTileJ, j1 - jt);
-        __syncthreads();
-        if ((int)threadIdx.x < cnt) {
-            const size_t jj = jt + threadIdx.x;
-#pragma unroll
-            for (int c = 0; c < D; ++c) {
-                sc[c][threadIdx.x] = cen[(size_t)c * n + jj];
-                su[c][threadIdx.x] = dir[(size_t)c * n + jj];
-            }
-            sa[threadIdx.x] = vol[jj];
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int t = 0; t < cnt; ++t) {
-            double r2 = 0.0, ang = 0.0;
-#pragma unroll
-            for (int c = 0; c < D; ++c) {
-                double df = ci[c] - sc[c][t];
-                r2 += df * df;
-                ang += ui[c] * su[c][t];
-            }
-            double dku;
-            acc += cen_value<CEN>(r2, inv_cs2) * dir_value<DIR>(ang, inv_ds2, dku) * sa[t];
-        }
-    }
-    if (i < n) part[(size_t)s * n + i] = acc;
-}
-
-__global__ __launch_bounds__(kBlock) void k_vf_value_finalize(int splits, int n, const double *__restrict__ part,
-                                                              const double *__restrict__ vol,
-                                                              double *__restrict__ eparts)
-{
-    __shared__ double red[kBlock];
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = blockIdx.x * kRedBlock + threadIdx.x;
     double e = 0.0;
     if (i < n) {
         double s = 0.0;
         for (int k = 0; k < splits; ++k) s += part[(size_t)k * n + i];
         e = vol[i] * s;
     }
-    double t = block_sum(e, red);
-    if (threadIdx.x == 0) eparts[blockIdx.x] = t;
+    e = wave_sum(e);
+    if (threadIdx.x == 0) eparts[blockIdx.x] = e;
 }
 
 // Vertex gather (Cpp/src/varifold3D.cpp:359-391, Cpp/src/varifold2D.cpp:218-240):
@@ -351,6 +355,7 @@ int dir_code(char t)
     throw Error(kErrArg, "dirKnlType must be 'B', 'L', 'O' or 'U'");
 }
 
+// f(Dim, Cen, Dir)
 template <typename F>
 void dispatch_vf(int dim, const VfParams &vp, F &&f)
 {
@@ -376,6 +381,8 @@ double inv_sq(double w) { return w != 0.0 ? 1.0 / (w * w) : 0.0; }
 
 }  // namespace
 
+PairSplit plan_vf(int ni, int nj, int target_blocks) { return make_split_ti(ni, nj, 64, target_blocks); }
+
 void launch_elm_geometry(int dim, const double *q, int nq, const int *elm, int ne, double *cen, double *dir,
                          double *vol, hipStream_t st)
 {
@@ -385,32 +392,42 @@ void launch_elm_geometry(int dim, const double *q, int nq, const int *elm, int n
     LDDMM_LAUNCH_CHECK();
 }
 
-void launch_vf_pairs(int dim, const VfParams &vp, const double *dcen, const double *ddir, const double *dvol, int ne,
-                     const double *tcen, const double *tdir, const double *tvol, int nt, const PairSplit &spd,
+void launch_vf_pairs(int dim, int prec, const VfParams &vp, const double *dcen, const double *ddir, const double *dvol,
+                     int ne, const double *tcen, const double *tdir, const double *tvol, int nt, const PairSplit &spd,
                      const PairSplit &spt, double *part, hipStream_t st)
 {
     if (ne <= 0) return;
-    const int sd = spd.splits, stt = nt > 0 ? spt.splits : 0;
-    const int nb = spd.tiles * (sd + stt);
-    dispatch_vf(dim, vp, [&](auto D, auto C, auto U) {
-        k_vf_pairs<D(), C(), U()><<<nb, kBlock, 0, st>>>(dcen, ddir, dvol, ne, tcen, tdir, tvol, nt, spd.tiles, sd,
-                                                         spd.chunk, spt.chunk, inv_sq(vp.cen_width),
-                                                         inv_sq(vp.dir_width), part);
+    if (prec != kFP64 && prec != kFP32) throw Error(kErrArg, "unknown precision policy");
+    dispatch_vf(dim, vp, [&](auto Dc, auto Cc, auto Uc) {
+        auto go = [&](auto Pc) {
+            using Op = OpVf<Dc(), Cc(), Uc(), decltype(Pc)>;
+            using real = typename Op::real;
+            const real ics2 = (real)inv_sq(vp.cen_width), ids2 = (real)inv_sq(vp.dir_width);
+            typename Op::Args dd{dcen, ddir, ne, dcen, ddir, dvol, ne, ics2, ids2};
+            launch_pairs<Op>(dd, ne, ne, spd, 0, part, st);
+            if (nt > 0) {
+                typename Op::Args dt{dcen, ddir, ne, tcen, tdir, tvol, nt, ics2, ids2};
+                launch_pairs<Op>(dt, ne, nt, spt, spd.splits, part, st);
+            }
+        };
+        if (prec == kFP32) go(PrecF32{});
+        else go(PrecF64{});
     });
-    LDDMM_LAUNCH_CHECK();
 }
 
-int vf_finalize_blocks(int ne) { return ceil_div(ne, kBlock); }
+int vf_finalize_blocks(int ne) { return ceil_div(ne, kRedBlock); }
 
-void launch_vf_finalize(int dim, const VfParams &vp, int splits, int ne, const double *part, const double *ddir,
-                        const double *dvol, double *dc, double *dd, double *eparts, hipStream_t st)
+void launch_vf_finalize(int dim, const VfParams &vp, int sd, int splits, int ne, const double *part,
+                        const double *ddir, const double *dvol, double *dc, double *dd, double *eparts,
+                        hipStream_t st)
 {
     if (ne <= 0) return;
     // dc_i = 2 a_i sum_j gk ku a_j (c_i - c_j) with gk = -2 kc^p / s^2
     const double gscale = -4.0 * inv_sq(vp.cen_width);
     const int nb = vf_finalize_blocks(ne);
-    if (dim == 2) k_vf_finalize<2><<<nb, kBlock, 0, st>>>(splits, ne, part, ddir, dvol, gscale, dc, dd, eparts);
-    else k_vf_finalize<3><<<nb, kBlock, 0, st>>>(splits, ne, part, ddir, dvol, gscale, dc, dd, eparts);
+    if (dim == 2)
+        k_vf_finalize<2><<<nb, kRedBlock, 0, st>>>(sd, splits, ne, part, ddir, dvol, gscale, dc, dd, eparts);
+    else k_vf_finalize<3><<<nb, kRedBlock, 0, st>>>(sd, splits, ne, part, ddir, dvol, gscale, dc, dd, eparts);
     LDDMM_LAUNCH_CHECK();
 }
 
@@ -418,19 +435,20 @@ void launch_vf_value_pairs(int dim, const VfParams &vp, const double *cen, const
                            int n, const PairSplit &sp, double *part, hipStream_t st)
 {
     if (n <= 0) return;
-    dispatch_vf(dim, vp, [&](auto D, auto C, auto U) {
-        k_vf_value<D(), C(), U()><<<sp.blocks(), kBlock, 0, st>>>(cen, dir, vol, n, sp.chunk, sp.tiles,
-                                                                  inv_sq(vp.cen_width), inv_sq(vp.dir_width),
-                                                                  part);
+    dispatch_vf(dim, vp, [&](auto Dc, auto Cc, auto Uc) {
+        using Op = OpVfValue<Dc(), Cc(), Uc()>;
+        typename Op::Args A{cen, dir, vol, n, inv_sq(vp.cen_width), inv_sq(vp.dir_width)};
+        launch_pairs<Op>(A, n, n, sp, 0, part, st);
     });
-    LDDMM_LAUNCH_CHECK();
 }
+
+int vf_value_finalize_blocks(int n) { return ceil_div(n, kRedBlock); }
 
 void launch_vf_value_finalize(int splits, int n, const double *part, const double *vol, double *eparts,
                               hipStream_t st)
 {
     if (n <= 0) return;
-    k_vf_value_finalize<<<ceil_div(n, kBlock), kBlock, 0, st>>>(splits, n, part, vol, eparts);
+    k_vf_value_finalize<<<vf_value_finalize_blocks(n), kRedBlock, 0, st>>>(splits, n, part, vol, eparts);
     LDDMM_LAUNCH_CHECK();
 }
 

@@ -59,3 +59,27 @@ def test_get_elm_ifo_host_only():
     assert np.array_equal(ifo, oracle.get_elm_ifo(tri, q.shape[0]))
     q2, seg = G.circle_curve(50)
     assert np.array_equal(lddmm_amd.getElmIfo(seg + 1, 50), oracle.get_elm_ifo(seg, 50))
+
+
+def test_option_structs_match_ctypes(tmp_path):
+    """lddmm_options / lddmm_result offsets in C equal the ctypes mirror's."""
+    import lddmm_amd
+    fields = {"lddmm_options": lddmm_amd.Options, "lddmm_result": lddmm_amd.Result}
+    src = ['#include "lddmm.h"', "#include <stdio.h>", "#include <stddef.h>", "int main(void){"]
+    for cname, py in fields.items():
+        src.append('printf("%%zu\\n", sizeof(%s));' % cname)
+        for f, _ in py._fields_:
+            src.append('printf("%%zu\\n", offsetof(%s, %s));' % (cname, f))
+    src.append("return 0;}")
+    c = tmp_path / "t.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "t"
+    r = subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(c), "-o", str(exe)], capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stderr
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    want = []
+    for py in fields.values():
+        want.append(ctypes.sizeof(py))
+        want += [getattr(py, f).offset for f, _ in py._fields_]
+    assert got == want

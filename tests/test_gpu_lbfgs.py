@@ -122,3 +122,43 @@ def test_matlab_style_driver(gpu):
     assert nde.shape == (21, 80, 2) and vlc.shape == (20, 80, 2) and alp.shape == (20, 80, 2)
     assert np.array_equal(nde[0], w["q0"])
     assert np.allclose(nde[1], nde[0] + 0.05 * vlc[0], rtol=0, atol=1e-15)
+
+
+def test_fp32_policy_reaches_fp64_energy(gpu):
+    """The registration protocol of the metric: the plain FP64 run (200
+    iterations) sets E_ref; the FP32 policy (finishing in FP64 if its FP32
+    phase stalls) runs until f <= E_ref (1 + 1e-6); its result, re-evaluated
+    in FP64, must be within 1e-6 of E_ref (plus the FP32 energy's own error
+    budget of 1e-8 relative)."""
+    print("fp32 vs fp64 registration, circle_square 400", flush=True)
+    w = circle_square_args()
+    n = w["q0"].shape[0]
+    ifo = oracle.get_elm_ifo(w["elm"], n)
+    common = (w["q0"], None, w["elm"], ifo, w["tgt_cen"], w["tgt_vol"], w["tgt_dir"], w["cen_type"],
+              w["cen_width"], w["dir_type"], w["dir_width"], w["dis_wgt"], w["knl_order"], w["knl_width"],
+              w["time_stp"], w["time_num"])
+    a64, i64 = gpu.run(*common, 200, 0.0, 0.0, 0.5, 5, precision="fp64")
+    e_ref = i64["fcn"]
+    a32, i32 = gpu.run(*common, 2000, 0.0, 0.0, 0.5, 5, precision="fp32", target_energy=e_ref * (1 + 1e-6))
+    obj = gpu.Objective(w["q0"], w["elm"], ifo, w["tgt_cen"], w["tgt_vol"], w["tgt_dir"], w["cen_type"],
+                        w["cen_width"], w["dir_type"], w["dir_width"], w["dis_wgt"], w["knl_order"], w["knl_width"],
+                        w["time_stp"], w["time_num"])
+    f32, _ = obj(a32)
+    print("fp64: %d it E=%.12e (%.3f s); fp32: %d it (%d in FP32) E=%.12e (%.3f s), rel %.3e, reached %s" %
+          (i64["iterations"], e_ref, i64["seconds"], i32["iterations"], i32["fp32_iterations"], f32,
+           i32["seconds"], (f32 - e_ref) / abs(e_ref), i32["reached_target"]), flush=True)
+    assert i32["reached_target"]
+    assert f32 <= e_ref * (1 + 1e-6 + 1e-8)
+
+
+def test_fp32_phase_hands_over_to_fp64_on_stall(gpu):
+    """Forced stall: a refine threshold of 1e-2 ends the FP32 phase at once;
+    the run continues in FP64 and its energy keeps decreasing."""
+    w = circle_square_args(100, T=6)
+    n = w["q0"].shape[0]
+    ifo = oracle.get_elm_ifo(w["elm"], n)
+    alp, info = gpu.run(w["q0"], None, w["elm"], ifo, w["tgt_cen"], w["tgt_vol"], w["tgt_dir"], w["cen_type"],
+                        w["cen_width"], w["dir_type"], w["dir_width"], w["dis_wgt"], w["knl_order"], w["knl_width"],
+                        w["time_stp"], w["time_num"], 30, 0.0, 0.0, 0.5, 5, precision="fp32", refine_rel_tol=1e-2,
+                        verbose=True)
+    assert info["status"] == 0 and 1 <= info["fp32_iterations"] < info["iterations"] == 30

@@ -187,3 +187,28 @@ def test_compute_node_position_matches_trajectory(gpu):
         assert rel_err(vlc[t], v) <= 1e-12
         q = q + w["time_stp"] * v
         assert rel_err(nde[t + 1], q) <= 1e-12
+
+
+@pytest.mark.parametrize("kind", ["circle_square", "sphere_cube", "superquadric"])
+def test_objgrd_fp32_policy_error(gpu, kind):
+    """FP32 pair arithmetic (FP64 differences and cross-tile sums) against
+    the FP64 oracle: relative error of the energy and of the gradient
+    (max norm) stated and bounded."""
+    w = make_problem(kind)
+    n = w["q0"].shape[0]
+    ifo = oracle.get_elm_ifo(w["elm"], n)
+    args = (w["q0"], w["elm"], ifo, w["tgt_cen"], w["tgt_vol"], w["tgt_dir"], w["cen_type"], w["cen_width"],
+            w["dir_type"], w["dir_width"], w["dis_wgt"], w["knl_order"], w["knl_width"], w["time_stp"], w["time_num"])
+    obj = gpu.Objective(*args, precision="fp32")
+    P = oracle.OracleProblem(w["q0"], w["elm"], w["tgt_cen"], w["tgt_dir"], w["tgt_vol"], w["cen_type"],
+                             w["cen_width"], w["dir_type"], w["dir_width"], w["dis_wgt"], w["knl_order"],
+                             w["knl_width"], w["time_stp"], w["time_num"], ifo=ifo)
+    for seed, scale in ((0, 0.0), (6, 0.3)):
+        a = scale * G.normal(seed + 100, obj.varnum)
+        f, g = obj(a)
+        ef, eg = P.objgrd(a)
+        e_rel, g_rel = abs(f - ef) / abs(ef), rel_err(g, eg)
+        print("fp32 %s scale %.2f: energy rel err %.3e, gradient rel err %.3e" % (kind, scale, e_rel, g_rel),
+              flush=True)
+        assert e_rel <= 1e-6 and g_rel <= 1e-4
+    obj.close()
