@@ -177,6 +177,37 @@ int lddmm_varifold(double *vfdVal, double *dqVfdMat, const double *dfmLmkPosMat,
                    char cenKnlType, double cenKnlWidth, char dirKnlType, double dirKnlWidth,
                    int dfmLmkNum, int dfmElmNum, int dfmElmIfoNum, int tgtElmNum, int dimNum);
 
+/* ---------------------------------------------------------------- batches
+ * Many independent registrations of uneven sizes in one set of launches
+ * (no counterpart in the reference, which registers one pair per MEX call;
+ * every problem follows Cpp/src/LBFGS.cpp and Cpp/src/objgrd.cpp).  Inputs
+ * are the host-layout arrays of each problem concatenated problem after
+ * problem: iniNdeMats D x N_p blocks, dfmElmVtxMats V x E_p blocks (0-based,
+ * local to the problem), targets D x Et_p / Et_p blocks, knlWidths one per
+ * problem; kernel and varifold types, weight and time grid are shared.  The
+ * adjacency tables are built by the library.  chunk = source points per
+ * work item (0 = autotuned on the batch). */
+typedef struct lddmm_batch lddmm_batch;
+int lddmm_batch_create(lddmm_batch **out, int dimNum, int numProblems, const int32_t *ndeNums,
+                       const int32_t *dfmElmNums, const int32_t *tgtElmNums, const double *iniNdeMats,
+                       const int32_t *dfmElmVtxMats, const double *tgtCenPosMats, const double *tgtElmVolVecs,
+                       const double *tgtUniDirMats, const double *knlWidths, char cenKnlType, double cenKnlWidth,
+                       char dirKnlType, double dirKnlWidth, double disWgt, int knlOrder, double timeStp,
+                       int timeNum, int precision, int chunk);
+/* fcnVals[p] and the concatenated D x N_p x (T-1) gradient blocks at the
+ * concatenated momenta (host layout), in the batch's precision. */
+int lddmm_batch_objgrd(lddmm_batch *b, double *fcnVals, double *grdStks, const double *alpStks);
+/* Runs every registration (L-BFGS per problem, one batched evaluation per
+ * round of line-search trials).  iniAlpStks may be NULL (zeros);
+ * targetEnergies (one per problem) may be NULL; results (one per problem)
+ * and rounds may be NULL. */
+int lddmm_batch_run(lddmm_batch *b, const double *iniAlpStks, double *optAlpStks, int itrMax, double tolVal,
+                    double wolfe1, double wolfe2, int memNum, const lddmm_options *opts,
+                    const double *targetEnergies, lddmm_result *results, int *rounds);
+/* chosen source chunk and the pair interactions per evaluation of each problem */
+int lddmm_batch_info(lddmm_batch *b, int *chunk, double *pairsPerEval);
+void lddmm_batch_destroy(lddmm_batch *b);
+
 /* ---------------------------------------------------------------- data formats
  * getElmIfo (Cpp/src/getElmIfo.m:1-59) from a 0-based V x E table; writes
  * the (1 + maxAdj*V) x N table when elmIfoMat != NULL and returns the row

@@ -346,6 +346,92 @@ def LDDMM(iniNdeMat, iniAlpStk, disObj, knlOrder, knlWidth, timeEnd, timeStp, op
 
 
 # ---------------------------------------------------------------- profiling
+class Batch:
+    """Many independent registrations of uneven sizes evaluated together
+    (lddmm_batch_*).  problems: list of dicts with q0 (N, D), elm (E, V)
+    0-based, tgt_cen, tgt_dir (Et, D), tgt_vol (Et,), knl_width; the kernel
+    and varifold types, weight and time grid are shared."""
+
+    def __init__(self, problems, cen_type, cen_width, dir_type, dir_width, dis_wgt, knl_order, time_stp, time_num,
+                 precision="fp64", chunk=0):
+        L = lib()
+        if not hasattr(L.lddmm_batch_create, "_set"):
+            L.lddmm_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, _ip, _ip, _ip, _dp, _ip, _dp,
+                                             _dp, _dp, _dp, C.c_char, C.c_double, C.c_char, C.c_double, C.c_double,
+                                             C.c_int, C.c_double, C.c_int, C.c_int, C.c_int]
+            L.lddmm_batch_objgrd.argtypes = [C.c_void_p, _dp, _dp, _dp]
+            L.lddmm_batch_run.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_double,
+                                          C.c_int, C.POINTER(Options), _dp, C.POINTER(Result), C.POINTER(C.c_int)]
+            L.lddmm_batch_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), _dp]
+            L.lddmm_batch_destroy.argtypes = [C.c_void_p]
+            L.lddmm_batch_destroy.restype = None
+            L.lddmm_batch_create._set = True
+        self.P = len(problems)
+        self.dim = np.asarray(problems[0]["q0"]).shape[1]
+        self.T = int(time_num)
+        self.n = np.array([len(p["q0"]) for p in problems], dtype=np.int32)
+        ne = np.array([len(p["elm"]) for p in problems], dtype=np.int32)
+        nt = np.array([len(p["tgt_vol"]) for p in problems], dtype=np.int32)
+        q0 = _f64(np.concatenate([p["q0"] for p in problems]))
+        elm = _i32(np.concatenate([p["elm"] for p in problems]))
+        tc = _f64(np.concatenate([p["tgt_cen"] for p in problems]))
+        tu = _f64(np.concatenate([p["tgt_dir"] for p in problems]))
+        ta = _f64(np.concatenate([p["tgt_vol"] for p in problems]))
+        widths = _f64([p["knl_width"] for p in problems])
+        h = C.c_void_p()
+        _check(L.lddmm_batch_create(C.byref(h), self.dim, self.P, _p(self.n), _p(ne), _p(nt), _p(q0), _p(elm),
+                                    _p(tc), _p(ta), _p(tu), _p(widths), _ch(cen_type), cen_width, _ch(dir_type),
+                                    dir_width, dis_wgt, knl_order, time_stp, self.T, PRECISIONS[precision],
+                                    int(chunk)))
+        self._h = h
+        self.sizes = self.dim * self.n * (self.T - 1)
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)])
+
+    def split(self, flat):
+        return [flat[self.offsets[p]:self.offsets[p + 1]].reshape(self.T - 1, self.n[p], self.dim)
+                for p in range(self.P)]
+
+    def objgrd(self, alps):
+        """alps: list of (T-1, N_p, D) momenta -> (energies, list of gradients)."""
+        a = _f64(np.concatenate([np.ravel(x) for x in alps]))
+        f = np.empty(self.P)
+        g = np.empty_like(a)
+        _check(lib().lddmm_batch_objgrd(self._h, _p(f), _p(g), _p(a)))
+        return f, self.split(g)
+
+    def run(self, itr_max, tol, wolfe1, wolfe2, mem, alp0=None, precision="fp64", rel_energy_tol=0.0,
+            fp64_refine=True, refine_rel_tol=0.0, targets=None):
+        a0 = None if alp0 is None else _f64(np.concatenate([np.ravel(x) for x in alp0]))
+        out = np.empty(int(self.offsets[-1]))
+        opts = Options(PRECISIONS[precision], 0 if fp64_refine else 1, rel_energy_tol, refine_rel_tol, 0.0, 0, 0)
+        res = (Result * self.P)()
+        rounds = C.c_int()
+        tg = None if targets is None else _f64(targets)
+        _check(lib().lddmm_batch_run(self._h, _p(a0), _p(out), itr_max, tol, wolfe1, wolfe2, mem, C.byref(opts),
+                                     _p(tg), res, C.byref(rounds)))
+        infos = [dict(fcn=r.fcn, grad_norm=r.grad_norm, seconds=r.seconds, pairs_per_eval=r.pairs_per_eval,
+                      iterations=r.iterations, evaluations=r.evaluations, status=r.status,
+                      fp32_iterations=r.fp32_iterations, reached_target=bool(r.reached_target)) for r in res]
+        return self.split(out), infos, rounds.value
+
+    def info(self):
+        ch = C.c_int()
+        pairs = np.empty(self.P)
+        _check(lib().lddmm_batch_info(self._h, C.byref(ch), _p(pairs)))
+        return ch.value, pairs
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lddmm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DeviceArray:
     """A device allocation made through the library (no torch involved)."""
 

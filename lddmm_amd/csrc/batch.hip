@@ -1,0 +1,1434 @@
+// batch.hip -- batched evaluation and L-BFGS for many uneven registrations.
+//
+// Layout: every per-problem array is a contiguous block in a concatenated
+// device array, SoA inside the block: points [D][N_p] at D*vo_p, the flow
+// [T][D][N_p] at T*D*vo_p, momenta and gradients [T-1][D][N_p] at
+// (T-1)*D*vo_p, elements [V][E_p] at V*eo_p (vertex indices local to the
+// problem), adjacency [rows][N_p] at rows*vo_p, targets [D][Et_p] at
+// D*to_p.  Work tables list (problem, tile, chunk) items of equal size
+// (64 targets x `chunk` sources), so a problem of 20,000 points contributes
+// many items and one of 500 points a few, and one launch per Euler step
+// keeps the whole chip busy.  All sums are in a fixed order: results do not
+// depend on which other problems share the launch.
+#include <algorithm>
+#include <chrono>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+#include "batch.hpp"
+#include "pair_ops.hpp"
+#include "varifold_ops.hpp"
+
+namespace lddmm {
+
+namespace {
+
+using namespace ops;
+
+constexpr int kTile = 64;          // targets per work item (one per lane)
+constexpr int kVecBlk = 4096;      // elements per block of the batched vector kernels
+constexpr int kMaxBDots = 4;
+
+struct BatchView {
+    const BatchDesc *desc;
+    double *nde, *padj, *part, *vpart;
+    const double *alp;
+    double *grd;
+    const double *q0, *tc, *tu, *ta;
+    const int *elm, *ifo;
+    int rows, T;
+    double h, w;
+    double *ecen, *edir, *evol, *dc, *dd, *efwd, *evf;
+};
+
+template <int NC>
+__device__ __forceinline__ void sum_splits(const double *__restrict__ part, int splits, int n, int i,
+                                           double (&v)[NC])
+{
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] = 0.0;
+    int k = 0;
+    for (; k + 4 <= splits; k += 4) {
+        double t[4][NC];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) t[u][c] = part[((size_t)(k + u) * NC + c) * n + i];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) v[c] += t[u][c];
+    }
+    for (; k < splits; ++k)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) v[c] += part[((size_t)k * NC + c) * n + i];
+}
+
+// ------------------------------------------------------------- flow kernels
+template <int D, int ORDER, class P, int R>
+__global__ __launch_bounds__(kBlock) void kb_kv(BatchView v, int t, const int4 *__restrict__ items)
+{
+    using Op = OpKV<D, ORDER, P, R>;
+    using real = typename Op::real;
+    __shared__ __align__(16) char smem[PairSmem<Op>::bytes];
+    const int4 it = items[xcd_remap(blockIdx.x, gridDim.x)];
+    const BatchDesc d = v.desc[it.x];
+    const double *q = v.nde + (size_t)v.T * D * d.vo + (size_t)t * D * d.n;
+    const double *a = v.alp + (size_t)(v.T - 1) * D * d.vo + (size_t)t * D * d.n;
+    typename Op::Args A{q, q, a, d.n, d.n, (real)d.iw, (real)d.iw2};
+    const int j0 = it.z * d.chunk, j1 = min(d.n, j0 + d.chunk);
+    pairs_block<Op>(A, d.n, it.y, j0, j1, v.part + d.po + (size_t)it.z * D * d.n, smem);
+}
+
+template <int D, int ORDER, class P>
+__global__ __launch_bounds__(kBlock) void kb_bwd(BatchView v, int t, const int4 *__restrict__ items)
+{
+    using Op = OpBwd<D, ORDER, P>;
+    using real = typename Op::real;
+    __shared__ __align__(16) char smem[PairSmem<Op>::bytes];
+    const int4 it = items[xcd_remap(blockIdx.x, gridDim.x)];
+    const BatchDesc d = v.desc[it.x];
+    const double *q = v.nde + (size_t)v.T * D * d.vo + (size_t)t * D * d.n;
+    const double *a = v.alp + (size_t)(v.T - 1) * D * d.vo + (size_t)t * D * d.n;
+    typename Op::Args A{q, a, v.padj + (size_t)D * d.vo, d.n, (real)d.iw, (real)d.iw2};
+    const int j0 = it.z * d.chunk, j1 = min(d.n, j0 + d.chunk);
+    pairs_block<Op>(A, d.n, it.y, j0, j1, v.part + d.po + (size_t)it.z * 2 * D * d.n, smem);
+}
+
+template <int D>
+__global__ __launch_bounds__(kTile) void kb_fwd_reduce(BatchView v, int t, const int4 *__restrict__ items)
+{
+    const int4 it = items[blockIdx.x];
+    const BatchDesc d = v.desc[it.x];
+    const int i = it.y * kTile + threadIdx.x;
+    if (i >= d.n) return;
+    double vv[D];
+    sum_splits<D>(v.part + d.po, d.splits, d.n, i, vv);
+    const double *q = v.nde + (size_t)v.T * D * d.vo + (size_t)t * D * d.n;
+    double *qn = v.nde + (size_t)v.T * D * d.vo + (size_t)(t + 1) * D * d.n;
+    const double *a = v.alp + (size_t)(v.T - 1) * D * d.vo + (size_t)t * D * d.n;
+    double e = 0.0;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        const size_t o = (size_t)c * d.n + i;
+        e += a[o] * vv[c];
+        qn[o] = q[o] + v.h * vv[c];
+    }
+    double *ef = v.efwd + d.vo + i;
+    *ef = (t == 0 ? 0.0 : *ef) + e;
+}
+
+template <int D>
+__global__ __launch_bounds__(kTile) void kb_bwd_reduce(BatchView v, int t, const int4 *__restrict__ items)
+{
+    const int4 it = items[blockIdx.x];
+    const BatchDesc d = v.desc[it.x];
+    const int i = it.y * kTile + threadIdx.x;
+    if (i >= d.n) return;
+    double vv[2 * D];
+    sum_splits<2 * D>(v.part + d.po, d.splits, d.n, i, vv);
+    double *g = v.grd + (size_t)(v.T - 1) * D * d.vo + (size_t)t * D * d.n;
+    double *p = v.padj + (size_t)D * d.vo;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        const size_t o = (size_t)c * d.n + i;
+        g[o] = v.h * vv[c];
+        p[o] = p[o] - v.h * vv[D + c];
+    }
+}
+
+// ---------------------------------------------------------------- varifold
+template <int D>
+__global__ __launch_bounds__(kTile) void kb_geometry(BatchView v, const int4 *__restrict__ items)
+{
+    const int4 it = items[blockIdx.x];
+    const BatchDesc d = v.desc[it.x];
+    const int e = it.y * kTile + threadIdx.x;
+    if (e >= d.ne) return;
+    const double *q = v.nde + (size_t)v.T * D * d.vo + (size_t)(v.T - 1) * D * d.n;
+    const int *elm = v.elm + (size_t)D * d.eo;   // D vertices per element (segment / triangle)
+    double *cen = v.ecen + (size_t)D * d.eo, *dir = v.edir + (size_t)D * d.eo, *vol = v.evol + d.eo;
+    const int ne = d.ne, n = d.n;
+    if constexpr (D == 2) {
+        const int i0 = elm[e], i1 = elm[ne + e];
+        double x0 = q[i0], y0 = q[n + i0], x1 = q[i1], y1 = q[n + i1];
+        double ux = x1 - x0, uy = y1 - y0, a = sqrt(ux * ux + uy * uy);
+        cen[e] = 0.5 * (x0 + x1);
+        cen[ne + e] = 0.5 * (y0 + y1);
+        dir[e] = ux / a;
+        dir[ne + e] = uy / a;
+        vol[e] = a;
+    } else {
+        const int i0 = elm[e], i1 = elm[ne + e], i2 = elm[2 * ne + e];
+        double e1[3], e2[3], nn[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double p0 = q[(size_t)c * n + i0], p1 = q[(size_t)c * n + i1], p2 = q[(size_t)c * n + i2];
+            e1[c] = p1 - p0;
+            e2[c] = p2 - p0;
+            cen[(size_t)c * ne + e] = (p0 + p1 + p2) / 3.0;
+        }
+        nn[0] = e1[1] * e2[2] - e1[2] * e2[1];
+        nn[1] = -e1[0] * e2[2] + e1[2] * e2[0];
+        nn[2] = e1[0] * e2[1] - e1[1] * e2[0];
+        double l = sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) dir[(size_t)c * ne + e] = nn[c] / l;
+        vol[e] = 0.5 * l;
+    }
+}
+
+template <int D, int CEN, int DIR, class P>
+__global__ __launch_bounds__(kBlock) void kb_vf(BatchView v, double ics2, double ids2,
+                                                const int4 *__restrict__ items)
+{
+    using Op = OpVf<D, CEN, DIR, P>;
+    using real = typename Op::real;
+    __shared__ __align__(16) char smem[PairSmem<Op>::bytes];
+    const int4 it = items[xcd_remap(blockIdx.x, gridDim.x)];
+    const BatchDesc d = v.desc[it.x];
+    const double *ci = v.ecen + (size_t)D * d.eo, *ui = v.edir + (size_t)D * d.eo;
+    const bool tgt = it.z >= d.vsd;
+    typename Op::Args A;
+    A.ci = ci;
+    A.ui = ui;
+    A.ni = d.ne;
+    A.cj = tgt ? v.tc + (size_t)D * d.to : ci;
+    A.uj = tgt ? v.tu + (size_t)D * d.to : ui;
+    A.aj = tgt ? v.ta + d.to : v.evol + d.eo;
+    A.nj = tgt ? d.nt : d.ne;
+    A.ics2 = (real)ics2;
+    A.ids2 = (real)ids2;
+    const int j0 = (tgt ? it.z - d.vsd : it.z) * d.vchunk, j1 = min(A.nj, j0 + d.vchunk);
+    pairs_block<Op>(A, d.ne, it.y, j0, j1, v.vpart + d.vpo + (size_t)it.z * (2 * D + 2) * d.ne, smem);
+}
+
+template <int D>
+__global__ __launch_bounds__(kTile) void kb_vf_finalize(BatchView v, double gscale, const int4 *__restrict__ items)
+{
+    constexpr int NC = 2 * D + 2;
+    const int4 it = items[blockIdx.x];
+    const BatchDesc d = v.desc[it.x];
+    const int i = it.y * kTile + threadIdx.x;
+    if (i >= d.ne) return;
+    const int ne = d.ne;
+    double acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] = 0.0;
+    const int splits = d.vsd + d.vst;
+    for (int s = 0; s < splits; ++s) {
+        const double *o = v.vpart + d.vpo + (size_t)s * NC * ne + i;
+        const double we = s < d.vsd ? 1.0 : -2.0, wg = s < d.vsd ? 1.0 : -1.0;
+        acc[0] += we * o[0];
+#pragma unroll
+        for (int c = 1; c < NC; ++c) acc[c] += wg * o[(size_t)c * ne];
+    }
+    const double a = v.evol[d.eo + i];
+    const double dv = (D == 3) ? 0.5 : 1.0;
+    const double *dir = v.edir + (size_t)D * d.eo;
+    double *dc = v.dc + (size_t)D * d.eo, *dd = v.dd + (size_t)D * d.eo;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        dc[(size_t)c * ne + i] = gscale * a * acc[2 + c];
+        dd[(size_t)c * ne + i] = 2.0 * dv * (acc[2 + D + c] + acc[1] * dir[(size_t)c * ne + i]);
+    }
+    v.evf[d.eo + i] = a * acc[0];
+}
+
+template <int D>
+__global__ __launch_bounds__(kTile) void kb_vf_gather(BatchView v, const int4 *__restrict__ items)
+{
+    const int4 it = items[blockIdx.x];
+    const BatchDesc d = v.desc[it.x];
+    const int n = it.y * kTile + threadIdx.x;
+    if (n >= d.n) return;
+    const int nq = d.n, ne = d.ne;
+    const int *ifo = v.ifo + (size_t)v.rows * d.vo;
+    const double *q = v.nde + (size_t)v.T * D * d.vo + (size_t)(v.T - 1) * D * d.n;
+    const double *dc = v.dc + (size_t)D * d.eo, *dd = v.dd + (size_t)D * d.eo;
+    double g[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) g[c] = 0.0;
+    const int adj = ifo[n];
+    for (int k = 0; k < adj; ++k) {
+        if constexpr (D == 2) {
+            const int e = ifo[(size_t)(1 + 2 * k) * nq + n];
+            const int sgn = ifo[(size_t)(2 + 2 * k) * nq + n];
+            g[0] += 0.5 * dc[e] + sgn * dd[e];
+            g[1] += 0.5 * dc[ne + e] + sgn * dd[ne + e];
+        } else {
+            const int e = ifo[(size_t)(1 + 3 * k) * nq + n];
+            const int ia = ifo[(size_t)(2 + 3 * k) * nq + n];
+            const int ib = ifo[(size_t)(3 + 3 * k) * nq + n];
+            double ab[3], dv[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                ab[c] = q[(size_t)c * nq + ia] - q[(size_t)c * nq + ib];
+                dv[c] = dd[(size_t)c * ne + e];
+            }
+            g[0] += dc[e] / 3.0 + (dv[1] * ab[2] - dv[2] * ab[1]);
+            g[1] += dc[ne + e] / 3.0 + (-dv[0] * ab[2] + dv[2] * ab[0]);
+            g[2] += dc[2 * (size_t)ne + e] / 3.0 + (dv[0] * ab[1] - dv[1] * ab[0]);
+        }
+    }
+    double *p = v.padj + (size_t)D * d.vo;
+#pragma unroll
+    for (int c = 0; c < D; ++c) p[(size_t)c * nq + n] = -v.w * g[c];
+}
+
+template <int D>
+__global__ __launch_bounds__(kTile) void kb_init_flow(BatchView v, const int4 *__restrict__ items)
+{
+    const int4 it = items[blockIdx.x];
+    const BatchDesc d = v.desc[it.x];
+    const int i = it.y * kTile + threadIdx.x;
+    if (i >= d.n) return;
+    double *q = v.nde + (size_t)v.T * D * d.vo;
+    const double *q0 = v.q0 + (size_t)D * d.vo;
+#pragma unroll
+    for (int c = 0; c < D; ++c) q[(size_t)c * d.n + i] = q0[(size_t)c * d.n + i];
+}
+
+__device__ __forceinline__ double block_sum256(double x, double *red)
+{
+    red[threadIdx.x] = x;
+    __syncthreads();
+#pragma unroll
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    return red[0];
+}
+
+// one block per listed problem: f = h/2 sum efwd + w (sum evf + tt)
+__global__ __launch_bounds__(kBlock) void kb_energy(BatchView v, const double *__restrict__ tt,
+                                                    const int *__restrict__ probs, double *__restrict__ f)
+{
+    __shared__ double red[kBlock];
+    const int p = probs[blockIdx.x];
+    const BatchDesc d = v.desc[p];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < d.n; i += kBlock) s += v.efwd[d.vo + i];
+    const double ef = block_sum256(s, red);
+    __syncthreads();
+    s = 0.0;
+    for (int i = threadIdx.x; i < d.ne; i += kBlock) s += v.evf[d.eo + i];
+    const double ev = block_sum256(s, red);
+    if (threadIdx.x == 0) f[p] = v.h * 0.5 * ef + v.w * (ev + tt[p]);
+}
+
+// target-target term, once per problem
+template <int D, int CEN, int DIR>
+__global__ __launch_bounds__(kBlock) void kb_tt(BatchView v, double ics2, double ids2, const int4 *__restrict__ items)
+{
+    using Op = OpVfValue<D, CEN, DIR>;
+    __shared__ __align__(16) char smem[PairSmem<Op>::bytes];
+    const int4 it = items[xcd_remap(blockIdx.x, gridDim.x)];
+    const BatchDesc d = v.desc[it.x];
+    typename Op::Args A{v.tc + (size_t)D * d.to, v.tu + (size_t)D * d.to, v.ta + d.to, d.nt, ics2, ids2};
+    const int j0 = it.z * d.vchunk, j1 = min(d.nt, j0 + d.vchunk);
+    pairs_block<Op>(A, d.nt, it.y, j0, j1, v.vpart + d.tpo + (size_t)it.z * d.nt, smem);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_tt_sum(BatchView v, double *__restrict__ tt)
+{
+    __shared__ double red[kBlock];
+    const int p = blockIdx.x;
+    const BatchDesc d = v.desc[p];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < d.nt; i += kBlock) {
+        double a = 0.0;
+        for (int k = 0; k < d.tsplits; ++k) a += v.vpart[d.tpo + (size_t)k * d.nt + i];
+        s += v.ta[d.to + i] * a;
+    }
+    const double r = block_sum256(s, red);
+    if (threadIdx.x == 0) tt[p] = r;
+}
+
+// ------------------------------------------------------------- vectors
+// A per-problem vector: base + slot[p]*stride + voff[p] (slot == nullptr:
+// no slot; slot[p] < 0: problem skipped).
+struct VRef {
+    const double *base;
+    const int *slot;
+    long long stride;
+};
+
+struct VecTab {
+    const int2 *blk;          // (problem, block within problem)
+    const long long *voff;    // per-problem variable offset
+    const long long *vn;      // per-problem variable count
+};
+
+__device__ __forceinline__ bool vref_ptr(const VRef &r, int p, long long voff, const double *&out)
+{
+    long long o = voff;
+    if (r.slot) {
+        const int s = r.slot[p];
+        if (s < 0) return false;
+        o += (long long)s * r.stride;
+    }
+    out = r.base + o;
+    return true;
+}
+
+struct BDotArgs {
+    VRef x[kMaxBDots], y[kMaxBDots];
+};
+
+__global__ __launch_bounds__(kBlock) void kb_dots1(BDotArgs a, VecTab vt, const int *__restrict__ mask,
+                                                   double *__restrict__ part, int nent)
+{
+    __shared__ double red[kBlock];
+    const int k = blockIdx.y;
+    const int2 b = vt.blk[blockIdx.x];
+    const int p = b.x;
+    const double *x, *y;
+    double s = 0.0;
+    const bool ok = (!mask || mask[p]) && vref_ptr(a.x[k], p, vt.voff[p], x) && vref_ptr(a.y[k], p, vt.voff[p], y);
+    if (ok) {
+        const long long lo = (long long)b.y * kVecBlk, hi = min(vt.vn[p], lo + kVecBlk);
+        for (long long i = lo + threadIdx.x; i < hi; i += kBlock) s += x[i] * y[i];
+    }
+    const double r = block_sum256(s, red);
+    if (threadIdx.x == 0) part[(size_t)k * nent + blockIdx.x] = r;
+}
+
+// one block per problem: its entries are first[p] .. first[p]+count[p]-1
+__global__ __launch_bounds__(kBlock) void kb_dots2(const double *__restrict__ part, int nent,
+                                                   const int *__restrict__ first, const int *__restrict__ count,
+                                                   const int *__restrict__ mask, double *__restrict__ out, int P)
+{
+    __shared__ double red[kBlock];
+    const int p = blockIdx.x, k = blockIdx.y;
+    if (mask && !mask[p]) return;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < count[p]; i += kBlock) s += part[(size_t)k * nent + first[p] + i];
+    const double r = block_sum256(s, red);
+    if (threadIdx.x == 0) out[(size_t)k * P + p] = r;
+}
+
+// generic per-problem update over the block table
+enum VecOp : int { kCopy = 0, kCopyNeg, kLoop1, kScaleH0, kLoop2, kTry, kAccept };
+
+struct VecArgs {
+    int op, m;
+    double *d;                 // direction (or xt for kTry)
+    const double *g, *x;       // gradient / iterate
+    const double *S, *Y;       // ring buffers [m][vtot]
+    double *Sw, *Yw;           // writable ring buffers (accept)
+    double *xw, *gw;           // writable iterate / gradient (accept)
+    const double *xt, *gt;     // trial point (accept)
+    long long vtot;
+    const int *mask;           // per problem, 0 = skip
+    const int *slot;           // per problem ring slot
+    const int *neg;            // per problem: negate after the update (loop2)
+    const double *sd, *yd;     // per problem scalars
+    const double *sy, *yy;     // per problem [m]
+    double *rec;               // per problem [m]
+    const double *rel;         // per problem step (kTry)
+};
+
+__global__ __launch_bounds__(kBlock) void kb_vec(VecArgs a, VecTab vt)
+{
+    const int2 b = vt.blk[blockIdx.x];
+    const int p = b.x;
+    if (a.mask && !a.mask[p]) return;
+    const int sl = a.slot ? a.slot[p] : 0;
+    if (sl < 0) return;
+    const long long vo = vt.voff[p];
+    const long long lo = (long long)b.y * kVecBlk, hi = min(vt.vn[p], lo + kVecBlk);
+    const long long so = (long long)sl * a.vtot + vo;
+    double coef = 0.0;
+    switch (a.op) {
+    case kLoop1:
+        coef = a.sd[p] / a.sy[p * a.m + sl];
+        if (b.y == 0 && threadIdx.x == 0) a.rec[p * a.m + sl] = coef;
+        break;
+    case kScaleH0: coef = a.sy[p * a.m + sl] / a.yy[p * a.m + sl]; break;
+    case kLoop2: coef = a.rec[p * a.m + sl] - a.yd[p] / a.sy[p * a.m + sl]; break;
+    case kTry: coef = a.rel[p]; break;
+    default: break;
+    }
+    const bool neg = a.neg && a.neg[p];
+    for (long long i = lo + threadIdx.x; i < hi; i += kBlock) {
+        const long long k = vo + i;
+        switch (a.op) {
+        case kCopy: a.d[k] = a.g[k]; break;
+        case kCopyNeg: a.d[k] = -a.g[k]; break;
+        case kLoop1: a.d[k] -= coef * a.Y[so + i]; break;
+        case kScaleH0: a.d[k] *= coef; break;
+        case kLoop2: {
+            double t = a.d[k] + coef * a.S[so + i];
+            a.d[k] = neg ? -t : t;
+            break;
+        }
+        case kTry: a.d[k] = a.x[k] + coef * a.g[k]; break;   // d := xt, g := direction
+        case kAccept: {
+            double xn = a.xt[k], gn = a.gt[k];
+            a.Sw[so + i] = xn - a.xw[k];
+            a.Yw[so + i] = gn - a.gw[k];
+            a.xw[k] = xn;
+            a.gw[k] = gn;
+            break;
+        }
+        default: break;
+        }
+    }
+}
+
+// curvature scalars of the newest pair into the per-slot tables
+__global__ void kb_store_pairs(const double *__restrict__ sdv, const double *__restrict__ ydv,
+                               const int *__restrict__ mask, const int *__restrict__ slot, double *__restrict__ sy,
+                               double *__restrict__ yy, int m, int P)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P || !mask[p]) return;
+    sy[(size_t)p * m + slot[p]] = sdv[p];
+    yy[(size_t)p * m + slot[p]] = ydv[p];
+}
+
+int cen_code(char t) { return t == 'G' ? 0 : 1; }
+int dir_code(char t) { return t == 'B' ? 0 : t == 'L' ? 1 : t == 'O' ? 2 : 3; }
+
+template <typename F>
+void dispatch_vf3(int dim, const VfParams &vp, F &&f)
+{
+    const int cc = cen_code(vp.cen_type), dc = dir_code(vp.dir_type);
+    auto with_dim = [&](auto C, auto U) {
+        if (dim == 2) f(std::integral_constant<int, 2>{}, C, U);
+        else f(std::integral_constant<int, 3>{}, C, U);
+    };
+    auto with_dir = [&](auto C) {
+        switch (dc) {
+        case 0: with_dim(C, std::integral_constant<int, 0>{}); break;
+        case 1: with_dim(C, std::integral_constant<int, 1>{}); break;
+        case 2: with_dim(C, std::integral_constant<int, 2>{}); break;
+        default: with_dim(C, std::integral_constant<int, 3>{}); break;
+        }
+    };
+    if (cc == 0) with_dir(std::integral_constant<int, 0>{});
+    else with_dir(std::integral_constant<int, 1>{});
+}
+
+double inv_sq(double w) { return w != 0.0 ? 1.0 / (w * w) : 0.0; }
+
+// Device buffer owned by a unique_ptr-like holder
+template <typename T>
+struct DArr {
+    T *p = nullptr;
+    size_t n = 0;
+    void ensure(size_t count)
+    {
+        if (count <= n) return;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        if (hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess)
+            throw Error(kErrAlloc, "device allocation failed");
+        n = count;
+    }
+    ~DArr()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+// work tables of one active set: per precision group the point-kernel and
+// varifold items; vertex / element tiles and the list of problems
+struct Batch::Tables {
+    DArr<int4> kv[2], bwd[2], vf[2], vtile, etile;
+    int nkv[2] = {0, 0}, nbwd[2] = {0, 0}, nvf[2] = {0, 0}, nvtile = 0, netile = 0;
+    DArr<int> probs;
+    int nprobs = 0;
+};
+
+// ======================================================================
+Batch::Batch(const BatchSpec &spec, const double *q0, const int *elm, const double *tc, const double *ta,
+             const double *tu, hipStream_t st)
+    : spec_(spec), st_(st), tab_(new Tables)
+{
+    const int P = spec_.P(), D = spec_.dim;
+    if (P <= 0) throw Error(kErrArg, "a batch needs at least one problem");
+    if (D != 2 && D != 3) throw Error(kErrArg, "dimension must be 2 or 3");
+    if (spec_.T < 2) throw Error(kErrArg, "timeNum must be at least 2");
+    if (D == 2 && spec_.order >= 0) throw Error(kErrUnsupported, "2D Matern kernels are not implemented on the GPU");
+    if (spec_.order < -1 || spec_.order > 4) throw Error(kErrArg, "knlOrder must be -1 or 0..4");
+    if ((int)spec_.ne.size() != P || (int)spec_.nt.size() != P || (int)spec_.width.size() != P)
+        throw Error(kErrArg, "per-problem arrays must have one entry per problem");
+    {
+        DeviceProblem chk;
+        chk.dim = D;
+        chk.n = 1;
+        chk.T = spec_.T;
+        chk.order = spec_.order;
+        chk.vp = spec_.vp;
+        chk.ne = 1;
+        validate(chk);
+    }
+    vo_.resize(P);
+    eo_.resize(P);
+    to_.resize(P);
+    for (int p = 0; p < P; ++p) {
+        if (spec_.n[p] <= 0 || spec_.ne[p] <= 0 || spec_.nt[p] < 0) throw Error(kErrArg, "bad problem size");
+        if (!(spec_.width[p] > 0.0)) throw Error(kErrArg, "kernel widths must be positive");
+        vo_[p] = ntot_;
+        eo_[p] = etot_;
+        to_[p] = ttot_;
+        ntot_ += spec_.n[p];
+        etot_ += spec_.ne[p];
+        ttot_ += spec_.nt[p];
+    }
+    vtot_ = (long long)D * (spec_.T - 1) * ntot_;
+
+    // adjacency tables per problem (getElmIfo, Cpp/src/getElmIfo.m), padded to one row count
+    std::vector<std::vector<int>> ifos(P);
+    for (int p = 0; p < P; ++p) {
+        const int n = spec_.n[p], ne = spec_.ne[p];
+        const int *ev = elm + (size_t)D * eo_[p];
+        std::vector<int> cnt(n, 0);
+        for (long long k = 0; k < (long long)ne * D; ++k) {
+            if (ev[k] < 0 || ev[k] >= n) throw Error(kErrArg, "element table holds a vertex outside its problem");
+            ++cnt[ev[k]];
+        }
+        int mx = 0;
+        for (int c : cnt) mx = std::max(mx, c);
+        rows_ = std::max(rows_, 1 + mx * D);
+    }
+    std::vector<int> hifo((size_t)rows_ * ntot_, 0);
+    for (int p = 0; p < P; ++p) {
+        const int n = spec_.n[p], ne = spec_.ne[p];
+        const int *ev = elm + (size_t)D * eo_[p];
+        int *blk = hifo.data() + (size_t)rows_ * vo_[p];   // [rows][n]
+        for (int e = 0; e < ne; ++e)
+            for (int k = 0; k < D; ++k) {
+                const int v = ev[(size_t)e * D + k];
+                int &count = blk[v];
+                const int base = 1 + count * D;
+                blk[(size_t)base * n + v] = e;
+                if (D == 2) {
+                    blk[(size_t)(base + 1) * n + v] = k == 0 ? -1 : 1;
+                } else {
+                    static const int pa[3] = {2, 0, 1}, pb[3] = {1, 2, 0};
+                    blk[(size_t)(base + 1) * n + v] = ev[(size_t)e * D + pa[k]];
+                    blk[(size_t)(base + 2) * n + v] = ev[(size_t)e * D + pb[k]];
+                }
+                ++count;
+            }
+    }
+    // host layout -> per-problem SoA blocks
+    std::vector<double> hq((size_t)D * ntot_), htc((size_t)D * std::max<long long>(ttot_, 1)),
+        htu((size_t)D * std::max<long long>(ttot_, 1));
+    std::vector<int> helm((size_t)D * etot_);
+    for (int p = 0; p < P; ++p) {
+        const int n = spec_.n[p], ne = spec_.ne[p], nt = spec_.nt[p];
+        for (int i = 0; i < n; ++i)
+            for (int c = 0; c < D; ++c) hq[(size_t)D * vo_[p] + (size_t)c * n + i] = q0[(size_t)D * (vo_[p] + i) + c];
+        for (int e = 0; e < ne; ++e)
+            for (int c = 0; c < D; ++c)
+                helm[(size_t)D * eo_[p] + (size_t)c * ne + e] = elm[(size_t)D * (eo_[p] + e) + c];
+        for (int e = 0; e < nt; ++e)
+            for (int c = 0; c < D; ++c) {
+                htc[(size_t)D * to_[p] + (size_t)c * nt + e] = tc[(size_t)D * (to_[p] + e) + c];
+                htu[(size_t)D * to_[p] + (size_t)c * nt + e] = tu[(size_t)D * (to_[p] + e) + c];
+            }
+    }
+
+    chunk_ = spec_.chunk > 0 ? ((spec_.chunk + 63) / 64) * 64 : 1024;
+    vchunk_ = 1024;
+    build_descs();
+    size_t part = 0, vpart = 0;
+    for (int p = 0; p < P; ++p) {
+        part = std::max<size_t>(part, desc_h_[p].po + (size_t)desc_h_[p].splits * 2 * D * spec_.n[p]);
+        vpart = std::max<size_t>(vpart, desc_h_[p].vpo + (size_t)(desc_h_[p].vsd + desc_h_[p].vst) * (2 * D + 2) *
+                                                             spec_.ne[p]);
+        vpart = std::max<size_t>(vpart, desc_h_[p].tpo + (size_t)desc_h_[p].tsplits * std::max(spec_.nt[p], 1));
+    }
+    // the point-kernel partial buffer is sized for the smallest tunable chunk
+    size_t part_max = 0;
+    for (int p = 0; p < P; ++p) part_max += (size_t)ceil_div(spec_.n[p], 256) * 2 * D * spec_.n[p];
+    part = std::max(part, part_max);
+
+    slab_.reserve(&desc_, P);
+    slab_.reserve(&q0_, hq.size());
+    slab_.reserve(&elm_, helm.size());
+    slab_.reserve(&ifo_, hifo.size());
+    slab_.reserve(&tc_, htc.size());
+    slab_.reserve(&tu_, htu.size());
+    slab_.reserve(&ta_, std::max<long long>(ttot_, 1));
+    slab_.reserve(&nde_, (size_t)spec_.T * D * ntot_);
+    slab_.reserve(&padj_, (size_t)D * ntot_);
+    slab_.reserve(&part_, part);
+    slab_.reserve(&vpart_, vpart);
+    slab_.reserve(&ecen_, (size_t)D * etot_);
+    slab_.reserve(&edir_, (size_t)D * etot_);
+    slab_.reserve(&evol_, etot_);
+    slab_.reserve(&dc_, (size_t)D * etot_);
+    slab_.reserve(&dd_, (size_t)D * etot_);
+    slab_.reserve(&efwd_, ntot_);
+    slab_.reserve(&evf_, etot_);
+    slab_.reserve(&tt_, P);
+    slab_.allocate();
+    LDDMM_HIP(hipMemcpyAsync(q0_, hq.data(), hq.size() * sizeof(double), hipMemcpyHostToDevice, st_));
+    LDDMM_HIP(hipMemcpyAsync(elm_, helm.data(), helm.size() * sizeof(int), hipMemcpyHostToDevice, st_));
+    LDDMM_HIP(hipMemcpyAsync(ifo_, hifo.data(), hifo.size() * sizeof(int), hipMemcpyHostToDevice, st_));
+    if (ttot_ > 0) {
+        LDDMM_HIP(hipMemcpyAsync(tc_, htc.data(), (size_t)D * ttot_ * sizeof(double), hipMemcpyHostToDevice, st_));
+        LDDMM_HIP(hipMemcpyAsync(tu_, htu.data(), (size_t)D * ttot_ * sizeof(double), hipMemcpyHostToDevice, st_));
+        LDDMM_HIP(hipMemcpyAsync(ta_, ta, (size_t)ttot_ * sizeof(double), hipMemcpyHostToDevice, st_));
+    }
+    LDDMM_HIP(hipMemcpyAsync(desc_, desc_h_.data(), P * sizeof(BatchDesc), hipMemcpyHostToDevice, st_));
+    LDDMM_HIP(hipStreamSynchronize(st_));
+    compute_target_terms();
+    if (spec_.chunk <= 0) autotune();
+}
+
+Batch::~Batch() = default;
+
+void Batch::build_descs()
+{
+    const int P = spec_.P(), D = spec_.dim;
+    desc_h_.assign(P, BatchDesc{});
+    long long po = 0, vpo = 0;
+    for (int p = 0; p < P; ++p) {
+        BatchDesc &d = desc_h_[p];
+        d.vo = vo_[p];
+        d.eo = eo_[p];
+        d.to = to_[p];
+        d.n = spec_.n[p];
+        d.ne = spec_.ne[p];
+        d.nt = spec_.nt[p];
+        d.chunk = chunk_;
+        d.splits = ceil_div(d.n, chunk_);
+        d.vchunk = vchunk_;
+        d.vsd = ceil_div(d.ne, vchunk_);
+        d.vst = d.nt > 0 ? ceil_div(d.nt, vchunk_) : 0;
+        d.tsplits = d.nt > 0 ? ceil_div(d.nt, vchunk_) : 0;
+        d.iw = 1.0 / spec_.width[p];
+        d.iw2 = 1.0 / (spec_.width[p] * spec_.width[p]);
+        d.po = po;
+        po += (long long)d.splits * 2 * D * d.n;
+        d.vpo = vpo;
+        d.tpo = vpo;   // the target-target pass runs once, before any evaluation
+        vpo += std::max((long long)(d.vsd + d.vst) * (2 * D + 2) * d.ne, (long long)d.tsplits * std::max(d.nt, 1));
+    }
+}
+
+double Batch::pairs_per_eval(int p) const
+{
+    const double n = spec_.n[p], T1 = spec_.T - 1;
+    return 2.0 * T1 * n * n + (double)spec_.ne[p] * spec_.ne[p] + (double)spec_.ne[p] * spec_.nt[p];
+}
+
+void Batch::build_tables(const std::vector<int> &prec_of)
+{
+    if (prec_of == tab_key_) return;
+    const int P = spec_.P();
+    Tables &T = *tab_;
+    std::vector<int4> kv[2], bwd[2], vf[2], vt, et;
+    std::vector<int> probs;
+    for (int p = 0; p < P; ++p) {
+        const int pr = prec_of[p];
+        if (pr < 0) continue;
+        const int g = pr == kFP32 ? 1 : 0;
+        const BatchDesc &d = desc_h_[p];
+        probs.push_back(p);
+        const int tkv = g ? 128 : 64;   // FP32 K.a keeps two targets per lane
+        for (int s = 0; s < d.splits; ++s) {
+            for (int t = 0; t < ceil_div(d.n, tkv); ++t) kv[g].push_back(make_int4(p, t, s, 0));
+            for (int t = 0; t < ceil_div(d.n, kTile); ++t) bwd[g].push_back(make_int4(p, t, s, 0));
+        }
+        for (int s = 0; s < d.vsd + d.vst; ++s)
+            for (int t = 0; t < ceil_div(d.ne, kTile); ++t) vf[g].push_back(make_int4(p, t, s, 0));
+        for (int t = 0; t < ceil_div(d.n, kTile); ++t) vt.push_back(make_int4(p, t, 0, 0));
+        for (int t = 0; t < ceil_div(d.ne, kTile); ++t) et.push_back(make_int4(p, t, 0, 0));
+    }
+    auto up = [&](DArr<int4> &a, int &n, const std::vector<int4> &h) {
+        n = (int)h.size();
+        a.ensure(h.size());
+        if (!h.empty())
+            LDDMM_HIP(hipMemcpyAsync(a.p, h.data(), h.size() * sizeof(int4), hipMemcpyHostToDevice, st_));
+    };
+    for (int g = 0; g < 2; ++g) {
+        up(T.kv[g], T.nkv[g], kv[g]);
+        up(T.bwd[g], T.nbwd[g], bwd[g]);
+        up(T.vf[g], T.nvf[g], vf[g]);
+    }
+    up(T.vtile, T.nvtile, vt);
+    up(T.etile, T.netile, et);
+    T.nprobs = (int)probs.size();
+    T.probs.ensure(probs.size());
+    if (!probs.empty())
+        LDDMM_HIP(hipMemcpyAsync(T.probs.p, probs.data(), probs.size() * sizeof(int), hipMemcpyHostToDevice, st_));
+    LDDMM_HIP(hipStreamSynchronize(st_));   // host vectors go out of scope
+    tab_key_ = prec_of;
+}
+
+void Batch::compute_target_terms()
+{
+    const int P = spec_.P(), D = spec_.dim;
+    std::vector<int4> items;
+    for (int p = 0; p < P; ++p) {
+        const BatchDesc &d = desc_h_[p];
+        for (int s = 0; s < d.tsplits; ++s)
+            for (int t = 0; t < ceil_div(d.nt, kTile); ++t) items.push_back(make_int4(p, t, s, 0));
+    }
+    BatchView v{};
+    v.desc = desc_;
+    v.vpart = vpart_;
+    v.tc = tc_;
+    v.tu = tu_;
+    v.ta = ta_;
+    DArr<int4> it;
+    if (!items.empty()) {
+        it.ensure(items.size());
+        LDDMM_HIP(hipMemcpyAsync(it.p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, st_));
+        dispatch_vf3(D, spec_.vp, [&](auto Dc, auto Cc, auto Uc) {
+            kb_tt<Dc(), Cc(), Uc()><<<(int)items.size(), kBlock, 0, st_>>>(v, inv_sq(spec_.vp.cen_width),
+                                                                           inv_sq(spec_.vp.dir_width), it.p);
+        });
+        LDDMM_LAUNCH_CHECK();
+    }
+    kb_tt_sum<<<P, kBlock, 0, st_>>>(v, tt_);
+    LDDMM_LAUNCH_CHECK();
+    LDDMM_HIP(hipStreamSynchronize(st_));
+}
+
+void Batch::eval(const double *alp, double *grd, double *f_dev, const std::vector<int> &prec_of)
+{
+    const int D = spec_.dim, T = spec_.T;
+    build_tables(prec_of);
+    const Tables &tb = *tab_;
+    if (tb.nprobs == 0) return;
+    BatchView v{desc_, nde_, padj_, part_, vpart_, alp, grd, q0_, tc_, tu_, ta_, elm_, ifo_, rows_, T,
+                spec_.h, spec_.w, ecen_, edir_, evol_, dc_, dd_, efwd_, evf_};
+
+    // q_0 of the listed problems
+    if (D == 2) kb_init_flow<2><<<tb.nvtile, kTile, 0, st_>>>(v, tb.vtile.p);
+    else kb_init_flow<3><<<tb.nvtile, kTile, 0, st_>>>(v, tb.vtile.p);
+    LDDMM_LAUNCH_CHECK();
+    auto each_prec = [&](auto &&fn) {
+        for (int g = 0; g < 2; ++g) {
+            if (g == 0) fn(g, PrecF64{});
+            else fn(g, PrecF32{});
+        }
+    };
+    auto dispatch = [&](auto &&fn) {
+        dispatch_order(spec_.order, [&](auto O) {
+            if (D == 2) fn(std::integral_constant<int, 2>{}, O);
+            else fn(std::integral_constant<int, 3>{}, O);
+        });
+    };
+    for (int t = 0; t < T - 1; ++t) {
+        {
+            ProfScope ps(kProfFwdPairs, st_);
+            each_prec([&](int g, auto Pc) {
+                if (!tb.nkv[g]) return;
+                dispatch([&](auto Dc, auto Oc) {
+                    constexpr int R = decltype(Pc)::fp32 ? 2 : 1;
+                    kb_kv<Dc(), Oc(), decltype(Pc), R><<<tb.nkv[g], kBlock, 0, st_>>>(v, t, tb.kv[g].p);
+                });
+                LDDMM_LAUNCH_CHECK();
+            });
+        }
+        ProfScope ps(kProfFwdReduce, st_);
+        if (D == 2) kb_fwd_reduce<2><<<tb.nvtile, kTile, 0, st_>>>(v, t, tb.vtile.p);
+        else kb_fwd_reduce<3><<<tb.nvtile, kTile, 0, st_>>>(v, t, tb.vtile.p);
+        LDDMM_LAUNCH_CHECK();
+    }
+    {
+        ProfScope ps(kProfVfGeometry, st_);
+        if (D == 2) kb_geometry<2><<<tb.netile, kTile, 0, st_>>>(v, tb.etile.p);
+        else kb_geometry<3><<<tb.netile, kTile, 0, st_>>>(v, tb.etile.p);
+        LDDMM_LAUNCH_CHECK();
+    }
+    {
+        ProfScope ps(kProfVfPairs, st_);
+        each_prec([&](int g, auto Pc) {
+            if (!tb.nvf[g]) return;
+            dispatch_vf3(D, spec_.vp, [&](auto Dc, auto Cc, auto Uc) {
+                kb_vf<Dc(), Cc(), Uc(), decltype(Pc)><<<tb.nvf[g], kBlock, 0, st_>>>(
+                    v, inv_sq(spec_.vp.cen_width), inv_sq(spec_.vp.dir_width), tb.vf[g].p);
+            });
+            LDDMM_LAUNCH_CHECK();
+        });
+    }
+    {
+        ProfScope ps(kProfVfFinalize, st_);
+        const double gscale = -4.0 * inv_sq(spec_.vp.cen_width);
+        if (D == 2) kb_vf_finalize<2><<<tb.netile, kTile, 0, st_>>>(v, gscale, tb.etile.p);
+        else kb_vf_finalize<3><<<tb.netile, kTile, 0, st_>>>(v, gscale, tb.etile.p);
+        LDDMM_LAUNCH_CHECK();
+    }
+    {
+        ProfScope ps(kProfVfGather, st_);
+        if (D == 2) kb_vf_gather<2><<<tb.nvtile, kTile, 0, st_>>>(v, tb.vtile.p);
+        else kb_vf_gather<3><<<tb.nvtile, kTile, 0, st_>>>(v, tb.vtile.p);
+        LDDMM_LAUNCH_CHECK();
+    }
+    {
+        ProfScope ps(kProfEnergy, st_);
+        kb_energy<<<tb.nprobs, kBlock, 0, st_>>>(v, tt_, tb.probs.p, f_dev);
+        LDDMM_LAUNCH_CHECK();
+    }
+    for (int t = T - 2; t >= 0; --t) {
+        {
+            ProfScope ps(kProfBwdPairs, st_);
+            each_prec([&](int g, auto Pc) {
+                if (!tb.nbwd[g]) return;
+                dispatch([&](auto Dc, auto Oc) {
+                    kb_bwd<Dc(), Oc(), decltype(Pc)><<<tb.nbwd[g], kBlock, 0, st_>>>(v, t, tb.bwd[g].p);
+                });
+                LDDMM_LAUNCH_CHECK();
+            });
+        }
+        ProfScope ps(kProfBwdReduce, st_);
+        if (D == 2) kb_bwd_reduce<2><<<tb.nvtile, kTile, 0, st_>>>(v, t, tb.vtile.p);
+        else kb_bwd_reduce<3><<<tb.nvtile, kTile, 0, st_>>>(v, t, tb.vtile.p);
+        LDDMM_LAUNCH_CHECK();
+    }
+}
+
+// Chunk autotuner: times one batched forward pair launch per candidate
+// source chunk (the work-item size) on this batch and keeps the fastest.
+// Results are cached per batch signature for the life of the process.
+void Batch::autotune()
+{
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, int, int, long long, long long>, int> cache;
+    long long sumsq = 0;
+    for (int n : spec_.n) sumsq += (long long)n * n;
+    const auto key = std::make_tuple(spec_.dim, spec_.order, spec_.prec, spec_.P(), ntot_, sumsq);
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            chunk_ = it->second;
+            build_descs();
+            LDDMM_HIP(hipMemcpyAsync(desc_, desc_h_.data(), desc_h_.size() * sizeof(BatchDesc),
+                                     hipMemcpyHostToDevice, st_));
+            LDDMM_HIP(hipStreamSynchronize(st_));
+            tab_key_.clear();
+            return;
+        }
+    }
+    const int cands[] = {256, 512, 1024, 2048};
+    const int P = spec_.P(), D = spec_.dim;
+    std::vector<int> all(P, spec_.prec);
+    DArr<double> alp;
+    alp.ensure((size_t)vtot_);
+    LDDMM_HIP(hipMemsetAsync(alp.p, 0, (size_t)vtot_ * sizeof(double), st_));
+    for (int p = 0; p < P; ++p)
+        LDDMM_HIP(hipMemcpyAsync(nde_ + (size_t)spec_.T * D * vo_[p], q0_ + (size_t)D * vo_[p],
+                                 (size_t)D * spec_.n[p] * sizeof(double), hipMemcpyDeviceToDevice, st_));
+    hipEvent_t e0, e1;
+    LDDMM_HIP(hipEventCreate(&e0));
+    LDDMM_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int best_chunk = chunk_;
+    for (int c : cands) {
+        chunk_ = c;
+        build_descs();
+        LDDMM_HIP(hipMemcpyAsync(desc_, desc_h_.data(), P * sizeof(BatchDesc), hipMemcpyHostToDevice, st_));
+        tab_key_.clear();
+        build_tables(all);
+        const Tables &tb = *tab_;
+        const int g = spec_.prec == kFP32 ? 1 : 0;
+        BatchView v{desc_, nde_, padj_, part_, vpart_, alp.p, nullptr, q0_, tc_, tu_, ta_, elm_, ifo_, rows_,
+                    spec_.T, spec_.h, spec_.w, ecen_, edir_, evol_, dc_, dd_, efwd_, evf_};
+        auto launch = [&] {
+            dispatch_order(spec_.order, [&](auto O) {
+                if (g) {
+                    if (D == 2) kb_kv<2, O(), PrecF32, 2><<<tb.nkv[1], kBlock, 0, st_>>>(v, 0, tb.kv[1].p);
+                    else kb_kv<3, O(), PrecF32, 2><<<tb.nkv[1], kBlock, 0, st_>>>(v, 0, tb.kv[1].p);
+                } else {
+                    if (D == 2) kb_kv<2, O(), PrecF64, 1><<<tb.nkv[0], kBlock, 0, st_>>>(v, 0, tb.kv[0].p);
+                    else kb_kv<3, O(), PrecF64, 1><<<tb.nkv[0], kBlock, 0, st_>>>(v, 0, tb.kv[0].p);
+                }
+            });
+            LDDMM_LAUNCH_CHECK();
+        };
+        launch();   // warm
+        LDDMM_HIP(hipEventRecord(e0, st_));
+        launch();
+        launch();
+        LDDMM_HIP(hipEventRecord(e1, st_));
+        LDDMM_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        LDDMM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            best_chunk = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    chunk_ = best_chunk;
+    build_descs();
+    LDDMM_HIP(hipMemcpyAsync(desc_, desc_h_.data(), P * sizeof(BatchDesc), hipMemcpyHostToDevice, st_));
+    LDDMM_HIP(hipStreamSynchronize(st_));
+    tab_key_.clear();
+    std::lock_guard<std::mutex> gl(mu);
+    cache[key] = chunk_;
+}
+
+std::vector<double> Batch::to_device_layout(const double *h) const
+{
+    const int D = spec_.dim, T1 = spec_.T - 1;
+    std::vector<double> out((size_t)vtot_);
+    for (int p = 0; p < spec_.P(); ++p) {
+        const int n = spec_.n[p];
+        const size_t off = (size_t)D * T1 * vo_[p];
+        for (int t = 0; t < T1; ++t)
+            for (int i = 0; i < n; ++i)
+                for (int c = 0; c < D; ++c)
+                    out[off + (size_t)t * D * n + (size_t)c * n + i] = h[off + (size_t)t * D * n + (size_t)i * D + c];
+    }
+    return out;
+}
+
+void Batch::to_host_layout(const std::vector<double> &dv, double *h) const
+{
+    const int D = spec_.dim, T1 = spec_.T - 1;
+    for (int p = 0; p < spec_.P(); ++p) {
+        const int n = spec_.n[p];
+        const size_t off = (size_t)D * T1 * vo_[p];
+        for (int t = 0; t < T1; ++t)
+            for (int i = 0; i < n; ++i)
+                for (int c = 0; c < D; ++c)
+                    h[off + (size_t)t * D * n + (size_t)i * D + c] = dv[off + (size_t)t * D * n + (size_t)c * n + i];
+    }
+}
+
+// ======================================================================
+// Batched L-BFGS: per-problem state machines advanced in rounds; every
+// round evaluates the current trial point of every active problem in one
+// batched evaluation.  The per-problem arithmetic is that of run_lbfgs
+// (Cpp/src/LBFGS.cpp, getDirection.cpp, lineSearch.cpp).
+// ======================================================================
+BatchRunResult Batch::run(double *x, const BatchLbfgsOptions &bo)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    const LbfgsOptions &o = bo.base;
+    const int P = spec_.P(), m = o.mem;
+    if (m < 1) throw Error(kErrArg, "memNum must be at least 1");
+    const long long V = vtot_;
+
+    // vector block table (fixed: per problem blocks of kVecBlk)
+    std::vector<int2> blk;
+    std::vector<int> first(P), count(P);
+    std::vector<long long> voff(P), vn(P);
+    for (int p = 0; p < P; ++p) {
+        voff[p] = var_offset(p);
+        vn[p] = varnum(p);
+        first[p] = (int)blk.size();
+        count[p] = ceil_div(vn[p], kVecBlk);
+        for (int b = 0; b < count[p]; ++b) blk.push_back(make_int2(p, b));
+    }
+    const int nent = (int)blk.size();
+    Slab s;
+    double *g, *d, *xt, *gt, *S, *Y, *sc, *sy, *yy, *rec, *part, *rel_d;
+    int2 *blk_d;
+    int *first_d, *count_d, *imask;
+    long long *voff_d, *vn_d;
+    // scalar block: f, gg, gd, dd, sd, yd  (P each)
+    enum { F = 0, GG, GD, DDs, SD, YD, NS };
+    s.reserve(&g, V);
+    s.reserve(&d, V);
+    s.reserve(&xt, V);
+    s.reserve(&gt, V);
+    s.reserve(&S, (size_t)V * m);
+    s.reserve(&Y, (size_t)V * m);
+    s.reserve(&sc, (size_t)NS * P);
+    s.reserve(&sy, (size_t)P * m);
+    s.reserve(&yy, (size_t)P * m);
+    s.reserve(&rec, (size_t)P * m);
+    s.reserve(&part, (size_t)kMaxBDots * nent);
+    s.reserve(&rel_d, P);
+    s.reserve(&blk_d, nent);
+    s.reserve(&first_d, P);
+    s.reserve(&count_d, P);
+    s.reserve(&voff_d, P);
+    s.reserve(&vn_d, P);
+    // int mask arrays: dir, trial, accept, newest slot, then per two-loop
+    // step the slot arrays of both loops and the negate flags
+    const int NM = 4 + 3 * m;
+    s.reserve(&imask, (size_t)NM * P);
+    s.allocate();
+    LDDMM_HIP(hipMemcpyAsync(blk_d, blk.data(), blk.size() * sizeof(int2), hipMemcpyHostToDevice, st_));
+    LDDMM_HIP(hipMemcpyAsync(first_d, first.data(), P * sizeof(int), hipMemcpyHostToDevice, st_));
+    LDDMM_HIP(hipMemcpyAsync(count_d, count.data(), P * sizeof(int), hipMemcpyHostToDevice, st_));
+    LDDMM_HIP(hipMemcpyAsync(voff_d, voff.data(), P * sizeof(long long), hipMemcpyHostToDevice, st_));
+    LDDMM_HIP(hipMemcpyAsync(vn_d, vn.data(), P * sizeof(long long), hipMemcpyHostToDevice, st_));
+    double *hs = nullptr;
+    int *hm = nullptr;
+    double *hrel = nullptr;
+    LDDMM_HIP(hipHostMalloc((void **)&hs, sizeof(double) * NS * P, hipHostMallocDefault));
+    LDDMM_HIP(hipHostMalloc((void **)&hm, sizeof(int) * NM * P, hipHostMallocDefault));
+    LDDMM_HIP(hipHostMalloc((void **)&hrel, sizeof(double) * P, hipHostMallocDefault));
+    struct HostFree {
+        void *a, *b, *c;
+        ~HostFree()
+        {
+            if (a) (void)hipHostFree(a);
+            if (b) (void)hipHostFree(b);
+            if (c) (void)hipHostFree(c);
+        }
+    } hf{hs, hm, hrel};
+    int *m_dir = imask, *m_trial = imask + P, *m_acc = imask + 2 * P, *m_new = imask + 3 * P;
+    int *m_slot = imask + 4 * P;            // 2*m arrays of P
+    int *m_neg = imask + (4 + 2 * m) * P;   // m arrays of P
+    int *h_dir = hm, *h_trial = hm + P, *h_acc = hm + 2 * P, *h_new = hm + 3 * P, *h_slot = hm + 4 * P,
+        *h_neg = hm + (4 + 2 * m) * P;
+    VecTab vt{blk_d, voff_d, vn_d};
+
+    // synchronous: the pinned host masks are rewritten right after
+    auto upload_masks = [&] {
+        LDDMM_HIP(hipMemcpyAsync(imask, hm, sizeof(int) * NM * P, hipMemcpyHostToDevice, st_));
+        LDDMM_HIP(hipStreamSynchronize(st_));
+    };
+    auto vec = [&](VecArgs a) {
+        a.m = m;
+        a.vtot = V;
+        ProfScope ps(kProfLbfgsVector, st_);
+        kb_vec<<<nent, kBlock, 0, st_>>>(a, vt);
+        LDDMM_LAUNCH_CHECK();
+    };
+    auto bdots = [&](int nd, const VRef *xs, const VRef *ys, const int *mask, double *out) {
+        BDotArgs a{};
+        for (int k = 0; k < nd; ++k) {
+            a.x[k] = xs[k];
+            a.y[k] = ys[k];
+        }
+        ProfScope ps(kProfLbfgsDots, st_);
+        kb_dots1<<<dim3(nent, nd), kBlock, 0, st_>>>(a, vt, mask, part, nent);
+        LDDMM_LAUNCH_CHECK();
+        kb_dots2<<<dim3(P, nd), kBlock, 0, st_>>>(part, nent, first_d, count_d, mask, out, P);
+        LDDMM_LAUNCH_CHECK();
+    };
+    auto fetch = [&](int slot0, int nslots) {
+        LDDMM_HIP(hipMemcpyAsync(hs + (size_t)slot0 * P, sc + (size_t)slot0 * P, sizeof(double) * nslots * P,
+                                 hipMemcpyDeviceToHost, st_));
+        LDDMM_HIP(hipStreamSynchronize(st_));
+    };
+    auto plain = [](const double *p) { return VRef{p, nullptr, 0}; };
+
+    // per-problem state
+    struct St {
+        bool active = true, need_dir = true;
+        int it = 0, new_idx = -1, status = 0, evals = 0, bis = 0, epd = 0, bis_max = 0, epd_max = 0, cnt = 0;
+        int prec = kFP64, fp32_iters = 0, reached = 0;
+        double f = 0, fprev = 0, glen = 0, slope = 0, dlen = 0, lo = 0, hi = DBL_MAX, rel = 1;
+    };
+    std::vector<St> st(P);
+    const int run_prec = bo.prec >= 0 ? bo.prec : spec_.prec;
+    if (run_prec != kFP64 && run_prec != kFP32) throw Error(kErrArg, "unknown precision policy");
+    const bool can_refine = o.fp64_refine && run_prec == kFP32;
+    std::vector<int> prec_of(P, run_prec);
+
+    // f, g at the start
+    eval(x, g, sc + (size_t)F * P, prec_of);
+    {
+        VRef xs[1] = {plain(g)}, ys[1] = {plain(g)};
+        bdots(1, xs, ys, nullptr, sc + (size_t)GG * P);
+    }
+    fetch(F, 2);
+    for (int p = 0; p < P; ++p) {
+        st[p].f = hs[F * P + p];
+        st[p].glen = std::sqrt(hs[GG * P + p]);
+        st[p].evals = 1;
+        st[p].prec = run_prec;
+        if (!(st[p].glen >= o.tol) || o.itr_max == 0) st[p].active = false;
+    }
+
+    // per-problem refine to FP64 (re-evaluate the current iterate)
+    auto refine = [&](const std::vector<int> &which) {
+        std::vector<int> pr(P, -1);
+        bool any = false;
+        for (int p : which) {
+            st[p].prec = kFP64;
+            pr[p] = kFP64;
+            any = true;
+        }
+        if (!any) return;
+        eval(x, g, sc + (size_t)F * P, pr);
+        for (int p = 0; p < P; ++p) h_acc[p] = pr[p] >= 0;
+        upload_masks();
+        VRef xs[1] = {plain(g)}, ys[1] = {plain(g)};
+        bdots(1, xs, ys, m_acc, sc + (size_t)GG * P);
+        fetch(F, 2);
+        for (int p : which) {
+            st[p].f = hs[F * P + p];
+            st[p].glen = std::sqrt(hs[GG * P + p]);
+            st[p].evals += 1;
+            st[p].need_dir = true;
+            st[p].lo = 0;
+            st[p].hi = DBL_MAX;
+            st[p].rel = 1;
+            st[p].cnt = 0;
+        }
+    };
+
+    int rounds = 0;
+    while (true) {
+        bool any = false;
+        for (int p = 0; p < P; ++p) any |= st[p].active;
+        if (!any) break;
+        ++rounds;
+
+        // ---- directions for the problems starting an iteration
+        std::vector<int> dirp;
+        for (int p = 0; p < P; ++p)
+            if (st[p].active && st[p].need_dir) dirp.push_back(p);
+        if (!dirp.empty()) {
+            int maxhis = 0;
+            std::memset(hm, 0, sizeof(int) * NM * P);
+            for (int k = 0; k < 2 * m; ++k)
+                for (int p = 0; p < P; ++p) h_slot[k * P + p] = -1;
+            for (int p = 0; p < P; ++p) h_new[p] = -1;
+            for (int p : dirp) {
+                const int it = st[p].it + 1;
+                const int his = it <= m ? it - 1 : m;
+                maxhis = std::max(maxhis, his);
+                h_dir[p] = 1;
+                h_trial[p] = his == 0;   // d = -g
+                h_acc[p] = his > 0;      // d = g, then the two loops
+                h_new[p] = his > 0 ? st[p].new_idx : -1;
+                for (int h = 0; h < his; ++h) {
+                    int idx = st[p].new_idx - h;
+                    if (idx < 0) idx += m;
+                    h_slot[h * P + p] = idx;
+                }
+                int old = his < m ? 0 : st[p].new_idx + 1;
+                if (old == m) old = 0;
+                for (int h = 0; h < his; ++h) {
+                    int idx = old + h;
+                    if (idx >= m) idx -= m;
+                    h_slot[(m + h) * P + p] = idx;
+                    h_neg[h * P + p] = (h == his - 1);
+                }
+            }
+            upload_masks();
+            {
+                VecArgs a{};
+                a.op = kCopyNeg;
+                a.d = d;
+                a.g = g;
+                a.mask = m_trial;
+                vec(a);
+                a.op = kCopy;
+                a.mask = m_acc;
+                vec(a);
+            }
+            for (int h = 0; h < maxhis; ++h) {
+                VRef xs[1] = {VRef{S, m_slot + h * P, V}}, ys[1] = {plain(d)};
+                bdots(1, xs, ys, nullptr, sc + (size_t)SD * P);
+                VecArgs a{};
+                a.op = kLoop1;
+                a.d = d;
+                a.Y = Y;
+                a.slot = m_slot + h * P;
+                a.sd = sc + (size_t)SD * P;
+                a.sy = sy;
+                a.rec = rec;
+                vec(a);
+            }
+            {
+                VecArgs a{};
+                a.op = kScaleH0;
+                a.d = d;
+                a.slot = m_new;
+                a.sy = sy;
+                a.yy = yy;
+                vec(a);
+            }
+            for (int h = 0; h < maxhis; ++h) {
+                VRef xs[1] = {VRef{Y, m_slot + (m + h) * P, V}}, ys[1] = {plain(d)};
+                bdots(1, xs, ys, nullptr, sc + (size_t)YD * P);
+                VecArgs a{};
+                a.op = kLoop2;
+                a.d = d;
+                a.S = S;
+                a.slot = m_slot + (m + h) * P;
+                a.neg = m_neg + h * P;   // negate on each problem's last second-loop step
+                a.yd = sc + (size_t)YD * P;
+                a.sy = sy;
+                a.rec = rec;
+                vec(a);
+            }
+            {
+                VRef xs[2] = {plain(g), plain(d)}, ys[2] = {plain(d), plain(d)};
+                bdots(2, xs, ys, m_dir, sc + (size_t)GD * P);
+            }
+            fetch(GD, 2);
+            std::vector<int> stalled;
+            for (int p : dirp) {
+                St &q = st[p];
+                q.slope = hs[GD * P + p];
+                q.need_dir = false;
+                if (q.slope > -DBL_MIN) {
+                    if (can_refine && q.prec == kFP32) {
+                        stalled.push_back(p);
+                        continue;
+                    }
+                    q.status = 1;
+                    q.active = false;
+                    continue;
+                }
+                q.dlen = std::sqrt(hs[DDs * P + p]);
+                q.bis_max = (int)std::fmax(50.0, std::log2(1e5 * q.dlen));
+                q.epd_max = (int)std::fmax(10.0, std::log2(1e5 / q.dlen));
+                q.lo = 0.0;
+                q.hi = DBL_MAX;
+                q.rel = 1.0;
+                q.bis = q.epd = q.cnt = 0;
+            }
+            refine(stalled);
+            if (!stalled.empty()) continue;
+        }
+
+        // ---- one trial point per active problem, one batched evaluation
+        std::vector<int> pr(P, -1);
+        bool anyt = false;
+        for (int p = 0; p < P; ++p) {
+            h_trial[p] = st[p].active && !st[p].need_dir;
+            hrel[p] = st[p].rel;
+            if (h_trial[p]) {
+                pr[p] = st[p].prec;
+                anyt = true;
+            }
+        }
+        if (!anyt) continue;
+        upload_masks();
+        LDDMM_HIP(hipMemcpyAsync(rel_d, hrel, sizeof(double) * P, hipMemcpyHostToDevice, st_));
+        {
+            VecArgs a{};
+            a.op = kTry;
+            a.d = xt;
+            a.x = x;
+            a.g = d;
+            a.rel = rel_d;
+            a.mask = m_trial;
+            vec(a);
+        }
+        eval(xt, gt, sc + (size_t)F * P, pr);
+        {
+            VRef xs[2] = {plain(gt), plain(gt)}, ys[2] = {plain(gt), plain(d)};
+            bdots(2, xs, ys, m_trial, sc + (size_t)GG * P);
+        }
+        fetch(F, 3);
+        std::vector<int> acc, stalled;
+        for (int p = 0; p < P; ++p) {
+            if (!h_trial[p]) continue;
+            St &q = st[p];
+            const double fn = hs[F * P + p], gn = std::sqrt(hs[GG * P + p]), sl = hs[GD * P + p];
+            ++q.cnt;
+            ++q.evals;
+            bool ok = false;
+            if (gn < o.tol) ok = true;
+            else if (fn >= q.f + o.wolfe1 * q.rel * q.slope) q.hi = q.rel;
+            else if (sl <= o.wolfe2 * q.slope) q.lo = q.rel;
+            else ok = true;
+            if (ok) {
+                acc.push_back(p);
+                q.it += 1;
+                const double fprev = q.f;
+                q.f = fn;
+                q.glen = gn;
+                if (q.prec == kFP32) ++q.fp32_iters;
+                q.need_dir = true;
+                // stop rules (checked after the accept below)
+                q.reached = 0;
+                const double tgt = p < (int)bo.targets.size() ? bo.targets[p] : o.target;
+                const bool has_t = p < (int)bo.targets.size() || o.has_target;
+                if (has_t && q.f <= tgt) q.reached = 1;
+                q.fprev = fprev;
+                continue;
+            }
+            if (q.hi == DBL_MAX) {
+                q.rel *= 2.0;
+                ++q.epd;
+            } else {
+                q.rel = 0.5 * (q.lo + q.hi);
+                ++q.bis;
+            }
+            if (!(q.bis <= q.bis_max && q.epd <= q.epd_max)) {
+                if (can_refine && q.prec == kFP32) {
+                    stalled.push_back(p);
+                } else {
+                    q.status = q.hi == DBL_MAX ? 2 : 3;
+                    q.active = false;
+                }
+            }
+        }
+        if (!acc.empty()) {
+            for (int p = 0; p < P; ++p) h_acc[p] = 0;
+            for (int p : acc) {
+                St &q = st[p];
+                q.new_idx = (q.new_idx == m - 1) ? 0 : q.new_idx + 1;
+                h_acc[p] = 1;
+                h_new[p] = q.new_idx;
+            }
+            upload_masks();
+            VecArgs a{};
+            a.op = kAccept;
+            a.xw = x;
+            a.gw = g;
+            a.xt = xt;
+            a.gt = gt;
+            a.Sw = S;
+            a.Yw = Y;
+            a.mask = m_acc;
+            a.slot = m_new;
+            vec(a);
+            VRef xs[2] = {VRef{S, m_new, V}, VRef{Y, m_new, V}}, ys[2] = {VRef{Y, m_new, V}, VRef{Y, m_new, V}};
+            bdots(2, xs, ys, m_acc, sc + (size_t)SD * P);
+            kb_store_pairs<<<ceil_div(P, 64), 64, 0, st_>>>(sc + (size_t)SD * P, sc + (size_t)YD * P, m_acc, m_new,
+                                                            sy, yy, m, P);
+            LDDMM_LAUNCH_CHECK();
+            for (int p : acc) {
+                St &q = st[p];
+                const double fprev = q.fprev;
+                q.rel = 1.0;
+                if (q.reached) {
+                    q.active = false;
+                    continue;
+                }
+                if (q.it >= o.itr_max || q.glen < o.tol) {
+                    q.active = false;
+                    continue;
+                }
+                const bool small = (fprev - q.f) < o.refine_rel_tol * std::fabs(q.f);
+                if (can_refine && q.prec == kFP32 && small) {
+                    stalled.push_back(p);
+                    continue;
+                }
+                if (o.rel_tol > 0.0 && (fprev - q.f) < o.rel_tol * std::fabs(q.f)) q.active = false;
+            }
+        }
+        refine(stalled);
+    }
+    LDDMM_HIP(hipStreamSynchronize(st_));
+    BatchRunResult r;
+    r.rounds = rounds;
+    r.prob.resize(P);
+    for (int p = 0; p < P; ++p) {
+        r.prob[p].f = st[p].f;
+        r.prob[p].gnorm = st[p].glen;
+        r.prob[p].iters = st[p].it;
+        r.prob[p].evals = st[p].evals;
+        r.prob[p].status = st[p].status;
+        r.prob[p].fp32_iters = st[p].fp32_iters;
+        r.prob[p].reached_target = st[p].reached;
+    }
+    r.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return r;
+}
+
+}  // namespace lddmm

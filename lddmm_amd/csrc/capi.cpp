@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/lddmm.h"
+#include "batch.hpp"
 #include "engine.hpp"
 
 using namespace lddmm;
@@ -455,6 +456,120 @@ int lddmm_objective_target_term(lddmm_objective *obj, double *value)
 }
 
 void lddmm_objective_destroy(lddmm_objective *obj) { delete obj; }
+
+// ---------------------------------------------------------------- batches
+struct lddmm_batch {
+    Stream st;
+    std::unique_ptr<Batch> b;
+};
+
+int lddmm_batch_create(lddmm_batch **out, int dimNum, int numProblems, const int32_t *ndeNums,
+                       const int32_t *dfmElmNums, const int32_t *tgtElmNums, const double *iniNdeMats,
+                       const int32_t *dfmElmVtxMats, const double *tgtCenPosMats, const double *tgtElmVolVecs,
+                       const double *tgtUniDirMats, const double *knlWidths, char cenKnlType, double cenKnlWidth,
+                       char dirKnlType, double dirKnlWidth, double disWgt, int knlOrder, double timeStp,
+                       int timeNum, int precision, int chunk)
+{
+    return guarded([&] {
+        require(out != nullptr, "null handle pointer");
+        *out = nullptr;
+        require(numProblems > 0, "numProblems must be positive");
+        require(ndeNums && dfmElmNums && tgtElmNums && iniNdeMats && dfmElmVtxMats && knlWidths, "null argument");
+        require(precision == LDDMM_PRECISION_FP64 || precision == LDDMM_PRECISION_FP32, "unknown precision");
+        BatchSpec sp;
+        sp.dim = dimNum;
+        sp.T = timeNum;
+        sp.order = knlOrder;
+        sp.prec = precision == LDDMM_PRECISION_FP32 ? kFP32 : kFP64;
+        sp.h = timeStp;
+        sp.w = disWgt;
+        sp.vp = VfParams{cenKnlType, dirKnlType, cenKnlWidth, dirKnlWidth};
+        sp.chunk = chunk;
+        long long tt = 0;
+        for (int p = 0; p < numProblems; ++p) {
+            sp.n.push_back(ndeNums[p]);
+            sp.ne.push_back(dfmElmNums[p]);
+            sp.nt.push_back(tgtElmNums[p]);
+            sp.width.push_back(knlWidths[p]);
+            tt += tgtElmNums[p];
+        }
+        require(tt == 0 || (tgtCenPosMats && tgtElmVolVecs && tgtUniDirMats), "null target array");
+        auto h = std::make_unique<lddmm_batch>();
+        h->b = std::make_unique<Batch>(sp, iniNdeMats, dfmElmVtxMats, tgtCenPosMats, tgtElmVolVecs, tgtUniDirMats,
+                                       h->st.s);
+        *out = h.release();
+        return 0;
+    });
+}
+
+int lddmm_batch_objgrd(lddmm_batch *b, double *fcnVals, double *grdStks, const double *alpStks)
+{
+    return guarded([&] {
+        require(b && fcnVals && grdStks && alpStks, "null argument");
+        Batch &B = *b->b;
+        auto ha = B.to_device_layout(alpStks);
+        DevBuf<double> a(ha.size()), g(ha.size()), f(B.P());
+        a.upload(ha.data(), b->st.s);
+        B.eval(a.p, g.p, f.p, std::vector<int>(B.P(), B.spec().prec));
+        g.download(ha.data(), b->st.s);
+        f.download(fcnVals, b->st.s);
+        b->st.sync();
+        B.to_host_layout(ha, grdStks);
+        return 0;
+    });
+}
+
+int lddmm_batch_run(lddmm_batch *b, const double *iniAlpStks, double *optAlpStks, int itrMax, double tolVal,
+                    double wolfe1, double wolfe2, int memNum, const lddmm_options *opts,
+                    const double *targetEnergies, lddmm_result *results, int *rounds)
+{
+    return guarded([&] {
+        require(b && optAlpStks, "null argument");
+        Batch &B = *b->b;
+        BatchLbfgsOptions bo;
+        bo.base = make_opts(itrMax, tolVal, wolfe1, wolfe2, memNum, 0, opts);
+        if (opts) bo.prec = precision_of(opts);
+        if (targetEnergies) bo.targets.assign(targetEnergies, targetEnergies + B.P());
+        std::vector<double> hx((size_t)B.varnum_total(), 0.0);
+        if (iniAlpStks) hx = B.to_device_layout(iniAlpStks);
+        DevBuf<double> x(hx.size());
+        x.upload(hx.data(), b->st.s);
+        BatchRunResult r = B.run(x.p, bo);
+        x.download(hx.data(), b->st.s);
+        b->st.sync();
+        B.to_host_layout(hx, optAlpStks);
+        if (rounds) *rounds = r.rounds;
+        if (results) {
+            for (int p = 0; p < B.P(); ++p) {
+                lddmm_result &q = results[p];
+                std::memset(&q, 0, sizeof(q));
+                q.fcn = r.prob[p].f;
+                q.grad_norm = r.prob[p].gnorm;
+                q.seconds = r.seconds;
+                q.pairs_per_eval = B.pairs_per_eval(p);
+                q.iterations = r.prob[p].iters;
+                q.evaluations = r.prob[p].evals;
+                q.status = r.prob[p].status;
+                q.fp32_iterations = r.prob[p].fp32_iters;
+                q.reached_target = r.prob[p].reached_target;
+            }
+        }
+        return 0;
+    });
+}
+
+int lddmm_batch_info(lddmm_batch *b, int *chunk, double *pairsPerEval)
+{
+    return guarded([&] {
+        require(b != nullptr, "null argument");
+        if (chunk) *chunk = b->b->chunk();
+        if (pairsPerEval)
+            for (int p = 0; p < b->b->P(); ++p) pairsPerEval[p] = b->b->pairs_per_eval(p);
+        return 0;
+    });
+}
+
+void lddmm_batch_destroy(lddmm_batch *b) { delete b; }
 
 // ---------------------------------------------------------------- operators
 int lddmm_multiply_kernel_cross(double *vlcMat, const double *lmkiMat, const double *lmkjMat, const double *alpMat,

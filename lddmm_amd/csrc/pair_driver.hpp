@@ -108,24 +108,28 @@ struct KernelFnT<ORDER, float> {
 
 constexpr int kWaves = kBlock / 64;
 
+// LDS bytes of one block: the staging tiles and the final wave reduction
+// share one buffer.
 template <class Op>
-__global__ __launch_bounds__(kBlock) void k_pairs(const typename Op::Args args, int ni, int nj, int chunk, int tiles,
-                                                  int split_offset, double *__restrict__ part)
+struct PairSmem {
+    static constexpr size_t stage =
+        sizeof(double) * Op::D * kTileJ + sizeof(typename Op::lds_t) * (Op::F > 0 ? Op::F : 1) * kTileJ;
+    static constexpr size_t red = sizeof(double) * kWaves * Op::OUT * 64 * Op::R;
+    static constexpr size_t bytes = stage > red ? stage : red;
+};
+
+// One block's work: targets [tile*TI, tile*TI + TI) against sources
+// [j0, j1); writes out[c*ni + i] for its targets.
+template <class Op>
+__device__ __forceinline__ void pairs_block(const typename Op::Args &args, int ni, int tile, int j0, int j1,
+                                            double *__restrict__ out, char *smem)
 {
     constexpr int D = Op::D, F = Op::F, OUT = Op::OUT, R = Op::R, TI = 64 * R;
     using lds_t = typename Op::lds_t;
-    // staging tiles and the final wave reduction share one LDS buffer
-    constexpr size_t kStage = sizeof(double) * D * kTileJ + sizeof(lds_t) * (F > 0 ? F : 1) * kTileJ;
-    constexpr size_t kRed = sizeof(double) * kWaves * OUT * TI;
-    __shared__ __align__(16) char smem[kStage > kRed ? kStage : kRed];
     auto sx = reinterpret_cast<double (*)[kTileJ]>(smem);
     auto sf = reinterpret_cast<lds_t (*)[kTileJ]>(smem + sizeof(double) * D * kTileJ);
     auto red = reinterpret_cast<double (*)[OUT][TI]>(smem);
-
-    const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const int tile = L % tiles, s = L / tiles;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int j0 = s * chunk, j1 = min(nj, j0 + chunk);
 
     typename Op::Tgt tg[R];
     double tot[R][OUT];
@@ -181,8 +185,19 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const typename Op::Args args, 
 #pragma unroll
         for (int w = 1; w < kWaves; ++w) v += red[w][c][il];
         const int i = tile * TI + il;
-        if (i < ni) part[((size_t)(s + split_offset) * OUT + c) * ni + i] = v;
+        if (i < ni) out[(size_t)c * ni + i] = v;
     }
+}
+
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_pairs(const typename Op::Args args, int ni, int nj, int chunk, int tiles,
+                                                  int split_offset, double *__restrict__ part)
+{
+    __shared__ __align__(16) char smem[PairSmem<Op>::bytes];
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int tile = L % tiles, s = L / tiles;
+    const int j0 = s * chunk, j1 = min(nj, j0 + chunk);
+    pairs_block<Op>(args, ni, tile, j0, j1, part + (size_t)(s + split_offset) * Op::OUT * ni, smem);
 }
 
 // Split plan: target tiles of TI, source chunks (multiples of 64) so that

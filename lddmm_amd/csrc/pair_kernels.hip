@@ -8,289 +8,13 @@
 // steps of objgrd (Cpp/src/objgrd.cpp:45-87).
 #include "kernels.hpp"
 #include "pair_driver.hpp"
+#include "pair_ops.hpp"
 
 namespace lddmm {
 
 namespace {
 
-// ---------------------------------------------------------------------------
-// v_i = sum_j k(qi_i, qj_j) a_j   (forward flow, multiplyKernel)
-// ---------------------------------------------------------------------------
-template <int D_, int ORDER, class P, int R_>
-struct OpKV {
-    static constexpr int D = D_, F = D_, OUT = D_, R = R_;
-    using real = typename P::real;
-    using lds_t = real;
-    struct Args {
-        const double *qi, *qj, *a;
-        int ni, nj;
-        real iw, iw2;
-    };
-    struct Tgt {
-        double x[D];
-    };
-    struct Acc {
-        real v[D];
-    };
-    __device__ __forceinline__ static void load_src(const Args &A, int j, double (&x)[D], lds_t (&f)[F])
-    {
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            x[c] = A.qj[(size_t)c * A.nj + j];
-            f[c] = (real)A.a[(size_t)c * A.nj + j];
-        }
-    }
-    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t)
-    {
-#pragma unroll
-        for (int c = 0; c < D; ++c) t.x[c] = i >= 0 ? A.qi[(size_t)c * A.ni + i] : 0.0;
-    }
-    __device__ __forceinline__ static void zero(Acc &a)
-    {
-#pragma unroll
-        for (int c = 0; c < D; ++c) a.v[c] = real(0);
-    }
-    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const double (&xj)[D],
-                                                const lds_t (&fj)[F], Acc &acc)
-    {
-        real r2 = real(0);
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            real df = (real)(t.x[c] - xj[c]);
-            r2 += df * df;
-        }
-        real k = KernelFnT<ORDER, real>::value(r2, A.iw, A.iw2);
-#pragma unroll
-        for (int c = 0; c < D; ++c) acc.v[c] += k * fj[c];
-    }
-    __device__ __forceinline__ static void flush(const Acc &a, double (&tot)[OUT])
-    {
-#pragma unroll
-        for (int c = 0; c < D; ++c) tot[c] += (double)a.v[c];
-    }
-};
-
-// ---------------------------------------------------------------------------
-// Fused backward pass of the adjoint (Cpp/src/objgrd.cpp:76-84): one kernel
-// evaluation per pair feeds K(q) amp and the dq sum of l^T K(q) r with
-// amp = a - p, l = amp - a/2, r = a.  Outputs: [K amp (D), dq (D)].
-// ---------------------------------------------------------------------------
-template <int D_, int ORDER, class P>
-struct OpBwd {
-    static constexpr int D = D_, F = 3 * D_, OUT = 2 * D_, R = 1;
-    using real = typename P::real;
-    using lds_t = real;
-    struct Args {
-        const double *q, *a, *p;
-        int n;
-        real iw, iw2;
-    };
-    struct Tgt {
-        double x[D];
-        real a[D], l[D];
-    };
-    struct Acc {
-        real g[D], pd[D];
-    };
-    __device__ __forceinline__ static void load_src(const Args &A, int j, double (&x)[D], lds_t (&f)[F])
-    {
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            const size_t o = (size_t)c * A.n + j;
-            double aj = A.a[o], amp = aj - A.p[o];
-            x[c] = A.q[o];
-            f[c] = (real)aj;
-            f[D + c] = (real)amp;
-            f[2 * D + c] = (real)(amp - 0.5 * aj);
-        }
-    }
-    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t)
-    {
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            const size_t o = (size_t)c * A.n + (i >= 0 ? i : 0);
-            double ai = i >= 0 ? A.a[o] : 0.0, amp = ai - (i >= 0 ? A.p[o] : 0.0);
-            t.x[c] = i >= 0 ? A.q[o] : 0.0;
-            t.a[c] = (real)ai;
-            t.l[c] = (real)(amp - 0.5 * ai);
-        }
-    }
-    __device__ __forceinline__ static void zero(Acc &a)
-    {
-#pragma unroll
-        for (int c = 0; c < D; ++c) a.g[c] = a.pd[c] = real(0);
-    }
-    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const double (&xj)[D],
-                                                const lds_t (&fj)[F], Acc &acc)
-    {
-        real df[D], r2 = real(0);
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            df[c] = (real)(t.x[c] - xj[c]);
-            r2 += df[c] * df[c];
-        }
-        real k, cc;
-        KernelFnT<ORDER, real>::eval(r2, A.iw, A.iw2, k, cc);
-        real lr = real(0);
-#pragma unroll
-        for (int c = 0; c < D; ++c) lr += t.l[c] * fj[c];
-#pragma unroll
-        for (int c = 0; c < D; ++c) lr += fj[2 * D + c] * t.a[c];
-        const real f = lr * cc;
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            acc.g[c] += k * fj[D + c];
-            acc.pd[c] += f * df[c];
-        }
-    }
-    __device__ __forceinline__ static void flush(const Acc &a, double (&tot)[OUT])
-    {
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            tot[c] += (double)a.g[c];
-            tot[D + c] += (double)a.pd[c];
-        }
-    }
-};
-
-// ---------------------------------------------------------------------------
-// dq of l^T K(q) r over one point set (Cpp/src/dqKernel.cpp:31-305)
-// ---------------------------------------------------------------------------
-template <int D_, int ORDER, class P>
-struct OpDqSelf {
-    static constexpr int D = D_, F = 2 * D_, OUT = D_, R = 1;
-    using real = typename P::real;
-    using lds_t = real;
-    struct Args {
-        const double *q, *l, *r;
-        int n;
-        real iw, iw2;
-    };
-    struct Tgt {
-        double x[D];
-        real l[D], r[D];
-    };
-    struct Acc {
-        real pd[D];
-    };
-    __device__ __forceinline__ static void load_src(const Args &A, int j, double (&x)[D], lds_t (&f)[F])
-    {
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            const size_t o = (size_t)c * A.n + j;
-            x[c] = A.q[o];
-            f[c] = (real)A.l[o];
-            f[D + c] = (real)A.r[o];
-        }
-    }
-    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t)
-    {
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            const size_t o = (size_t)c * A.n + (i >= 0 ? i : 0);
-            t.x[c] = i >= 0 ? A.q[o] : 0.0;
-            t.l[c] = i >= 0 ? (real)A.l[o] : real(0);
-            t.r[c] = i >= 0 ? (real)A.r[o] : real(0);
-        }
-    }
-    __device__ __forceinline__ static void zero(Acc &a)
-    {
-#pragma unroll
-        for (int c = 0; c < D; ++c) a.pd[c] = real(0);
-    }
-    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const double (&xj)[D],
-                                                const lds_t (&fj)[F], Acc &acc)
-    {
-        real df[D], r2 = real(0);
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            df[c] = (real)(t.x[c] - xj[c]);
-            r2 += df[c] * df[c];
-        }
-        real k, cc;
-        KernelFnT<ORDER, real>::eval(r2, A.iw, A.iw2, k, cc);
-        real lr = real(0);
-#pragma unroll
-        for (int c = 0; c < D; ++c) lr += t.l[c] * fj[D + c];
-#pragma unroll
-        for (int c = 0; c < D; ++c) lr += fj[c] * t.r[c];
-        const real f = lr * cc;
-#pragma unroll
-        for (int c = 0; c < D; ++c) acc.pd[c] += f * df[c];
-    }
-    __device__ __forceinline__ static void flush(const Acc &a, double (&tot)[OUT])
-    {
-#pragma unroll
-        for (int c = 0; c < D; ++c) tot[c] += (double)a.pd[c];
-    }
-};
-
-// ---------------------------------------------------------------------------
-// Cross dq: sum_j (u_i.v_j) c_ij (qi_i - qj_j) (Cpp/src/dqKernel.cpp:307-591;
-// the qj-side gradient is the same sum with the roles of the sets swapped)
-// ---------------------------------------------------------------------------
-template <int D_, int ORDER, class P>
-struct OpDqCross {
-    static constexpr int D = D_, F = D_, OUT = D_, R = 1;
-    using real = typename P::real;
-    using lds_t = real;
-    struct Args {
-        const double *qi, *u, *qj, *v;
-        int ni, nj;
-        real iw, iw2;
-    };
-    struct Tgt {
-        double x[D];
-        real u[D];
-    };
-    struct Acc {
-        real pd[D];
-    };
-    __device__ __forceinline__ static void load_src(const Args &A, int j, double (&x)[D], lds_t (&f)[F])
-    {
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            x[c] = A.qj[(size_t)c * A.nj + j];
-            f[c] = (real)A.v[(size_t)c * A.nj + j];
-        }
-    }
-    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t)
-    {
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            t.x[c] = i >= 0 ? A.qi[(size_t)c * A.ni + i] : 0.0;
-            t.u[c] = i >= 0 ? (real)A.u[(size_t)c * A.ni + i] : real(0);
-        }
-    }
-    __device__ __forceinline__ static void zero(Acc &a)
-    {
-#pragma unroll
-        for (int c = 0; c < D; ++c) a.pd[c] = real(0);
-    }
-    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const double (&xj)[D],
-                                                const lds_t (&fj)[F], Acc &acc)
-    {
-        real df[D], r2 = real(0);
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            df[c] = (real)(t.x[c] - xj[c]);
-            r2 += df[c] * df[c];
-        }
-        real k, cc;
-        KernelFnT<ORDER, real>::eval(r2, A.iw, A.iw2, k, cc);
-        real lr = real(0);
-#pragma unroll
-        for (int c = 0; c < D; ++c) lr += t.u[c] * fj[c];
-        const real f = lr * cc;
-#pragma unroll
-        for (int c = 0; c < D; ++c) acc.pd[c] += f * df[c];
-    }
-    __device__ __forceinline__ static void flush(const Acc &a, double (&tot)[OUT])
-    {
-#pragma unroll
-        for (int c = 0; c < D; ++c) tot[c] += (double)a.pd[c];
-    }
-};
+using namespace ops;
 
 // ---------------------------------------------------------------------------
 // Dense kernel matrix (Cpp/src/computeKernel.cpp:246-470 layout K[j*ni + i])

@@ -23,181 +23,13 @@
 // it is evaluated once per problem, in FP64, and cached.
 #include "kernels.hpp"
 #include "pair_driver.hpp"
+#include "varifold_ops.hpp"
 
 namespace lddmm {
 
 namespace {
 
-enum { kCenG = 0, kCenC = 1 };
-enum { kDirB = 0, kDirL = 1, kDirO = 2, kDirU = 3 };
-
-template <int CEN, typename real>
-__device__ __forceinline__ real cen_value(real r2, real inv_cs2)
-{
-    if constexpr (CEN == kCenG) {
-        if constexpr (sizeof(real) == 4) return __builtin_amdgcn_exp2f(-r2 * inv_cs2 * 1.4426950408889634f);
-        else return exp(-r2 * inv_cs2);
-    } else {
-        return real(1) / (real(1) + r2 * inv_cs2);
-    }
-}
-
-template <typename real>
-__device__ __forceinline__ real ex(real x)
-{
-    if constexpr (sizeof(real) == 4) return __builtin_amdgcn_exp2f(x * 1.4426950408889634f);
-    else return exp(x);
-}
-
-template <int DIR, typename real>
-__device__ __forceinline__ real dir_value(real ang, real inv_ds2, real &dku)
-{
-    if constexpr (DIR == kDirB) {
-        dku = real(2) * ang;
-        return ang * ang;
-    } else if constexpr (DIR == kDirL) {
-        dku = real(1);
-        return ang;
-    } else if constexpr (DIR == kDirO) {
-        real f = ex<real>(real(2) * (ang - real(1)) * inv_ds2);
-        dku = real(2) * f * inv_ds2;
-        return f;
-    } else {
-        real f = ex<real>(real(2) * (ang * ang - real(1)) * inv_ds2);
-        dku = real(4) * ang * f * inv_ds2;
-        return f;
-    }
-}
-
-template <int D_, int CEN, int DIR, class P>
-struct OpVf {
-    static constexpr int D = D_, F = D_ + 1, OUT = 2 * D_ + 2, R = 1;
-    using real = typename P::real;
-    using lds_t = real;
-    struct Args {
-        const double *ci, *ui;   // targets: deformed element centres / directions
-        int ni;
-        const double *cj, *uj, *aj;   // sources
-        int nj;
-        real ics2, ids2;
-    };
-    struct Tgt {
-        double c[D];
-        real u[D];
-    };
-    struct Acc {
-        real e, s, g[D], u[D];
-    };
-    __device__ __forceinline__ static void load_src(const Args &A, int j, double (&x)[D], lds_t (&f)[F])
-    {
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            x[c] = A.cj[(size_t)c * A.nj + j];
-            f[c] = (real)A.uj[(size_t)c * A.nj + j];
-        }
-        f[D] = (real)A.aj[j];
-    }
-    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t)
-    {
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            t.c[c] = i >= 0 ? A.ci[(size_t)c * A.ni + i] : 0.0;
-            t.u[c] = i >= 0 ? (real)A.ui[(size_t)c * A.ni + i] : real(0);
-        }
-    }
-    __device__ __forceinline__ static void zero(Acc &a)
-    {
-        a.e = a.s = real(0);
-#pragma unroll
-        for (int c = 0; c < D; ++c) a.g[c] = a.u[c] = real(0);
-    }
-    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const double (&xj)[D],
-                                                const lds_t (&fj)[F], Acc &acc)
-    {
-        real df[D], r2 = real(0), ang = real(0);
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            df[c] = (real)(t.c[c] - xj[c]);
-            r2 += df[c] * df[c];
-            ang += t.u[c] * fj[c];
-        }
-        const real kc = cen_value<CEN, real>(r2, A.ics2);
-        real dku;
-        const real ku = dir_value<DIR, real>(ang, A.ids2, dku);
-        const real w = kc * fj[D];
-        const real wk = w * ku;
-        const real gw = (CEN == kCenG) ? wk : wk * kc;
-        const real wd = w * dku;
-        acc.e += wk;
-        acc.s += wk - wd * ang;
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            acc.g[c] += gw * df[c];
-            acc.u[c] += wd * fj[c];
-        }
-    }
-    __device__ __forceinline__ static void flush(const Acc &a, double (&tot)[OUT])
-    {
-        tot[0] += (double)a.e;
-        tot[1] += (double)a.s;
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            tot[2 + c] += (double)a.g[c];
-            tot[2 + D + c] += (double)a.u[c];
-        }
-    }
-};
-
-// value-only: sum_j kc ku a_j (target-target term)
-template <int D_, int CEN, int DIR>
-struct OpVfValue {
-    static constexpr int D = D_, F = D_ + 1, OUT = 1, R = 1;
-    using real = double;
-    using lds_t = double;
-    struct Args {
-        const double *c, *u, *a;
-        int n;
-        double ics2, ids2;
-    };
-    struct Tgt {
-        double c[D], u[D];
-    };
-    struct Acc {
-        double e;
-    };
-    __device__ __forceinline__ static void load_src(const Args &A, int j, double (&x)[D], lds_t (&f)[F])
-    {
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            x[c] = A.c[(size_t)c * A.n + j];
-            f[c] = A.u[(size_t)c * A.n + j];
-        }
-        f[D] = A.a[j];
-    }
-    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t)
-    {
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            t.c[c] = i >= 0 ? A.c[(size_t)c * A.n + i] : 0.0;
-            t.u[c] = i >= 0 ? A.u[(size_t)c * A.n + i] : 0.0;
-        }
-    }
-    __device__ __forceinline__ static void zero(Acc &a) { a.e = 0.0; }
-    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const double (&xj)[D],
-                                                const lds_t (&fj)[F], Acc &acc)
-    {
-        double r2 = 0.0, ang = 0.0;
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            double df = t.c[c] - xj[c];
-            r2 += df * df;
-            ang += t.u[c] * fj[c];
-        }
-        double dku;
-        acc.e += cen_value<CEN, double>(r2, A.ics2) * dir_value<DIR, double>(ang, A.ids2, dku) * fj[D];
-    }
-    __device__ __forceinline__ static void flush(const Acc &a, double (&tot)[OUT]) { tot[0] += a.e; }
-};
+using namespace ops;
 
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_elm_geometry(const double *__restrict__ q, int nq,

@@ -228,6 +228,39 @@ def sphere_cube(subdivisions=5, cells=58):
                 dir_width=0.0, dis_wgt=100.0, knl_order=1, knl_width=0.3, time_stp=0.05, time_num=21)
 
 
+_GEO_CACHE = {}
+
+
+def _geodesic_cached(freq):
+    if freq not in _GEO_CACHE:
+        _GEO_CACHE[freq] = geodesic_sphere(freq)
+    return _GEO_CACHE[freq]
+
+
+def batch_problems(count, seed, nmin=500, nmax=20000, jitter=0.01):
+    """Uneven batch of 3D registrations: vertex counts log-uniform in
+    [nmin, nmax] (rounded to a geodesic sphere of frequency
+    f = round(sqrt((N - 2) / 10)), N = 10 f^2 + 2); sources are jittered
+    unit spheres (Gaussian vertex noise of std `jitter`), targets seeded
+    superquadrics of the same frequency; deformation kernel widths uniform
+    in [0.15, 0.35]."""
+    u = uniform(seed, 2 * count)
+    out = []
+    for p in range(count):
+        nn = np.exp(np.log(nmin) + (np.log(nmax) - np.log(nmin)) * u[2 * p])
+        f = max(1, int(round(np.sqrt((nn - 2) / 10.0))))
+        s, tri = _geodesic_cached(f)
+        q = s + jitter * normal(seed * 1000 + 7 * p + 1, s.size).reshape(s.shape)
+        tq, ttri = superquadric(f, seed * 1000 + 7 * p + 3)
+        tc, tu, ta = triangle_varifold(tq, ttri)
+        out.append(dict(q0=q, elm=tri, tgt_cen=tc, tgt_dir=tu, tgt_vol=ta, knl_width=0.15 + 0.2 * u[2 * p + 1]))
+    return out
+
+
+BATCH_KERNELS = dict(cen_type="G", cen_width=0.1, dir_type="B", dir_width=0.0, dis_wgt=100.0, knl_order=-1,
+                     time_stp=0.05, time_num=21)
+
+
 def superquadric_pair(freq, seed, knl_width=0.2):
     """Single large 3D problem: two independent seeded superquadrics."""
     q, tri = superquadric(freq, seed)

@@ -1,0 +1,100 @@
+"""Batched uneven registrations on the GPU: every problem of a batch must
+give what the single-problem path (itself pinned to the oracle) gives."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_err
+from lddmm_amd import geometry as G
+
+pytestmark = pytest.mark.gpu
+
+
+def small_batch():
+    probs = G.batch_problems(5, seed=3, nmin=12, nmax=700)
+    probs.append(G.batch_problems(1, seed=9, nmin=2500, nmax=2600)[0])
+    return probs
+
+
+def kernels(T=6):
+    k = dict(G.BATCH_KERNELS)
+    k["time_num"], k["time_stp"] = T, 1.0 / (T - 1)
+    return k
+
+
+def single(gpu, p, k, precision="fp64"):
+    ifo = oracle.get_elm_ifo(p["elm"], len(p["q0"]))
+    return gpu.Objective(p["q0"], p["elm"], ifo, p["tgt_cen"], p["tgt_vol"], p["tgt_dir"], k["cen_type"],
+                         k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"], k["knl_order"], p["knl_width"],
+                         k["time_stp"], k["time_num"], precision=precision), ifo
+
+
+@pytest.mark.parametrize("chunk", [0, 64, 512])
+def test_batched_objgrd_matches_single_problems(gpu, chunk):
+    probs, k = small_batch(), kernels()
+    print("batch objgrd chunk=%d sizes=%s" % (chunk, [len(p["q0"]) for p in probs]), flush=True)
+    B = gpu.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"], k["knl_order"],
+                  k["time_stp"], k["time_num"], chunk=chunk)
+    ch, pairs = B.info()
+    assert ch in ((256, 512, 1024, 2048) if chunk == 0 else (chunk,))
+    alps = [0.05 * G.normal(40 + i, B.sizes[i]).reshape(k["time_num"] - 1, B.n[i], 3) for i in range(B.P)]
+    f, g = B.objgrd(alps)
+    for i, p in enumerate(probs):
+        obj, _ = single(gpu, p, k)
+        fs, gs = obj(alps[i])
+        assert abs(f[i] - fs) <= 1e-12 * abs(fs), (i, f[i], fs)
+        assert rel_err(g[i].reshape(-1), gs) <= 1e-11, (i, rel_err(g[i].reshape(-1), gs))
+        assert pairs[i] == obj.pairs_per_eval()
+        obj.close()
+    B.close()
+
+
+def test_batched_fp32_objgrd_error(gpu):
+    probs, k = small_batch(), kernels()
+    B = gpu.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"], k["knl_order"],
+                  k["time_stp"], k["time_num"], precision="fp32", chunk=256)
+    alps = [0.05 * G.normal(60 + i, B.sizes[i]).reshape(k["time_num"] - 1, B.n[i], 3) for i in range(B.P)]
+    f, g = B.objgrd(alps)
+    for i, p in enumerate(probs):
+        obj, _ = single(gpu, p, k)
+        fs, gs = obj(alps[i])
+        e_rel, g_rel = abs(f[i] - fs) / abs(fs), rel_err(g[i].reshape(-1), gs)
+        print("batched fp32 problem %d (N=%d): energy %.2e gradient %.2e" % (i, B.n[i], e_rel, g_rel), flush=True)
+        assert e_rel <= 1e-6 and g_rel <= 1e-4
+        obj.close()
+
+
+def test_batched_run_matches_single_runs(gpu):
+    probs, k = small_batch(), kernels()
+    B = gpu.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"], k["knl_order"],
+                  k["time_stp"], k["time_num"], chunk=256)
+    alps, infos, rounds = B.run(8, 0.0, 0.0, 0.5, 5)
+    print("batched run: %d rounds, iterations %s" % (rounds, [i["iterations"] for i in infos]), flush=True)
+    for i, p in enumerate(probs):
+        obj, ifo = single(gpu, p, k)
+        a1, i1 = gpu.run(p["q0"], None, p["elm"], ifo, p["tgt_cen"], p["tgt_vol"], p["tgt_dir"], k["cen_type"],
+                         k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"], k["knl_order"], p["knl_width"],
+                         k["time_stp"], k["time_num"], 8, 0.0, 0.0, 0.5, 5)
+        assert infos[i]["status"] == i1["status"] and infos[i]["iterations"] == i1["iterations"]
+        assert abs(infos[i]["fcn"] - i1["fcn"]) <= 1e-8 * abs(i1["fcn"]), (i, infos[i]["fcn"], i1["fcn"])
+        fb, _ = obj(alps[i])
+        assert abs(fb - infos[i]["fcn"]) <= 1e-12 * abs(fb)
+        obj.close()
+
+
+def test_batched_run_relative_decrease_stop_and_fp32(gpu):
+    probs, k = small_batch(), kernels()
+    B = gpu.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"], k["knl_order"],
+                  k["time_stp"], k["time_num"], chunk=256)
+    a64, i64, r64 = B.run(300, 0.0, 0.0, 0.5, 5, rel_energy_tol=1e-6)
+    a32, i32, r32 = B.run(300, 0.0, 0.0, 0.5, 5, precision="fp32", rel_energy_tol=1e-6)
+    print("fp64 rounds %d its %s; fp32 rounds %d its %s fp32-its %s" %
+          (r64, [i["iterations"] for i in i64], r32, [i["iterations"] for i in i32],
+           [i["fp32_iterations"] for i in i32]), flush=True)
+    f64, _ = B.objgrd(a64)
+    f32, _ = B.objgrd(a32)
+    for i in range(B.P):
+        assert i64[i]["status"] == 0 and i32[i]["status"] == 0
+        assert i64[i]["iterations"] < 300
+        # both stopped on a relative decrease below 1e-6: energies agree to the scale of that stop
+        assert abs(f32[i] - f64[i]) <= 2e-3 * abs(f64[i]), (i, f32[i], f64[i])
