@@ -103,6 +103,49 @@ def cpu_baseline(w, ifo, pairs_per_eval):
                 seconds_per_eval=t_eval)
 
 
+def batch_bench(count, nmin, nmax, itr, seed=4):
+    """Uneven batch (the batched-registration workload, scaled by `count`
+    and the size range): plain FP64 registrations one after another
+    (zero initial momentum, stop at a relative energy decrease below 1e-6 or
+    `itr` iterations) against the batched FP64 run and the batched FP32
+    policy run to each plain run's final energy within 1e-6 relative."""
+    probs = G.batch_problems(count, seed, nmin, nmax)
+    k = G.BATCH_KERNELS
+    seq = []
+    t0 = time.perf_counter()
+    for p in probs:
+        ifo = lddmm_amd.getElmIfo(p["elm"], len(p["q0"]), one_based=False)
+        _, info = lddmm_amd.run(p["q0"], None, p["elm"], ifo, p["tgt_cen"], p["tgt_vol"], p["tgt_dir"],
+                                k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
+                                k["knl_order"], p["knl_width"], k["time_stp"], k["time_num"], itr, 0.0, 0.0, 0.5, 5,
+                                rel_energy_tol=1e-6)
+        seq.append(info)
+    t_seq = time.perf_counter() - t0
+    e_ref = np.array([i["fcn"] for i in seq])
+    B = lddmm_amd.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
+                        k["knl_order"], k["time_stp"], k["time_num"], precision="fp32", chunk=0)
+    chunk, pairs = B.info()
+    a64, i64, r64 = B.run(itr, 0.0, 0.0, 0.5, 5, precision="fp64", rel_energy_tol=1e-6)
+    a32, i32, r32 = B.run(3 * itr, 0.0, 0.0, 0.5, 5, precision="fp32", targets=e_ref * (1.0 + 1e-6))
+    B64 = lddmm_amd.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
+                          k["knl_order"], k["time_stp"], k["time_num"], precision="fp64", chunk=chunk)
+    f32_in_fp64, _ = B64.objgrd(a32)
+    rel = (f32_in_fp64 - e_ref) / np.abs(e_ref)
+    t64, t32 = i64[0]["seconds"], i32[0]["seconds"]
+    return dict(problems=count, n_range=[int(min(B.n)), int(max(B.n))], chunk=chunk,
+                pairs_per_eval_total=float(pairs.sum()),
+                plain_fp64_sequential=dict(seconds=t_seq, registrations_per_s=count / t_seq,
+                                           median_iterations=float(np.median([i["iterations"] for i in seq]))),
+                batched_fp64=dict(seconds=t64, registrations_per_s=count / t64, rounds=r64,
+                                  median_iterations=float(np.median([i["iterations"] for i in i64]))),
+                batched_fp32=dict(seconds=t32, registrations_per_s=count / t32, rounds=r32,
+                                  median_iterations=float(np.median([i["iterations"] for i in i32])),
+                                  reached_target=int(sum(i["reached_target"] for i in i32)),
+                                  within_1e6=int(np.sum(rel <= 1e-6 + 1e-8)),
+                                  max_rel_energy_vs_plain=float(rel.max())),
+                speedup_batched_fp32_vs_plain_fp64=t_seq / t32)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +154,9 @@ def main():
     ap.add_argument("--config", default="sphere_cube")
     ap.add_argument("--no-registration", action="store_true")
     ap.add_argument("--reg-iters", type=int, default=0, help="L-BFGS iterations of the registration (0 = config)")
+    ap.add_argument("--batch", type=int, default=32, help="problems of the uneven-batch leg (0 = skip)")
+    ap.add_argument("--batch-nmin", type=int, default=500)
+    ap.add_argument("--batch-nmax", type=int, default=5000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -205,9 +251,22 @@ def main():
     obj.synchronize()
     el32 = time.perf_counter() - t0
     f32, g32 = d_f.get()[0], d_grd.get()
+    # kernel breakdown of the FP32 policy (separate, event-timed pass)
+    lddmm_amd.profile_reset()
+    lddmm_amd.profile_enable(True)
+    for _ in range(max(2, args.steps // 4)):
+        obj.eval_device(d_f.ptr, d_grd.ptr, d_alp.ptr)
+    obj.synchronize()
+    lddmm_amd.profile_enable(False)
+    p32 = lddmm_amd.profile_read()
+    k32 = {}
+    for k, (np_, _) in per_launch.items():
+        ms, cnt = p32.get(k, (0.0, 0))
+        if cnt:
+            k32[k] = dict(avg_ms=ms / cnt, pairs_per_s=np_ / (ms / cnt * 1e-3))
     out["fp32_policy"] = dict(value=pairs * args.steps / el32, ms_per_step=1e3 * el32 / args.steps,
                               speedup_vs_fp64=elapsed / el32, energy_rel_err=abs(f32 - f64) / abs(f64),
-                              grad_rel_err=float(np.max(np.abs(g32 - g64)) / np.max(np.abs(g64))))
+                              grad_rel_err=float(np.max(np.abs(g32 - g64)) / np.max(np.abs(g64))), kernels=k32)
     obj.set_precision("fp64")
 
     if not args.no_registration:
@@ -233,6 +292,9 @@ def main():
                             final_energy_fp64_eval=e32, energy_rel_to_fp64_run=(e32 - e_ref) / abs(e_ref))
         regs["fp32_speedup"] = i64["seconds"] / i32["seconds"]
         out["registration"] = regs
+
+    if args.batch > 0:
+        out["batch"] = batch_bench(args.batch, args.batch_nmin, args.batch_nmax, 300)
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, ifo, pairs)
