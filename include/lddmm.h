@@ -208,6 +208,43 @@ int lddmm_batch_run(lddmm_batch *b, const double *iniAlpStks, double *optAlpStks
 int lddmm_batch_info(lddmm_batch *b, int *chunk, double *pairsPerEval);
 void lddmm_batch_destroy(lddmm_batch *b);
 
+/* ---------------------------------------------------------------- learned warm start
+ * (no counterpart in the reference).  A per-vertex multilayer perceptron
+ * maps 32 point features of the source to a momentum vector used as the
+ * initial L-BFGS iterate.  Trained on the GPU with BF16 MFMA GEMMs (FP32
+ * accumulation, FP32 master weights, Adam). */
+#define LDDMM_FEATURES 32
+/* featMat (N x 32 row-major, FP32) of a 3D triangulated source (host
+ * layout ndeMat D x N, elmVtxMat 3 x E 0-based) against a target varifold
+ * (host layout) for a deformation kernel of width knlWidth. */
+int lddmm_point_features(float *featMat, int dimNum, const double *ndeMat, int ndeNum, const int32_t *elmVtxMat,
+                         int elmNum, const double *tgtCenPosMat, const double *tgtUniDirMat,
+                         const double *tgtElmVolVec, int tgtElmNum, double knlWidth);
+typedef struct lddmm_mlp lddmm_mlp;
+/* layers >= 2 dense layers: 32 -> hidden -> ... -> hidden -> 3 (hidden a
+ * multiple of 32), ReLU between layers, He-uniform init from seed. */
+int lddmm_mlp_create(lddmm_mlp **out, int hidden, int layers, uint64_t seed);
+int lddmm_mlp_param_count(lddmm_mlp *m, int *count);
+/* parameters in the order W0 (32 x hidden row-major), b0, W1, b1, ...; the
+ * last layer is stored with 32 output columns of which the first 3 are used */
+int lddmm_mlp_get_params(lddmm_mlp *m, float *params);
+int lddmm_mlp_set_params(lddmm_mlp *m, const float *params);   /* also resets Adam */
+/* outMat (rows x 3 row-major) from featMat (rows x 32); host arrays */
+int lddmm_mlp_forward(lddmm_mlp *m, const float *featMat, int rows, float *outMat);
+/* steps Adam steps (lr) on the mean squared error to tgtMat (rows x 3),
+ * minibatches of batchRows consecutive rows cycling through the data (the
+ * caller shuffles); losses[s] (may be NULL) = loss before step s.  With
+ * steps == 1, lr == 0 and batchRows >= rows, grad (may be NULL) receives the
+ * full-batch gradient in the parameter order and nothing changes. */
+int lddmm_mlp_train(lddmm_mlp *m, const float *featMat, const float *tgtMat, int rows, int steps, int batchRows,
+                    float lr, double *losses, float *grad);
+void lddmm_mlp_destroy(lddmm_mlp *m);
+/* C (M x N row-major FP32) = op(A) op(B), A and B BF16 bit patterns
+ * (row-major, transposed when transA / transB), on the MFMA GEMM kernel the
+ * MLP uses; M, N multiples of 32, K of 16.  Host arrays. */
+int lddmm_gemm_bf16(float *C, const uint16_t *A, const uint16_t *B, int M, int N, int K, int transA, int transB,
+                    int splitK);
+
 /* ---------------------------------------------------------------- data formats
  * getElmIfo (Cpp/src/getElmIfo.m:1-59) from a 0-based V x E table; writes
  * the (1 + maxAdj*V) x N table when elmIfoMat != NULL and returns the row

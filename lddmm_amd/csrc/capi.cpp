@@ -15,6 +15,7 @@
 #include "../../include/lddmm.h"
 #include "batch.hpp"
 #include "engine.hpp"
+#include "mlp.hpp"
 
 using namespace lddmm;
 
@@ -570,6 +571,126 @@ int lddmm_batch_info(lddmm_batch *b, int *chunk, double *pairsPerEval)
 }
 
 void lddmm_batch_destroy(lddmm_batch *b) { delete b; }
+
+// ---------------------------------------------------------------- learned warm start
+int lddmm_point_features(float *featMat, int dimNum, const double *ndeMat, int ndeNum, const int32_t *elmVtxMat,
+                         int elmNum, const double *tgtCenPosMat, const double *tgtUniDirMat,
+                         const double *tgtElmVolVec, int tgtElmNum, double knlWidth)
+{
+    return guarded([&] {
+        require(featMat && ndeMat && elmVtxMat, "null argument");
+        require(tgtElmNum >= 0 && (tgtElmNum == 0 || (tgtCenPosMat && tgtUniDirMat && tgtElmVolVec)),
+                "null target array");
+        Stream st;
+        point_features(dimNum, ndeMat, ndeNum, elmVtxMat, elmNum, tgtCenPosMat, tgtUniDirMat, tgtElmVolVec,
+                       tgtElmNum, knlWidth, featMat, st.s);
+        return 0;
+    });
+}
+
+struct lddmm_mlp {
+    Stream st;
+    std::unique_ptr<Mlp> m;
+};
+
+int lddmm_mlp_create(lddmm_mlp **out, int hidden, int layers, uint64_t seed)
+{
+    return guarded([&] {
+        require(out != nullptr, "null handle pointer");
+        *out = nullptr;
+        auto h = std::make_unique<lddmm_mlp>();
+        h->m = std::make_unique<Mlp>(hidden, layers, (unsigned long long)seed, h->st.s);
+        *out = h.release();
+        return 0;
+    });
+}
+
+int lddmm_mlp_param_count(lddmm_mlp *m, int *count)
+{
+    return guarded([&] {
+        require(m && count, "null argument");
+        *count = m->m->param_count();
+        return 0;
+    });
+}
+
+int lddmm_mlp_get_params(lddmm_mlp *m, float *params)
+{
+    return guarded([&] {
+        require(m && params, "null argument");
+        m->m->get_params(params);
+        return 0;
+    });
+}
+
+int lddmm_mlp_set_params(lddmm_mlp *m, const float *params)
+{
+    return guarded([&] {
+        require(m && params, "null argument");
+        m->m->set_params(params);
+        return 0;
+    });
+}
+
+int lddmm_mlp_forward(lddmm_mlp *m, const float *featMat, int rows, float *outMat)
+{
+    return guarded([&] {
+        require(m && featMat && outMat, "null argument");
+        require(rows > 0, "rows must be positive");
+        DevBuf<float> x((size_t)rows * kFeat), y((size_t)rows * 3);
+        x.upload(featMat, m->st.s);
+        m->m->forward(x.p, rows, y.p);
+        y.download(outMat, m->st.s);
+        m->st.sync();
+        return 0;
+    });
+}
+
+int lddmm_mlp_train(lddmm_mlp *m, const float *featMat, const float *tgtMat, int rows, int steps, int batchRows,
+                    float lr, double *losses, float *grad)
+{
+    return guarded([&] {
+        require(m && featMat && tgtMat, "null argument");
+        require(rows > 0 && steps >= 0 && batchRows > 0, "rows and batchRows must be positive");
+        require(!grad || (steps == 1 && batchRows >= rows), "grad needs one full-batch step");
+        DevBuf<float> x((size_t)rows * kFeat), y((size_t)rows * 3);
+        x.upload(featMat, m->st.s);
+        y.upload(tgtMat, m->st.s);
+        int start = 0;
+        for (int s = 0; s < steps; ++s) {
+            const int cnt = std::min(batchRows, rows - start);
+            const double l = m->m->train_step(x.p + (size_t)start * kFeat, y.p + (size_t)start * 3, cnt, lr, grad);
+            if (losses) losses[s] = l;
+            start += cnt;
+            if (start >= rows) start = 0;
+        }
+        m->st.sync();
+        return 0;
+    });
+}
+
+void lddmm_mlp_destroy(lddmm_mlp *m) { delete m; }
+
+int lddmm_gemm_bf16(float *C, const uint16_t *A, const uint16_t *B, int M, int N, int K, int transA, int transB,
+                    int splitK)
+{
+    return guarded([&] {
+        require(C && A && B, "null argument");
+        require(M > 0 && N > 0 && K > 0 && M % 32 == 0 && N % 32 == 0 && K % 16 == 0,
+                "M, N must be positive multiples of 32 and K of 16");
+        require(splitK >= 1, "splitK must be at least 1");
+        Stream st;
+        DevBuf<uint16_t> a((size_t)M * K), b((size_t)K * N);
+        DevBuf<float> c((size_t)M * N), part((size_t)M * N * splitK);
+        a.upload(A, st.s);
+        b.upload(B, st.s);
+        gemm_bf16(transA != 0, transB != 0, (const __bf16 *)a.p, transA ? M : K, (const __bf16 *)b.p,
+                  transB ? K : N, c.p, N, M, N, K, splitK, part.p, st.s);
+        c.download(C, st.s);
+        st.sync();
+        return 0;
+    });
+}
 
 // ---------------------------------------------------------------- operators
 int lddmm_multiply_kernel_cross(double *vlcMat, const double *lmkiMat, const double *lmkjMat, const double *alpMat,

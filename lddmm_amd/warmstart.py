@@ -1,0 +1,254 @@
+"""Learned warm start for batched registrations (no counterpart in the
+reference, which always starts L-BFGS from zero momentum: Cpp/src/LDDMM.m
+passes iniAlpStk, normally zeros).
+
+A per-vertex multilayer perceptron (32 point features -> 3, hidden layers
+of 128, ReLU) is trained on the GPU with the library's BF16 MFMA kernels
+(lddmm_mlp_*, FP32 master weights, Adam) to predict the endpoint
+displacement q_T - q_0 of each source vertex in solved registrations.  A
+few conjugate-gradient steps on the deformation kernel turn the predicted
+displacement into a momentum field, used as every time slice of the
+initial momentum stack; `choose_start` scales it per problem by the
+multiple with the lowest objective, zero included, so the plain start
+remains the fallback.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _check, _dp, _f64, _i32, _ip, _p, computeNodePosition, lib, multiplyKernel
+
+__all__ = ["point_features", "Mlp", "gemm_bf16", "WarmStart", "to_bf16_bits", "from_bf16_bits"]
+
+_fp = C.POINTER(C.c_float)
+_u16p = C.POINTER(C.c_uint16)
+NFEAT = 32
+
+
+def _setup():
+    L = lib()
+    if not hasattr(L.lddmm_mlp_create, "_set"):
+        L.lddmm_point_features.argtypes = [_fp, C.c_int, _dp, C.c_int, _ip, C.c_int, _dp, _dp, _dp, C.c_int,
+                                           C.c_double]
+        L.lddmm_mlp_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_uint64]
+        L.lddmm_mlp_param_count.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.lddmm_mlp_get_params.argtypes = [C.c_void_p, _fp]
+        L.lddmm_mlp_set_params.argtypes = [C.c_void_p, _fp]
+        L.lddmm_mlp_forward.argtypes = [C.c_void_p, _fp, C.c_int, _fp]
+        L.lddmm_mlp_train.argtypes = [C.c_void_p, _fp, _fp, C.c_int, C.c_int, C.c_int, C.c_float, _dp, _fp]
+        L.lddmm_mlp_destroy.argtypes = [C.c_void_p]
+        L.lddmm_mlp_destroy.restype = None
+        L.lddmm_gemm_bf16.argtypes = [_fp, _u16p, _u16p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.lddmm_mlp_create._set = True
+    return L
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _pf(a):
+    return None if a is None else a.ctypes.data_as(_fp)
+
+
+def to_bf16_bits(x):
+    """float32 -> bfloat16 bit patterns, round to nearest even."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def from_bf16_bits(b):
+    return (np.asarray(b, dtype=np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+def gemm_bf16(A, B, trans_a=False, trans_b=False, split_k=1):
+    """C = op(A) op(B) on the library's MFMA kernel; A, B are bf16 bit
+    arrays (uint16), C float32."""
+    L = _setup()
+    A = np.ascontiguousarray(A, dtype=np.uint16)
+    B = np.ascontiguousarray(B, dtype=np.uint16)
+    M, K = (A.shape[1], A.shape[0]) if trans_a else A.shape
+    N = B.shape[0] if trans_b else B.shape[1]
+    Cm = np.empty((M, N), dtype=np.float32)
+    _check(L.lddmm_gemm_bf16(_pf(Cm), A.ctypes.data_as(_u16p), B.ctypes.data_as(_u16p), M, N, K, int(trans_a),
+                             int(trans_b), int(split_k)))
+    return Cm
+
+
+def point_features(q0, elm, tgt_cen, tgt_dir, tgt_vol, knl_width):
+    """(N, 32) float32 features of every source vertex (lddmm_point_features)."""
+    L = _setup()
+    q = _f64(q0)
+    e = _i32(elm)
+    tc, tu, ta = _f64(tgt_cen), _f64(tgt_dir), _f64(tgt_vol)
+    out = np.empty((len(q), NFEAT), dtype=np.float32)
+    _check(L.lddmm_point_features(_pf(out), q.shape[1], _p(q), len(q), _p(e), len(e), _p(tc), _p(tu), _p(ta),
+                                  len(ta), float(knl_width)))
+    return out
+
+
+class Mlp:
+    """The GPU multilayer perceptron (lddmm_mlp_*)."""
+
+    def __init__(self, hidden=128, layers=4, seed=1):
+        L = _setup()
+        self.hidden, self.layers = hidden, layers
+        h = C.c_void_p()
+        _check(L.lddmm_mlp_create(C.byref(h), hidden, layers, seed))
+        self._h = h
+        n = C.c_int()
+        _check(L.lddmm_mlp_param_count(self._h, C.byref(n)))
+        self.nparam = n.value
+
+    def shapes(self):
+        """[(din, dout)] per layer, the last one padded to 32 outputs."""
+        dims = [NFEAT] + [self.hidden] * (self.layers - 1) + [32]
+        return list(zip(dims[:-1], dims[1:]))
+
+    def unpack(self, flat):
+        out, o = [], 0
+        for din, dout in self.shapes():
+            W = flat[o:o + din * dout].reshape(din, dout)
+            o += din * dout
+            b = flat[o:o + dout]
+            o += dout
+            out.append((W, b))
+        return out
+
+    def get_params(self):
+        out = np.empty(self.nparam, dtype=np.float32)
+        _check(lib().lddmm_mlp_get_params(self._h, _pf(out)))
+        return out
+
+    def set_params(self, flat):
+        flat = _f32(flat)
+        assert flat.size == self.nparam
+        _check(lib().lddmm_mlp_set_params(self._h, _pf(flat)))
+
+    def forward(self, feat):
+        feat = _f32(feat)
+        out = np.empty((len(feat), 3), dtype=np.float32)
+        _check(lib().lddmm_mlp_forward(self._h, _pf(feat), len(feat), _pf(out)))
+        return out
+
+    def train(self, feat, tgt, steps, batch_rows, lr):
+        feat, tgt = _f32(feat), _f32(tgt)
+        losses = np.empty(max(steps, 1))
+        _check(lib().lddmm_mlp_train(self._h, _pf(feat), _pf(tgt), len(feat), steps, batch_rows, lr,
+                                     _p(losses), None))
+        return losses[:steps]
+
+    def gradient(self, feat, tgt):
+        """(loss, gradient) of the full batch, parameters unchanged."""
+        feat, tgt = _f32(feat), _f32(tgt)
+        loss = np.empty(1)
+        g = np.empty(self.nparam, dtype=np.float32)
+        _check(lib().lddmm_mlp_train(self._h, _pf(feat), _pf(tgt), len(feat), 1, len(feat), 0.0, _p(loss), _pf(g)))
+        return loss[0], g
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lddmm_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def kernel_solve(q0, v, knl_order, knl_width, iters=15, reg=1e-4):
+    """Approximate momentum a with (K + reg I) a = v by `iters` conjugate
+    gradient steps per component (K applied by the GPU multiplyKernel).
+    Few iterations keep a smooth: the ill-conditioned directions of K are
+    left out instead of amplified."""
+    q = _f64(q0)
+    v = _f64(v)
+    a = np.zeros_like(v)
+    r = v.copy()
+    p = r.copy()
+    rr = np.sum(r * r, axis=0)
+    for _ in range(iters):
+        Kp = multiplyKernel(q, p, knl_order, knl_width) + reg * p
+        den = np.sum(p * Kp, axis=0)
+        step = np.where(den > 0, rr / np.where(den > 0, den, 1.0), 0.0)
+        a += step * p
+        r -= step * Kp
+        rr_new = np.sum(r * r, axis=0)
+        p = r + np.where(rr > 0, rr_new / np.where(rr > 0, rr, 1.0), 0.0) * p
+        rr = rr_new
+    return a
+
+
+class WarmStart:
+    """Feature standardisation + target scale + the GPU network.  The
+    network predicts the endpoint displacement q_T - q_0 of each source
+    vertex; the warm-start momentum is the constant-in-time field whose
+    first-order flow K a (T - 1) h gives that displacement."""
+
+    def __init__(self, hidden=128, layers=4, seed=1):
+        self.net = Mlp(hidden, layers, seed)
+        self.fmean = np.zeros(NFEAT, dtype=np.float32)
+        self.fstd = np.ones(NFEAT, dtype=np.float32)
+        self.yscale = 1.0
+
+    @staticmethod
+    def features(p):
+        return point_features(p["q0"], p["elm"], p["tgt_cen"], p["tgt_dir"], p["tgt_vol"], p["knl_width"])
+
+    def fit(self, problems, solutions, knl_order, time_stp, time_num, steps=3000, batch_rows=4096, lr=2e-3,
+            seed=0):
+        """problems: batch problem dicts; solutions: their optimal momentum
+        stacks (T-1, N, 3).  Returns the per-step training losses."""
+        feats, ys = [], []
+        for p, a in zip(problems, solutions):
+            nde, _ = computeNodePosition(p["q0"], a, knl_order, p["knl_width"], time_stp, time_num)
+            feats.append(self.features(p))
+            ys.append(nde[-1] - p["q0"])
+        X = np.concatenate(feats)
+        Y = np.concatenate(ys)
+        self.fmean = X.mean(axis=0).astype(np.float32)
+        self.fstd = np.maximum(X.std(axis=0), 1e-6).astype(np.float32)
+        self.yscale = float(np.sqrt(np.mean(Y ** 2))) or 1.0
+        perm = np.random.default_rng(seed).permutation(len(X))
+        Xn = ((X - self.fmean) / self.fstd)[perm]
+        Yn = (Y / self.yscale)[perm]
+        return self.net.train(Xn, Yn, steps, min(batch_rows, len(Xn)), lr)
+
+    def displacement(self, problem):
+        f = self.features(problem)
+        return self.net.forward((f - self.fmean) / self.fstd).astype(np.float64) * self.yscale
+
+    def predict(self, problem, knl_order, time_stp, time_num, cg_iters=15):
+        disp = self.displacement(problem)
+        v = disp / (time_stp * (time_num - 1))
+        a = kernel_solve(problem["q0"], v, knl_order, problem["knl_width"], cg_iters)
+        return np.repeat(a[None], time_num - 1, axis=0)
+
+    def save(self, path):
+        np.savez(path, params=self.net.get_params(), fmean=self.fmean, fstd=self.fstd, yscale=self.yscale,
+                 hidden=self.net.hidden, layers=self.net.layers)
+
+    @classmethod
+    def load(cls, path):
+        z = np.load(path)
+        ws = cls(int(z["hidden"]), int(z["layers"]))
+        ws.net.set_params(z["params"])
+        ws.fmean, ws.fstd, ws.yscale = z["fmean"], z["fstd"], float(z["yscale"])
+        return ws
+
+
+def choose_start(batch, warm, scales=(0.5, 0.75, 1.0, 1.25)):
+    """Per problem, the multiple c * warm (c in scales, or c = 0: zero
+    momentum) with the lowest objective, from one batched evaluation per
+    scale.  Returns (starts, chosen scale per problem)."""
+    f_best, _ = batch.objgrd([np.zeros_like(a) for a in warm])
+    c_best = np.zeros(len(warm))
+    for c in scales:
+        f, _ = batch.objgrd([c * a for a in warm])
+        better = f < f_best
+        f_best = np.where(better, f, f_best)
+        c_best = np.where(better, c, c_best)
+    return [c * a for c, a in zip(c_best, warm)], c_best
