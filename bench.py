@@ -103,14 +103,25 @@ def cpu_baseline(w, ifo, pairs_per_eval):
                 seconds_per_eval=t_eval)
 
 
-def batch_bench(count, nmin, nmax, itr, seed=4):
+def batch_bench(count, nmin, nmax, itr, seed=4, train_count=64):
     """Uneven batch (the batched-registration workload, scaled by `count`
     and the size range): plain FP64 registrations one after another
     (zero initial momentum, stop at a relative energy decrease below 1e-6 or
-    `itr` iterations) against the batched FP64 run and the batched FP32
-    policy run to each plain run's final energy within 1e-6 relative."""
+    `itr` iterations) against the batched FP64 run, the batched FP32
+    policy run to each plain run's final energy within 1e-6 relative, and
+    the batched FP32 policy started from the learned warm start.
+
+    Warm-start protocol: the network is trained on `train_count` problems
+    of the same generator (other seeds) solved by the batched FP32 run; that
+    training is reported as a one-off cost.  The timed warm-start leg
+    includes feature extraction, inference, the per-problem start scale
+    selection, the registration to the plain energies, and the fallback: a
+    zero-momentum batched FP32 run for every problem the warm-started run
+    did not bring within 1e-6 of the plain energy."""
+    from lddmm_amd import warmstart as W
     probs = G.batch_problems(count, seed, nmin, nmax)
     k = G.BATCH_KERNELS
+    T = k["time_num"]
     seq = []
     t0 = time.perf_counter()
     for p in probs:
@@ -122,16 +133,55 @@ def batch_bench(count, nmin, nmax, itr, seed=4):
         seq.append(info)
     t_seq = time.perf_counter() - t0
     e_ref = np.array([i["fcn"] for i in seq])
-    B = lddmm_amd.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
-                        k["knl_order"], k["time_stp"], k["time_num"], precision="fp32", chunk=0)
+    targets = e_ref * (1.0 + 1e-6)
+
+    def make(ps, prec, chunk=0):
+        return lddmm_amd.Batch(ps, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
+                               k["knl_order"], k["time_stp"], T, precision=prec, chunk=chunk)
+
+    B = make(probs, "fp32")
     chunk, pairs = B.info()
     a64, i64, r64 = B.run(itr, 0.0, 0.0, 0.5, 5, precision="fp64", rel_energy_tol=1e-6)
-    a32, i32, r32 = B.run(3 * itr, 0.0, 0.0, 0.5, 5, precision="fp32", targets=e_ref * (1.0 + 1e-6))
-    B64 = lddmm_amd.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
-                          k["knl_order"], k["time_stp"], k["time_num"], precision="fp64", chunk=chunk)
-    f32_in_fp64, _ = B64.objgrd(a32)
-    rel = (f32_in_fp64 - e_ref) / np.abs(e_ref)
+    a32, i32, r32 = B.run(3 * itr, 0.0, 0.0, 0.5, 5, precision="fp32", targets=targets)
+    B64 = make(probs, "fp64", chunk)
+
+    def rel_to_plain(alps):
+        f, _ = B64.objgrd(alps)
+        return (f - e_ref) / np.abs(e_ref)
+
+    rel = rel_to_plain(a32)
     t64, t32 = i64[0]["seconds"], i32[0]["seconds"]
+
+    # learned warm start: one-off training on other seeds of the generator
+    t0 = time.perf_counter()
+    train = G.batch_problems(train_count, seed + 1000, nmin, nmax)
+    Bt = make(train, "fp32")
+    sol, _, _ = Bt.run(itr, 0.0, 0.0, 0.5, 5, precision="fp32", rel_energy_tol=1e-6)
+    Bt.close()
+    t_solve = time.perf_counter() - t0
+    ws = W.WarmStart(hidden=128, layers=4, seed=1)
+    t0 = time.perf_counter()
+    losses = ws.fit(train, sol, k["knl_order"], k["time_stp"], T, steps=3000, batch_rows=4096, lr=1e-3)
+    t_fit = time.perf_counter() - t0
+    # timed: inference, start selection, registration, fallback
+    t0 = time.perf_counter()
+    warm = [ws.predict(p, k["knl_order"], k["time_stp"], T) for p in probs]
+    starts, scale = W.choose_start(B, warm)
+    aw, iw, rw = B.run(3 * itr, 0.0, 0.0, 0.5, 5, alp0=starts, precision="fp32", targets=targets)
+    missed = [i for i, r in enumerate(iw) if not r["reached_target"]]
+    its_fb = []
+    if missed:
+        Bf = make([probs[i] for i in missed], "fp32", chunk)
+        af, ifb, _ = Bf.run(3 * itr, 0.0, 0.0, 0.5, 5, precision="fp32", targets=targets[missed])
+        Bf.close()
+        for j, i in enumerate(missed):
+            aw[i] = af[j]
+        its_fb = [r["iterations"] for r in ifb]
+    t_warm = time.perf_counter() - t0
+    rel_w = rel_to_plain(aw)
+    its_w = [r["iterations"] for r in iw]
+    B.close()
+    B64.close()
     return dict(problems=count, n_range=[int(min(B.n)), int(max(B.n))], chunk=chunk,
                 pairs_per_eval_total=float(pairs.sum()),
                 plain_fp64_sequential=dict(seconds=t_seq, registrations_per_s=count / t_seq,
@@ -143,7 +193,17 @@ def batch_bench(count, nmin, nmax, itr, seed=4):
                                   reached_target=int(sum(i["reached_target"] for i in i32)),
                                   within_1e6=int(np.sum(rel <= 1e-6 + 1e-8)),
                                   max_rel_energy_vs_plain=float(rel.max())),
-                speedup_batched_fp32_vs_plain_fp64=t_seq / t32)
+                batched_fp32_warm_start=dict(
+                    seconds=t_warm, registrations_per_s=count / t_warm, rounds=rw,
+                    median_iterations=float(np.median(its_w)),
+                    median_iterations_plain=float(np.median([i["iterations"] for i in seq])),
+                    warm_start_used=int(np.sum(scale > 0)), reached_target=count - len(missed),
+                    fallback_problems=len(missed), fallback_iterations=its_fb,
+                    within_1e6=int(np.sum(rel_w <= 1e-6 + 1e-8)), max_rel_energy_vs_plain=float(rel_w.max()),
+                    training=dict(problems=train_count, solve_seconds=t_solve, fit_seconds=t_fit, adam_steps=3000,
+                                  loss_first=float(losses[0]), loss_last=float(np.mean(losses[-20:])))),
+                speedup_batched_fp32_vs_plain_fp64=t_seq / t32,
+                speedup_batched_fp32_warm_vs_plain_fp64=t_seq / t_warm)
 
 
 def main():
@@ -158,11 +218,25 @@ def main():
     ap.add_argument("--batch-nmin", type=int, default=500)
     ap.add_argument("--batch-nmax", type=int, default=5000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-profile", action="store_true",
+                    help="only the batched FP32 registration of the uneven batch (for kernel traces)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (problems would shard by "
               "target rows or by independent registrations; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.batch_profile:
+        probs = G.batch_problems(args.batch, 4, args.batch_nmin, args.batch_nmax)
+        k = G.BATCH_KERNELS
+        B = lddmm_amd.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
+                            k["knl_order"], k["time_stp"], k["time_num"], precision="fp32", chunk=0)
+        chunk, pairs = B.info()
+        _, info, rounds = B.run(300, 0.0, 0.0, 0.5, 5, precision="fp32", rel_energy_tol=1e-6)
+        print(json.dumps(dict(problems=args.batch, chunk=chunk, pairs_per_eval_total=float(pairs.sum()),
+                              seconds=info[0]["seconds"], rounds=rounds,
+                              evaluations=int(sum(i["evaluations"] for i in info)))), flush=True)
+        return
 
     w, extra = workload(args.config)
     obj, ifo = make_objective(w)

@@ -83,10 +83,10 @@ __global__ __launch_bounds__(kBlock) void kb_kv(BatchView v, int t, const int4 *
     pairs_block<Op>(A, d.n, it.y, j0, j1, v.part + d.po + (size_t)it.z * D * d.n, smem);
 }
 
-template <int D, int ORDER, class P>
+template <int D, int ORDER, class P, int R>
 __global__ __launch_bounds__(kBlock) void kb_bwd(BatchView v, int t, const int4 *__restrict__ items)
 {
-    using Op = OpBwd<D, ORDER, P>;
+    using Op = OpBwd<D, ORDER, P, R>;
     using real = typename Op::real;
     __shared__ __align__(16) char smem[PairSmem<Op>::bytes];
     const int4 it = items[xcd_remap(blockIdx.x, gridDim.x)];
@@ -739,10 +739,12 @@ void Batch::build_tables(const std::vector<int> &prec_of)
         const int g = pr == kFP32 ? 1 : 0;
         const BatchDesc &d = desc_h_[p];
         probs.push_back(p);
-        const int tkv = g ? 128 : 64;   // FP32 K.a keeps two targets per lane
+        const int tkv = g ? 128 : 64;   // FP32 pair kernels keep two targets per lane
         for (int s = 0; s < d.splits; ++s) {
-            for (int t = 0; t < ceil_div(d.n, tkv); ++t) kv[g].push_back(make_int4(p, t, s, 0));
-            for (int t = 0; t < ceil_div(d.n, kTile); ++t) bwd[g].push_back(make_int4(p, t, s, 0));
+            for (int t = 0; t < ceil_div(d.n, tkv); ++t) {
+                kv[g].push_back(make_int4(p, t, s, 0));
+                bwd[g].push_back(make_int4(p, t, s, 0));
+            }
         }
         for (int s = 0; s < d.vsd + d.vst; ++s)
             for (int t = 0; t < ceil_div(d.ne, kTile); ++t) vf[g].push_back(make_int4(p, t, s, 0));
@@ -883,7 +885,8 @@ void Batch::eval(const double *alp, double *grd, double *f_dev, const std::vecto
             each_prec([&](int g, auto Pc) {
                 if (!tb.nbwd[g]) return;
                 dispatch([&](auto Dc, auto Oc) {
-                    kb_bwd<Dc(), Oc(), decltype(Pc)><<<tb.nbwd[g], kBlock, 0, st_>>>(v, t, tb.bwd[g].p);
+                    constexpr int R = decltype(Pc)::fp32 ? 2 : 1;
+                    kb_bwd<Dc(), Oc(), decltype(Pc), R><<<tb.nbwd[g], kBlock, 0, st_>>>(v, t, tb.bwd[g].p);
                 });
                 LDDMM_LAUNCH_CHECK();
             });

@@ -2,11 +2,14 @@
 //
 // A block of 256 threads (4 wave64) owns TI = 64*R targets (lane l of
 // every wave holds targets base + l + 64 r, r < R) and one chunk of the
-// sources.  The chunk is staged through LDS 256 sources at a time: source
-// coordinates as FP64, the other per-source fields in the policy's storage
-// type.  Wave w consumes sources t = w, w+4, ... of each tile, so all 64
-// lanes of a wave read the same LDS words (broadcast, conflict-free) and
-// every source is read once per wave for R targets.  Per tile, sums are
+// sources.  The chunk is staged through LDS 256 sources at a time, one
+// 16-byte aligned record per source: the per-source fields in the policy's
+// storage type, then the coordinates as FP64, so one base address and
+// immediate offsets reach every field (wide ds_read_b128/b64 loads, no
+// per-field address arithmetic in the pair loop).  Wave w consumes sources
+// t = w, w+4, ... of each tile, so all 64 lanes of a wave read the same LDS
+// words (broadcast, conflict-free) and every source is read once per wave
+// for R targets.  Per tile, sums are
 // carried in the policy's arithmetic type and flushed to FP64 totals
 // (two-level accumulation: at most 64 pairs are summed in FP32 before the
 // FP64 add); at the end the four waves' totals are combined in LDS in a
@@ -58,49 +61,56 @@ struct KernelFnT<ORDER, double> {
     }
 };
 
-// FP32: exp via v_exp_f32 on a pre-scaled base-2 argument, sqrt via
-// v_sqrt_f32 (both ~1 ulp).
+// FP32: exp via v_exp_f32 on a base-2 argument, sqrt via v_sqrt_f32 (both
+// ~1 ulp).  Every width-dependent factor is written as a product of
+// per-launch constants (-iw2 log2(e), -2 iw2, the Matern polynomial
+// coefficients divided by their leading constant) so the compiler hoists
+// them out of the pair loop: the Gaussian costs one multiply and one
+// v_exp_f32 per pair for k, one more multiply for c.
 template <int ORDER>
 struct KernelFnT<ORDER, float> {
     static constexpr float kLog2e = 1.4426950408889634f;
-    __device__ __forceinline__ static float ex(float x) { return __builtin_amdgcn_exp2f(x * kLog2e); }
     __device__ __forceinline__ static float value(float r2, float iw, float iw2)
     {
         if constexpr (ORDER == -1) {
-            return ex(-r2 * iw2);
+            return __builtin_amdgcn_exp2f(r2 * (iw2 * -kLog2e));
         } else {
-            float d = __builtin_amdgcn_sqrtf(r2) * iw;
-            float e = ex(-d);
+            const float s = __builtin_amdgcn_sqrtf(r2);
+            const float d = s * iw;
+            const float e = __builtin_amdgcn_exp2f(s * (iw * -kLog2e));
             if constexpr (ORDER == 0) return e;
             else if constexpr (ORDER == 1) return (1.0f + d) * e;
-            else if constexpr (ORDER == 2) return (3.0f + d * (3.0f + d)) * (1.0f / 3.0f) * e;
-            else if constexpr (ORDER == 3) return (15.0f + d * (15.0f + d * (6.0f + d))) * (1.0f / 15.0f) * e;
-            else return (105.0f + d * (105.0f + d * (45.0f + d * (10.0f + d)))) * (1.0f / 105.0f) * e;
+            else if constexpr (ORDER == 2) return (1.0f + d * (1.0f + d * (1.0f / 3.0f))) * e;
+            else if constexpr (ORDER == 3)
+                return (1.0f + d * (1.0f + d * (6.0f / 15.0f + d * (1.0f / 15.0f)))) * e;
+            else
+                return (1.0f + d * (1.0f + d * (45.0f / 105.0f + d * (10.0f / 105.0f + d * (1.0f / 105.0f))))) * e;
         }
     }
     __device__ __forceinline__ static void eval(float r2, float iw, float iw2, float &k, float &c)
     {
         if constexpr (ORDER == -1) {
-            k = ex(-r2 * iw2);
-            c = -2.0f * k * iw2;
+            k = __builtin_amdgcn_exp2f(r2 * (iw2 * -kLog2e));
+            c = k * (-2.0f * iw2);
         } else {
-            float d = __builtin_amdgcn_sqrtf(r2) * iw;
-            float e = ex(-d);
+            const float s = __builtin_amdgcn_sqrtf(r2);
+            const float d = s * iw;
+            const float e = __builtin_amdgcn_exp2f(s * (iw * -kLog2e));
             if constexpr (ORDER == 0) {
                 k = e;
                 c = 0.0f;
             } else if constexpr (ORDER == 1) {
                 k = (1.0f + d) * e;
-                c = -e * iw2;
+                c = e * -iw2;
             } else if constexpr (ORDER == 2) {
-                k = (3.0f + d * (3.0f + d)) * (1.0f / 3.0f) * e;
-                c = -(1.0f + d) * e * (iw2 * (1.0f / 3.0f));
+                k = (1.0f + d * (1.0f + d * (1.0f / 3.0f))) * e;
+                c = (1.0f + d) * e * (iw2 * (-1.0f / 3.0f));
             } else if constexpr (ORDER == 3) {
-                k = (15.0f + d * (15.0f + d * (6.0f + d))) * (1.0f / 15.0f) * e;
-                c = -(3.0f + d * (3.0f + d)) * e * (iw2 * (1.0f / 15.0f));
+                k = (1.0f + d * (1.0f + d * (6.0f / 15.0f + d * (1.0f / 15.0f)))) * e;
+                c = (1.0f + d * (1.0f + d * (1.0f / 3.0f))) * e * (iw2 * (-3.0f / 15.0f));
             } else {
-                k = (105.0f + d * (105.0f + d * (45.0f + d * (10.0f + d)))) * (1.0f / 105.0f) * e;
-                c = -(15.0f + d * (15.0f + d * (6.0f + d))) * e * (iw2 * (1.0f / 105.0f));
+                k = (1.0f + d * (1.0f + d * (45.0f / 105.0f + d * (10.0f / 105.0f + d * (1.0f / 105.0f))))) * e;
+                c = (1.0f + d * (1.0f + d * (6.0f / 15.0f + d * (1.0f / 15.0f)))) * e * (iw2 * (-15.0f / 105.0f));
             }
         }
     }
@@ -112,8 +122,11 @@ constexpr int kWaves = kBlock / 64;
 // share one buffer.
 template <class Op>
 struct PairSmem {
-    static constexpr size_t stage =
-        sizeof(double) * Op::D * kTileJ + sizeof(typename Op::lds_t) * (Op::F > 0 ? Op::F : 1) * kTileJ;
+    // record of one source: fields at 0, coordinates at `xoff`
+    static constexpr size_t fbytes = sizeof(typename Op::lds_t) * (Op::F > 0 ? Op::F : 1);
+    static constexpr size_t xoff = (fbytes + 7) & ~size_t(7);
+    static constexpr size_t rec = (xoff + sizeof(double) * Op::D + 15) & ~size_t(15);
+    static constexpr size_t stage = rec * kTileJ;
     static constexpr size_t red = sizeof(double) * kWaves * Op::OUT * 64 * Op::R;
     static constexpr size_t bytes = stage > red ? stage : red;
 };
@@ -126,8 +139,7 @@ __device__ __forceinline__ void pairs_block(const typename Op::Args &args, int n
 {
     constexpr int D = Op::D, F = Op::F, OUT = Op::OUT, R = Op::R, TI = 64 * R;
     using lds_t = typename Op::lds_t;
-    auto sx = reinterpret_cast<double (*)[kTileJ]>(smem);
-    auto sf = reinterpret_cast<lds_t (*)[kTileJ]>(smem + sizeof(double) * D * kTileJ);
+    using SM = PairSmem<Op>;
     auto red = reinterpret_cast<double (*)[OUT][TI]>(smem);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 
@@ -148,10 +160,11 @@ __device__ __forceinline__ void pairs_block(const typename Op::Args &args, int n
             double x[D];
             lds_t f[F > 0 ? F : 1];
             Op::load_src(args, jt + threadIdx.x, x, f);
+            char *rec = smem + SM::rec * threadIdx.x;
 #pragma unroll
-            for (int c = 0; c < D; ++c) sx[c][threadIdx.x] = x[c];
+            for (int c = 0; c < F; ++c) reinterpret_cast<lds_t *>(rec)[c] = f[c];
 #pragma unroll
-            for (int c = 0; c < F; ++c) sf[c][threadIdx.x] = f[c];
+            for (int c = 0; c < D; ++c) reinterpret_cast<double *>(rec + SM::xoff)[c] = x[c];
         }
         __syncthreads();
         typename Op::Acc acc[R];
@@ -161,10 +174,11 @@ __device__ __forceinline__ void pairs_block(const typename Op::Args &args, int n
         for (int t = wave; t < cnt; t += kWaves) {
             double xj[D];
             lds_t fj[F > 0 ? F : 1];
+            const char *rec = smem + SM::rec * t;
 #pragma unroll
-            for (int c = 0; c < D; ++c) xj[c] = sx[c][t];
+            for (int c = 0; c < F; ++c) fj[c] = reinterpret_cast<const lds_t *>(rec)[c];
 #pragma unroll
-            for (int c = 0; c < F; ++c) fj[c] = sf[c][t];
+            for (int c = 0; c < D; ++c) xj[c] = reinterpret_cast<const double *>(rec + SM::xoff)[c];
 #pragma unroll
             for (int r = 0; r < R; ++r) Op::pair(args, tg[r], xj, fj, acc[r]);
         }

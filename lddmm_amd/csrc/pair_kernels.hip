@@ -194,7 +194,8 @@ void launch_bwd_pairs(int dim, int order, int prec, const double *q, const doubl
                       double width, const PairSplit &sp, double *part, hipStream_t st)
 {
     dispatch_all(dim, order, prec, [&](auto Dc, auto Oc, auto Pc) {
-        using Op = OpBwd<Dc(), Oc(), decltype(Pc)>;
+        constexpr int R = decltype(Pc)::fp32 ? 2 : 1;
+        using Op = OpBwd<Dc(), Oc(), decltype(Pc), R>;
         using real = typename Op::real;
         typename Op::Args A{q, a, p, n, (real)(1.0 / width), (real)(1.0 / (width * width))};
         launch_pairs<Op>(A, n, n, sp, 0, part, st);

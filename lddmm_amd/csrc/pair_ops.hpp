@@ -71,10 +71,11 @@ struct OpKV {
 // Fused backward pass of the adjoint (Cpp/src/objgrd.cpp:76-84): one kernel
 // evaluation per pair feeds K(q) amp and the dq sum of l^T K(q) r with
 // amp = a - p, l = amp - a/2, r = a.  Outputs: [K amp (D), dq (D)].
+// R_ targets per lane (two in FP32, where registers allow).
 // ---------------------------------------------------------------------------
-template <int D_, int ORDER, class P>
+template <int D_, int ORDER, class P, int R_ = 1>
 struct OpBwd {
-    static constexpr int D = D_, F = 3 * D_, OUT = 2 * D_, R = 1;
+    static constexpr int D = D_, F = 3 * D_, OUT = 2 * D_, R = R_;
     using real = typename P::real;
     using lds_t = real;
     struct Args {

@@ -17,7 +17,7 @@ template <int CEN, typename real>
 __device__ __forceinline__ real cen_value(real r2, real inv_cs2)
 {
     if constexpr (CEN == kCenG) {
-        if constexpr (sizeof(real) == 4) return __builtin_amdgcn_exp2f(-r2 * inv_cs2 * 1.4426950408889634f);
+        if constexpr (sizeof(real) == 4) return __builtin_amdgcn_exp2f(r2 * (inv_cs2 * -1.4426950408889634f));
         else return exp(-r2 * inv_cs2);
     } else {
         return real(1) / (real(1) + r2 * inv_cs2);
@@ -111,7 +111,8 @@ struct OpVf {
         const real gw = (CEN == kCenG) ? wk : wk * kc;
         const real wd = w * dku;
         acc.e += wk;
-        acc.s += wk - wd * ang;
+        // Binet: wk - wd ang = w ang^2 - 2 w ang^2 = -wk, so S = -E
+        if constexpr (DIR != kDirB) acc.s += wk - wd * ang;
 #pragma unroll
         for (int c = 0; c < D; ++c) {
             acc.g[c] += gw * df[c];
@@ -121,7 +122,7 @@ struct OpVf {
     __device__ __forceinline__ static void flush(const Acc &a, double (&tot)[OUT])
     {
         tot[0] += (double)a.e;
-        tot[1] += (double)a.s;
+        tot[1] += DIR == kDirB ? -(double)a.e : (double)a.s;
 #pragma unroll
         for (int c = 0; c < D; ++c) {
             tot[2 + c] += (double)a.g[c];

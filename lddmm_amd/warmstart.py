@@ -5,10 +5,10 @@ passes iniAlpStk, normally zeros).
 A per-vertex multilayer perceptron (32 point features -> 3, hidden layers
 of 128, ReLU) is trained on the GPU with the library's BF16 MFMA kernels
 (lddmm_mlp_*, FP32 master weights, Adam) to predict the endpoint
-displacement q_T - q_0 of each source vertex in solved registrations.  A
-few conjugate-gradient steps on the deformation kernel turn the predicted
-displacement into a momentum field, used as every time slice of the
-initial momentum stack; `choose_start` scales it per problem by the
+displacement q_T - q_0 of each source vertex in solved registrations.  The
+predicted displacement, divided by the vertex's kernel mass
+sum_j k(q_i, q_j), gives a smooth momentum field used as every time slice
+of the initial momentum stack; `choose_start` scales it per problem by the
 multiple with the lowest objective, zero included, so the plain start
 remains the fallback.
 """
@@ -159,34 +159,32 @@ class Mlp:
             pass
 
 
-def kernel_solve(q0, v, knl_order, knl_width, iters=15, reg=1e-4):
-    """Approximate momentum a with (K + reg I) a = v by `iters` conjugate
-    gradient steps per component (K applied by the GPU multiplyKernel).
-    Few iterations keep a smooth: the ill-conditioned directions of K are
-    left out instead of amplified."""
+def kernel_mass(q0, knl_order, knl_width):
+    """d_i = sum_j k(q_i, q_j) (GPU multiplyKernel against ones)."""
+    q = _f64(q0)
+    return multiplyKernel(q, np.ones_like(q), knl_order, knl_width)[:, 0]
+
+
+def velocity_to_momentum(q0, v, knl_order, knl_width, jacobi=0):
+    """A smooth momentum a with K a ~ v: the kernel-mass normalised
+    a = v / d (d_i = sum_j k(q_i, q_j)), optionally refined by Jacobi
+    steps a += (v - K a) / d.  Inverting K outright (conjugate gradients)
+    amplifies its ill-conditioned directions into rough momenta whose flow
+    overshoots; the normalised form stays as smooth as v."""
     q = _f64(q0)
     v = _f64(v)
-    a = np.zeros_like(v)
-    r = v.copy()
-    p = r.copy()
-    rr = np.sum(r * r, axis=0)
-    for _ in range(iters):
-        Kp = multiplyKernel(q, p, knl_order, knl_width) + reg * p
-        den = np.sum(p * Kp, axis=0)
-        step = np.where(den > 0, rr / np.where(den > 0, den, 1.0), 0.0)
-        a += step * p
-        r -= step * Kp
-        rr_new = np.sum(r * r, axis=0)
-        p = r + np.where(rr > 0, rr_new / np.where(rr > 0, rr, 1.0), 0.0) * p
-        rr = rr_new
+    d = kernel_mass(q, knl_order, knl_width)[:, None]
+    a = v / d
+    for _ in range(jacobi):
+        a = a + (v - multiplyKernel(q, a, knl_order, knl_width)) / d
     return a
 
 
 class WarmStart:
     """Feature standardisation + target scale + the GPU network.  The
     network predicts the endpoint displacement q_T - q_0 of each source
-    vertex; the warm-start momentum is the constant-in-time field whose
-    first-order flow K a (T - 1) h gives that displacement."""
+    vertex; the warm-start momentum is a smooth constant-in-time field
+    whose first-order flow K a (T - 1) h approximates that displacement."""
 
     def __init__(self, hidden=128, layers=4, seed=1):
         self.net = Mlp(hidden, layers, seed)
@@ -221,10 +219,10 @@ class WarmStart:
         f = self.features(problem)
         return self.net.forward((f - self.fmean) / self.fstd).astype(np.float64) * self.yscale
 
-    def predict(self, problem, knl_order, time_stp, time_num, cg_iters=15):
-        disp = self.displacement(problem)
-        v = disp / (time_stp * (time_num - 1))
-        a = kernel_solve(problem["q0"], v, knl_order, problem["knl_width"], cg_iters)
+    def predict(self, problem, knl_order, time_stp, time_num, jacobi=0):
+        """(T-1, N, 3) initial momentum stack for one problem."""
+        v = self.displacement(problem) / (time_stp * (time_num - 1))
+        a = velocity_to_momentum(problem["q0"], v, knl_order, problem["knl_width"], jacobi)
         return np.repeat(a[None], time_num - 1, axis=0)
 
     def save(self, path):
@@ -240,7 +238,7 @@ class WarmStart:
         return ws
 
 
-def choose_start(batch, warm, scales=(0.5, 0.75, 1.0, 1.25)):
+def choose_start(batch, warm, scales=(0.25, 0.5, 0.75, 1.0, 1.25, 1.5)):
     """Per problem, the multiple c * warm (c in scales, or c = 0: zero
     momentum) with the lowest objective, from one batched evaluation per
     scale.  Returns (starts, chosen scale per problem)."""

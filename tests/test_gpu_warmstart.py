@@ -114,9 +114,11 @@ def test_point_features_match_numpy(ws):
 
 
 def test_warm_start_reduces_iterations_on_held_out_problems(gpu, ws):
-    """Train on solved registrations, then register held-out problems from
-    the warm start and from zero: the warm start reaches the zero start's
-    final energy in fewer L-BFGS iterations in total."""
+    """Train on registrations the method solved, then register held-out
+    problems from the warm start and from zero momentum: starting from the
+    warm start lowers the objective at a small iteration budget for most
+    problems and reaches the zero start's final energy (within 1e-6) in
+    fewer L-BFGS iterations at the median, fallbacks counted."""
     k = dict(G.BATCH_KERNELS)
     T = 6
     k["time_num"], k["time_stp"] = T, 1.0 / (T - 1)
@@ -125,25 +127,38 @@ def test_warm_start_reduces_iterations_on_held_out_problems(gpu, ws):
         return gpu.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
                          k["knl_order"], k["time_stp"], k["time_num"], chunk=256)
 
-    train = G.batch_problems(24, seed=101, nmin=100, nmax=800)
+    train = G.batch_problems(48, seed=101, nmin=100, nmax=800)
     Bt = batch(train)
-    sol, _, _ = Bt.run(200, 0.0, 0.0, 0.5, 5, rel_energy_tol=1e-7)
+    sol, _, _ = Bt.run(300, 0.0, 0.0, 0.5, 5, precision="fp32", rel_energy_tol=1e-6)
     Bt.close()
     w = ws.WarmStart(hidden=128, layers=4, seed=3)
-    losses = w.fit(train, sol, k["knl_order"], k["time_stp"], T, steps=1500, batch_rows=2048, lr=2e-3)
+    losses = w.fit(train, sol, k["knl_order"], k["time_stp"], T, steps=3000, batch_rows=4096, lr=1e-3)
     print("warm-start training loss %.3e -> %.3e" % (losses[0], losses[-20:].mean()), flush=True)
-    test = G.batch_problems(8, seed=202, nmin=100, nmax=800)
+    assert losses[-20:].mean() < 0.1 * losses[0]
+    test = G.batch_problems(16, seed=202, nmin=100, nmax=800)
     B = batch(test)
     _, i0, _ = B.run(300, 0.0, 0.0, 0.5, 5, rel_energy_tol=1e-6)
     targets = np.array([r["fcn"] for r in i0]) * (1 + 1e-6)
     warm = [w.predict(p, k["knl_order"], k["time_stp"], T) for p in test]
     starts, scale = ws.choose_start(B, warm)
-    used = scale > 0
+    f0, _ = B.objgrd([np.zeros_like(a) for a in warm])
+    fs, _ = B.objgrd(starts)
+    assert np.all(fs <= f0)
+    _, z10, _ = B.run(10, 0.0, 0.0, 0.5, 5)
+    _, w10, _ = B.run(10, 0.0, 0.0, 0.5, 5, alp0=starts)
+    better10 = sum(b["fcn"] < a["fcn"] for a, b in zip(z10, w10))
     _, iw, _ = B.run(300, 0.0, 0.0, 0.5, 5, alp0=starts, rel_energy_tol=1e-6, targets=targets)
-    its0 = sum(r["iterations"] for r in i0)
-    itsw = sum(r["iterations"] for r in iw)
-    print("iterations zero start %d, warm start %d (warm used for %d of %d)" % (its0, itsw, used.sum(), len(test)),
-          flush=True)
-    assert used.sum() >= len(test) // 2
-    assert itsw < its0
+    its0 = np.array([r["iterations"] for r in i0])
+    reached = np.array([r["reached_target"] for r in iw])
+    # a warm-started run that misses the zero start's energy falls back to
+    # the zero-start run: it costs both runs' iterations
+    costw = np.array([r["iterations"] for r in iw]) + np.where(reached, 0, its0)
+    print("scales %s; lower energy after 10 iterations in %d of %d; iterations to the zero start's energy: "
+          "zero %s (median %d), warm %s (median %d), reached %d of %d" %
+          (np.round(scale, 2).tolist(), better10, len(test), its0.tolist(), np.median(its0), costw.tolist(),
+           np.median(costw), reached.sum(), len(test)), flush=True)
+    assert np.sum(scale > 0) >= len(test) // 2
+    assert better10 >= (2 * len(test)) // 3
+    assert reached.sum() >= len(test) // 2
+    assert np.median(costw) < 0.8 * np.median(its0)
     B.close()
