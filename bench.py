@@ -46,6 +46,20 @@ FLOPS_PER_PAIR = {
 TRANS_PER_PAIR = {("fwd_pairs", -1): 1, ("fwd_pairs", 1): 2, ("bwd_pairs", -1): 1, ("bwd_pairs", 1): 2,
                   ("vf_pairs", "GB"): 1}
 
+# Per-launch counter figures of the pair kernels on the sphere->cube workload, from
+# profiles/pair_kernels_sphere_cube.md (rocprofv3 PMC passes on this build):
+# traffic = HBM bytes per launch (FETCH_SIZE x 1.9925 + WRITE_SIZE, calibrated on a
+# copy kernel of known size), valu_busy = AMD's VALUBusy, valu_issue = share of SIMD
+# issue slots at 4 cycles per wave64 VALU instruction.
+ROCPROF_SPHERE_CUBE = {
+    ("fwd_pairs", "f64"): dict(traffic=5.635e6, valu_busy=0.75, valu_issue=0.72),
+    ("bwd_pairs", "f64"): dict(traffic=1.234e7, valu_busy=0.75, valu_issue=0.73),
+    ("vf_pairs", "f64"): dict(traffic=1.322e7, valu_busy=0.78, valu_issue=0.79),
+    ("fwd_pairs", "f32"): dict(traffic=8.055e6, valu_busy=0.80, valu_issue=0.74),
+    ("bwd_pairs", "f32"): dict(traffic=1.724e7, valu_busy=0.85, valu_issue=0.81),
+    ("vf_pairs", "f32"): dict(traffic=1.321e7, valu_busy=1.04, valu_issue=1.03),
+}
+
 
 def workload(name):
     if name == "sphere_cube":
@@ -286,10 +300,17 @@ def main():
                               tflops=(np_ * fl / avg / 1e12) if fl else None)
     dom = max(kernels, key=lambda k: prof[k][0])
     kd = kernels[dom]
+    rp = ROCPROF_SPHERE_CUBE.get((dom, "f64")) if args.config == "sphere_cube" else None
     roofline = dict(bound="valu", kernel=dom, achieved=kd["tflops"], peak=FP64_PEAK / 1e12, unit="TFLOP/s",
-                    frac=(kd["tflops"] / (FP64_PEAK / 1e12)) if kd["tflops"] else None, traffic=None,
+                    frac=(kd["tflops"] / (FP64_PEAK / 1e12)) if kd["tflops"] else None,
+                    traffic=rp["traffic"] if rp else None,
                     pairs_per_s=kd["pairs_per_s"], avg_launch_ms=kd["avg_ms"],
-                    flops_per_pair=kd["flops_per_pair"])
+                    flops_per_pair=kd["flops_per_pair"],
+                    valu_busy_rocprof=rp["valu_busy"] if rp else None,
+                    valu_issue_rocprof=rp["valu_issue"] if rp else None,
+                    note="VALU-issue bound: exp/sqrt count zero flops and FP64 exp/sqrt are instruction "
+                         "sequences, so algorithmic TFLOP/s sits below the issue-slot figures; the "
+                         "rocprof fields come from profiles/pair_kernels_sphere_cube.md")
 
     out = {
         "metric": "kernel pair-interactions/s (objective+gradient evaluation, FP64)",
@@ -338,9 +359,20 @@ def main():
         ms, cnt = p32.get(k, (0.0, 0))
         if cnt:
             k32[k] = dict(avg_ms=ms / cnt, pairs_per_s=np_ / (ms / cnt * 1e-3))
+    dom32 = max(k32, key=lambda k: k32[k]["avg_ms"] * p32[k][1]) if k32 else None
+    rp32 = ROCPROF_SPHERE_CUBE.get((dom32, "f32")) if args.config == "sphere_cube" else None
+    roof32 = None
+    if dom32 and per_launch[dom32][1]:
+        t32 = per_launch[dom32][0] * per_launch[dom32][1] / (k32[dom32]["avg_ms"] * 1e-3) / 1e12
+        roof32 = dict(bound="valu", kernel=dom32, achieved=t32, peak=FP64_PEAK / 1e12, unit="TFLOP/s",
+                      frac=t32 / (FP64_PEAK / 1e12), traffic=rp32["traffic"] if rp32 else None,
+                      valu_busy_rocprof=rp32["valu_busy"] if rp32 else None,
+                      valu_issue_rocprof=rp32["valu_issue"] if rp32 else None,
+                      note="FP32 VALU peak without packed FP32 = 78.6 TFLOP/s")
     out["fp32_policy"] = dict(value=pairs * args.steps / el32, ms_per_step=1e3 * el32 / args.steps,
                               speedup_vs_fp64=elapsed / el32, energy_rel_err=abs(f32 - f64) / abs(f64),
-                              grad_rel_err=float(np.max(np.abs(g32 - g64)) / np.max(np.abs(g64))), kernels=k32)
+                              grad_rel_err=float(np.max(np.abs(g32 - g64)) / np.max(np.abs(g64))), kernels=k32,
+                              roofline=roof32)
     obj.set_precision("fp64")
 
     if not args.no_registration:
