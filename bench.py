@@ -220,6 +220,58 @@ def batch_bench(count, nmin, nmax, itr, seed=4, train_count=64):
                 speedup_batched_fp32_warm_vs_plain_fp64=t_seq / t_warm)
 
 
+def precision_report(freq=100, seed=31, itr=10):
+    """FP32 policy vs FP64 on the single large problem: a seeded superquadric
+    source (geodesic frequency 100: 100,002 vertices, 200,000 triangles) and an
+    independent superquadric target (200,000 triangles), Gaussian kernel of
+    width 0.2, varifold Gaussian 0.1 x Binet, T-1 = 20.  Reports (1) one
+    evaluation in both policies at the same momentum (the FP64 run's result
+    after `itr` iterations): relative energy, gradient (max norm) and final
+    vertex error; (2) `itr`-iteration registrations from zero momentum in each
+    policy (FP32 without the FP64 finish), the FP32 result re-evaluated in
+    FP64, and the distance between the two final shapes."""
+    w = G.superquadric_pair(freq, seed, knl_width=0.2)
+    obj, ifo = make_objective(w)
+    n, d = w["q0"].shape
+    common = (w["q0"], None, w["elm"], ifo, w["tgt_cen"], w["tgt_vol"], w["tgt_dir"], w["cen_type"],
+              w["cen_width"], w["dir_type"], w["dir_width"], w["dis_wgt"], w["knl_order"], w["knl_width"],
+              w["time_stp"], w["time_num"])
+    a64, i64 = lddmm_amd.run(*common, itr, 0.0, 0.0, 0.5, 5, precision="fp64")
+    a32, i32 = lddmm_amd.run(*common, itr, 0.0, 0.0, 0.5, 5, precision="fp32", fp64_refine=False)
+    scale = float(np.max(np.abs(w["q0"])))
+    t0 = time.perf_counter()
+    f64, g64 = obj(a64)
+    t_e64 = time.perf_counter() - t0
+    q64 = obj.trajectory()[-1]
+    obj.set_precision("fp32")
+    t0 = time.perf_counter()
+    f32, g32 = obj(a64)
+    t_e32 = time.perf_counter() - t0
+    q32 = obj.trajectory()[-1]
+    obj.set_precision("fp64")
+    e32, _ = obj(a32)
+    qr32 = obj.trajectory()[-1]
+    _ = obj(a64)
+    qr64 = obj.trajectory()[-1]
+    disp = float(np.sqrt(np.mean(np.sum((q64 - w["q0"]) ** 2, axis=1))))
+    return dict(
+        workload="superquadric pair, N=%d vertices, Ed=%d, Et=%d triangles, Gaussian w=0.2, varifold G0.1xBinet, "
+                 "T-1=%d, seed %d" % (n, w["elm"].shape[0], len(w["tgt_vol"]), w["time_num"] - 1, seed),
+        pairs_per_eval=obj.pairs_per_eval(),
+        same_momentum=dict(energy_fp64=f64, energy_fp32=f32, energy_rel_err=abs(f32 - f64) / abs(f64),
+                           grad_rel_err=float(np.max(np.abs(g32 - g64)) / np.max(np.abs(g64))),
+                           final_vertex_max_err=float(np.max(np.abs(q32 - q64))),
+                           final_vertex_max_err_rel_to_extent=float(np.max(np.abs(q32 - q64))) / scale,
+                           rms_displacement=disp, eval_seconds_fp64=t_e64, eval_seconds_fp32=t_e32),
+        registration=dict(iterations=itr, fp64=dict(energy=i64["fcn"], seconds=i64["seconds"],
+                                                    evaluations=i64["evaluations"]),
+                          fp32=dict(energy=i32["fcn"], energy_fp64_eval=e32, seconds=i32["seconds"],
+                                    evaluations=i32["evaluations"]),
+                          energy_rel_diff=(e32 - i64["fcn"]) / abs(i64["fcn"]),
+                          final_shape_max_dist=float(np.max(np.linalg.norm(qr32 - qr64, axis=1))),
+                          final_shape_rms_dist=float(np.sqrt(np.mean(np.sum((qr32 - qr64) ** 2, axis=1))))))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -232,6 +284,8 @@ def main():
     ap.add_argument("--batch-nmin", type=int, default=500)
     ap.add_argument("--batch-nmax", type=int, default=5000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision-report", action="store_true",
+                    help="only the FP32-vs-FP64 report on the 100k-vertex superquadric pair")
     ap.add_argument("--batch-profile", action="store_true",
                     help="only the batched FP32 registration of the uneven batch (for kernel traces)")
     args = ap.parse_args()
@@ -240,6 +294,9 @@ def main():
               "target rows or by independent registrations; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    if args.precision_report:
+        print(json.dumps(precision_report()), flush=True)
+        return
     if args.batch_profile:
         probs = G.batch_problems(args.batch, 4, args.batch_nmin, args.batch_nmax)
         k = G.BATCH_KERNELS
