@@ -272,6 +272,54 @@ def precision_report(freq=100, seed=31, itr=10):
                           final_shape_rms_dist=float(np.sqrt(np.mean(np.sum((qr32 - qr64) ** 2, axis=1))))))
 
 
+def c4_report(count=256, nmin=500, nmax=20000, itr=300, seed=4):
+    """The batched-registration configuration at full size: `count` problems,
+    vertex counts log-uniform in [nmin, nmax].  The plain FP64 method
+    registers them one after another (zero start, stop at a relative energy
+    decrease below 1e-6 or `itr` iterations) and sets each problem's energy;
+    the batched FP32 policy then registers all of them in one run until each
+    reaches its plain energy x (1 + 1e-6) (at most 3 x `itr` iterations,
+    finishing in FP64 on a stall), and its results are re-evaluated in FP64."""
+    probs = G.batch_problems(count, seed, nmin, nmax)
+    k = G.BATCH_KERNELS
+    seq = []
+    t0 = time.perf_counter()
+    for i, p in enumerate(probs):
+        ifo = lddmm_amd.getElmIfo(p["elm"], len(p["q0"]), one_based=False)
+        _, info = lddmm_amd.run(p["q0"], None, p["elm"], ifo, p["tgt_cen"], p["tgt_vol"], p["tgt_dir"],
+                                k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
+                                k["knl_order"], p["knl_width"], k["time_stp"], k["time_num"], itr, 0.0, 0.0, 0.5, 5,
+                                rel_energy_tol=1e-6)
+        seq.append(info)
+        if i % 16 == 15:
+            print("plain FP64: %d of %d problems, %.1f s" % (i + 1, count, time.perf_counter() - t0),
+                  file=sys.stderr, flush=True)
+    t_seq = time.perf_counter() - t0
+    e_ref = np.array([i["fcn"] for i in seq])
+    B = lddmm_amd.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
+                        k["knl_order"], k["time_stp"], k["time_num"], precision="fp32", chunk=0)
+    chunk, pairs = B.info()
+    a32, i32, r32 = B.run(3 * itr, 0.0, 0.0, 0.5, 5, precision="fp32", targets=e_ref * (1.0 + 1e-6), progress=25)
+    t32 = i32[0]["seconds"]
+    B.close()
+    B64 = lddmm_amd.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
+                          k["knl_order"], k["time_stp"], k["time_num"], precision="fp64", chunk=chunk)
+    f64, _ = B64.objgrd(a32)
+    B64.close()
+    rel = (f64 - e_ref) / np.abs(e_ref)
+    return dict(problems=count, n_range=[int(min(B.n)), int(max(B.n))], chunk=chunk,
+                pairs_per_eval_total=float(pairs.sum()),
+                plain_fp64_sequential=dict(seconds=t_seq, registrations_per_s=count / t_seq,
+                                           median_iterations=float(np.median([i["iterations"] for i in seq]))),
+                batched_fp32=dict(seconds=t32, registrations_per_s=count / t32, rounds=r32,
+                                  median_iterations=float(np.median([r["iterations"] for r in i32])),
+                                  max_iterations=int(max(r["iterations"] for r in i32)),
+                                  reached_target=int(sum(r["reached_target"] for r in i32)),
+                                  within_1e6=int(np.sum(rel <= 1e-6 + 1e-8)),
+                                  max_rel_energy_vs_plain=float(rel.max())),
+                speedup_batched_fp32_vs_plain_fp64=t_seq / t32)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -286,6 +334,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision-report", action="store_true",
                     help="only the FP32-vs-FP64 report on the 100k-vertex superquadric pair")
+    ap.add_argument("--c4-full", action="store_true",
+                    help="only the full-size batched configuration (256 problems, N up to 20,000)")
     ap.add_argument("--batch-profile", action="store_true",
                     help="only the batched FP32 registration of the uneven batch (for kernel traces)")
     args = ap.parse_args()
@@ -296,6 +346,9 @@ def main():
 
     if args.precision_report:
         print(json.dumps(precision_report()), flush=True)
+        return
+    if args.c4_full:
+        print(json.dumps(c4_report()), flush=True)
         return
     if args.batch_profile:
         probs = G.batch_problems(args.batch, 4, args.batch_nmin, args.batch_nmax)

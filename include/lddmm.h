@@ -50,7 +50,9 @@ typedef struct lddmm_options {
     double target_energy;     /* used when has_target: stop once f <= target_energy */
     int has_target;
     int pair_target_blocks;   /* workgroups per all-pairs launch; 0 = default */
-    int reserved[4];
+    int progress;             /* batch runs: > 0 prints a progress line to stderr every
+                                 `progress` rounds; 0 = silent */
+    int reserved[3];
 } lddmm_options;
 
 /* Run summary; may be NULL. */
@@ -206,6 +208,10 @@ int lddmm_batch_run(lddmm_batch *b, const double *iniAlpStks, double *optAlpStks
                     const double *targetEnergies, lddmm_result *results, int *rounds);
 /* chosen source chunk and the pair interactions per evaluation of each problem */
 int lddmm_batch_info(lddmm_batch *b, int *chunk, double *pairsPerEval);
+/* on != 0 (default): each batched evaluation replays a hipGraph captured for
+ * the current active set; 0: launches are issued one by one.  The
+ * environment variable LDDMM_BATCH_GRAPHS=0 sets the default to off. */
+int lddmm_batch_set_graphs(lddmm_batch *b, int on);
 void lddmm_batch_destroy(lddmm_batch *b);
 
 /* ---------------------------------------------------------------- learned warm start

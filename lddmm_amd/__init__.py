@@ -39,7 +39,7 @@ class LddmmError(RuntimeError):
 class Options(C.Structure):
     _fields_ = [("precision", C.c_int), ("no_fp64_refine", C.c_int), ("rel_energy_tol", C.c_double),
                 ("refine_rel_tol", C.c_double), ("target_energy", C.c_double), ("has_target", C.c_int),
-                ("pair_target_blocks", C.c_int), ("reserved", C.c_int * 4)]
+                ("pair_target_blocks", C.c_int), ("progress", C.c_int), ("reserved", C.c_int * 3)]
 
 
 class Result(C.Structure):
@@ -363,6 +363,7 @@ class Batch:
             L.lddmm_batch_run.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_double,
                                           C.c_int, C.POINTER(Options), _dp, C.POINTER(Result), C.POINTER(C.c_int)]
             L.lddmm_batch_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), _dp]
+            L.lddmm_batch_set_graphs.argtypes = [C.c_void_p, C.c_int]
             L.lddmm_batch_destroy.argtypes = [C.c_void_p]
             L.lddmm_batch_destroy.restype = None
             L.lddmm_batch_create._set = True
@@ -400,10 +401,11 @@ class Batch:
         return f, self.split(g)
 
     def run(self, itr_max, tol, wolfe1, wolfe2, mem, alp0=None, precision="fp64", rel_energy_tol=0.0,
-            fp64_refine=True, refine_rel_tol=0.0, targets=None):
+            fp64_refine=True, refine_rel_tol=0.0, targets=None, progress=0):
         a0 = None if alp0 is None else _f64(np.concatenate([np.ravel(x) for x in alp0]))
         out = np.empty(int(self.offsets[-1]))
-        opts = Options(PRECISIONS[precision], 0 if fp64_refine else 1, rel_energy_tol, refine_rel_tol, 0.0, 0, 0)
+        opts = Options(PRECISIONS[precision], 0 if fp64_refine else 1, rel_energy_tol, refine_rel_tol, 0.0, 0, 0,
+                       int(progress))
         res = (Result * self.P)()
         rounds = C.c_int()
         tg = None if targets is None else _f64(targets)
@@ -413,6 +415,10 @@ class Batch:
                       iterations=r.iterations, evaluations=r.evaluations, status=r.status,
                       fp32_iterations=r.fp32_iterations, reached_target=bool(r.reached_target)) for r in res]
         return self.split(out), infos, rounds.value
+
+    def set_graphs(self, on):
+        """Replay captured hipGraphs for evaluations (default) or launch directly."""
+        _check(lib().lddmm_batch_set_graphs(self._h, int(bool(on))))
 
     def info(self):
         ch = C.c_int()

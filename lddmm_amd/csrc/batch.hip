@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <tuple>
@@ -554,6 +555,7 @@ Batch::Batch(const BatchSpec &spec, const double *q0, const int *elm, const doub
              const double *tu, hipStream_t st)
     : spec_(spec), st_(st), tab_(new Tables)
 {
+    if (const char *e = std::getenv("LDDMM_BATCH_GRAPHS")) use_graphs_ = std::atoi(e) != 0;
     const int P = spec_.P(), D = spec_.dim;
     if (P <= 0) throw Error(kErrArg, "a batch needs at least one problem");
     if (D != 2 && D != 3) throw Error(kErrArg, "dimension must be 2 or 3");
@@ -689,7 +691,7 @@ Batch::Batch(const BatchSpec &spec, const double *q0, const int *elm, const doub
     if (spec_.chunk <= 0) autotune();
 }
 
-Batch::~Batch() = default;
+Batch::~Batch() { clear_graphs(); }
 
 void Batch::build_descs()
 {
@@ -729,6 +731,7 @@ double Batch::pairs_per_eval(int p) const
 void Batch::build_tables(const std::vector<int> &prec_of)
 {
     if (prec_of == tab_key_) return;
+    clear_graphs();   // captured launches refer to the old tables
     const int P = spec_.P();
     Tables &T = *tab_;
     std::vector<int4> kv[2], bwd[2], vf[2], vt, et;
@@ -802,12 +805,61 @@ void Batch::compute_target_terms()
     LDDMM_HIP(hipStreamSynchronize(st_));
 }
 
+// One batched evaluation replays a captured hipGraph of its ~4T launches
+// when the same work tables and buffers were evaluated before: the graph
+// is captured on first use per (work tables, alpha, gradient, energy)
+// combination and dropped whenever the work tables are rebuilt (a problem
+// finished or changed precision).  The captured sequence holds kernel
+// launches only (no memset or copy nodes).  With the kernel-time profiler
+// on, or graphs switched off, the launches are issued directly.
 void Batch::eval(const double *alp, double *grd, double *f_dev, const std::vector<int> &prec_of)
 {
-    const int D = spec_.dim, T = spec_.T;
     build_tables(prec_of);
+    if (tab_->nprobs == 0) return;
+    if (!use_graphs_ || Profiler::get().enabled()) {
+        launch_eval(alp, grd, f_dev);
+        return;
+    }
+    for (const GraphEntry &e : graphs_)
+        if (e.alp == alp && e.grd == grd && e.f == f_dev) {
+            LDDMM_HIP(hipGraphLaunch(e.exec, st_));
+            return;
+        }
+    if (graphs_.size() >= 8) clear_graphs();
+    hipGraph_t graph = nullptr;
+    LDDMM_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeRelaxed));
+    try {
+        launch_eval(alp, grd, f_dev);
+    } catch (...) {
+        if (hipStreamEndCapture(st_, &graph) == hipSuccess && graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    LDDMM_HIP(hipStreamEndCapture(st_, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    LDDMM_HIP(ie);
+    graphs_.push_back(GraphEntry{alp, grd, f_dev, exec});
+    LDDMM_HIP(hipGraphLaunch(exec, st_));
+}
+
+void Batch::clear_graphs()
+{
+    for (GraphEntry &e : graphs_)
+        if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    graphs_.clear();
+}
+
+void Batch::set_graphs(bool on)
+{
+    use_graphs_ = on;
+    if (!on) clear_graphs();
+}
+
+void Batch::launch_eval(const double *alp, double *grd, double *f_dev)
+{
+    const int D = spec_.dim, T = spec_.T;
     const Tables &tb = *tab_;
-    if (tb.nprobs == 0) return;
     BatchView v{desc_, nde_, padj_, part_, vpart_, alp, grd, q0_, tc_, tu_, ta_, elm_, ifo_, rows_, T,
                 spec_.h, spec_.w, ecen_, edir_, evol_, dc_, dd_, efwd_, evf_};
 
@@ -1150,6 +1202,9 @@ BatchRunResult Batch::run(double *x, const BatchLbfgsOptions &bo)
         if (!(st[p].glen >= o.tol) || o.itr_max == 0) st[p].active = false;
     }
 
+    auto has_target = [&](int p) { return p < (int)bo.targets.size() || o.has_target; };
+    auto target_of = [&](int p) { return p < (int)bo.targets.size() ? bo.targets[p] : o.target; };
+
     // per-problem refine to FP64 (re-evaluate the current iterate)
     auto refine = [&](const std::vector<int> &which) {
         std::vector<int> pr(P, -1);
@@ -1180,10 +1235,16 @@ BatchRunResult Batch::run(double *x, const BatchLbfgsOptions &bo)
 
     int rounds = 0;
     while (true) {
-        bool any = false;
-        for (int p = 0; p < P; ++p) any |= st[p].active;
-        if (!any) break;
+        int nactive = 0;
+        for (int p = 0; p < P; ++p) nactive += st[p].active ? 1 : 0;
+        if (!nactive) break;
         ++rounds;
+        if (bo.progress > 0 && rounds % bo.progress == 0) {
+            const double el =
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            std::fprintf(stderr, "batch round %d: %d of %d problems active, %.1f s\n", rounds, nactive, P, el);
+            std::fflush(stderr);
+        }
 
         // ---- directions for the problems starting an iteration
         std::vector<int> dirp;
@@ -1293,6 +1354,11 @@ BatchRunResult Batch::run(double *x, const BatchLbfgsOptions &bo)
                 q.bis = q.epd = q.cnt = 0;
             }
             refine(stalled);
+            for (int p : stalled)
+                if (has_target(p) && st[p].f <= target_of(p)) {
+                    st[p].reached = 1;
+                    st[p].active = false;
+                }
             if (!stalled.empty()) continue;
         }
 
@@ -1348,9 +1414,7 @@ BatchRunResult Batch::run(double *x, const BatchLbfgsOptions &bo)
                 q.need_dir = true;
                 // stop rules (checked after the accept below)
                 q.reached = 0;
-                const double tgt = p < (int)bo.targets.size() ? bo.targets[p] : o.target;
-                const bool has_t = p < (int)bo.targets.size() || o.has_target;
-                if (has_t && q.f <= tgt) q.reached = 1;
+                if (has_target(p) && q.f <= target_of(p)) q.reached = 1;
                 q.fprev = fprev;
                 continue;
             }
@@ -1400,7 +1464,13 @@ BatchRunResult Batch::run(double *x, const BatchLbfgsOptions &bo)
                 const double fprev = q.fprev;
                 q.rel = 1.0;
                 if (q.reached) {
-                    q.active = false;
+                    // a target met in FP32 arithmetic is confirmed in FP64 before the run stops
+                    if (can_refine && q.prec == kFP32) {
+                        q.reached = 0;
+                        stalled.push_back(p);
+                    } else {
+                        q.active = false;
+                    }
                     continue;
                 }
                 if (q.it >= o.itr_max || q.glen < o.tol) {
@@ -1416,6 +1486,11 @@ BatchRunResult Batch::run(double *x, const BatchLbfgsOptions &bo)
             }
         }
         refine(stalled);
+        for (int p : stalled)
+            if (has_target(p) && st[p].f <= target_of(p)) {
+                st[p].reached = 1;
+                st[p].active = false;
+            }
     }
     LDDMM_HIP(hipStreamSynchronize(st_));
     BatchRunResult r;

@@ -48,6 +48,7 @@ struct BatchLbfgsOptions {
     LbfgsOptions base;   // itr_max, mem, tol, wolfe1, wolfe2, rel_tol, refine, target (shared)
     std::vector<double> targets;   // per-problem target energies (optional)
     int prec = -1;                 // pair arithmetic of the run; -1 = the batch's
+    int progress = 0;              // > 0: a progress line on stderr every `progress` rounds
 };
 
 struct BatchProblemResult {
@@ -89,6 +90,9 @@ public:
     const double *target_terms() const { return tt_; }
 
     BatchRunResult run(double *x, const BatchLbfgsOptions &o);
+    // evaluations replay captured hipGraphs (default on)
+    void set_graphs(bool on);
+    bool graphs() const { return use_graphs_; }
 
 private:
     BatchSpec spec_;
@@ -108,6 +112,15 @@ private:
     struct Tables;
     std::unique_ptr<Tables> tab_;
     std::vector<int> tab_key_;
+    struct GraphEntry {
+        const double *alp;
+        double *grd, *f;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs_;
+    bool use_graphs_ = true;
+    void clear_graphs();
+    void launch_eval(const double *alp, double *grd, double *f_dev);
     void build_descs();
     void build_tables(const std::vector<int> &prec_of);
     void compute_target_terms();

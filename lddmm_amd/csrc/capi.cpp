@@ -529,7 +529,10 @@ int lddmm_batch_run(lddmm_batch *b, const double *iniAlpStks, double *optAlpStks
         Batch &B = *b->b;
         BatchLbfgsOptions bo;
         bo.base = make_opts(itrMax, tolVal, wolfe1, wolfe2, memNum, 0, opts);
-        if (opts) bo.prec = precision_of(opts);
+        if (opts) {
+            bo.prec = precision_of(opts);
+            bo.progress = opts->progress;
+        }
         if (targetEnergies) bo.targets.assign(targetEnergies, targetEnergies + B.P());
         std::vector<double> hx((size_t)B.varnum_total(), 0.0);
         if (iniAlpStks) hx = B.to_device_layout(iniAlpStks);
@@ -566,6 +569,16 @@ int lddmm_batch_info(lddmm_batch *b, int *chunk, double *pairsPerEval)
         if (chunk) *chunk = b->b->chunk();
         if (pairsPerEval)
             for (int p = 0; p < b->b->P(); ++p) pairsPerEval[p] = b->b->pairs_per_eval(p);
+        return 0;
+    });
+}
+
+int lddmm_batch_set_graphs(lddmm_batch *b, int on)
+{
+    return guarded([&] {
+        require(b != nullptr, "null argument");
+        b->st.sync();
+        b->b->set_graphs(on != 0);
         return 0;
     });
 }

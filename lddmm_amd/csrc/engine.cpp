@@ -434,6 +434,10 @@ LbfgsResult run_lbfgs(Objective &obj, double *x, const LbfgsOptions &o, hipStrea
         const double slope = s.host[kGD];
         if (slope > -DBL_MIN && in_fp32 && can_refine) {
             refine("not a descent direction");
+            if (o.has_target && fnow <= o.target) {
+                r.reached_target = 1;
+                break;
+            }
             --it;
             continue;
         }
@@ -486,6 +490,10 @@ LbfgsResult run_lbfgs(Objective &obj, double *x, const LbfgsOptions &o, hipStrea
         evals += cnt;
         if (le != 0 && in_fp32 && can_refine) {
             refine("line search failed");
+            if (o.has_target && fnow <= o.target) {
+                r.reached_target = 1;
+                break;
+            }
             --it;
             continue;
         }
@@ -528,12 +536,21 @@ LbfgsResult run_lbfgs(Objective &obj, double *x, const LbfgsOptions &o, hipStrea
         push_trace(fnow, glen, step, cnt);
         if (in_fp32) ++r.fp32_iters;
         if (o.has_target && fnow <= o.target) {
-            r.reached_target = 1;
-            break;
+            // a target met in FP32 arithmetic is confirmed in FP64 before the run stops
+            if (in_fp32 && can_refine) refine("target met in FP32, confirming in FP64");
+            if (fnow <= o.target) {
+                r.reached_target = 1;
+                break;
+            }
+            continue;
         }
         const bool small = (fprev - fnow) < o.refine_rel_tol * std::fabs(fnow);
         if (in_fp32 && can_refine && small) {
             refine("relative decrease below threshold");
+            if (o.has_target && fnow <= o.target) {
+                r.reached_target = 1;
+                break;
+            }
             continue;
         }
         if (o.rel_tol > 0.0 && (fprev - fnow) < o.rel_tol * std::fabs(fnow)) break;

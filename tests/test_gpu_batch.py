@@ -103,3 +103,57 @@ def test_batched_run_relative_decrease_stop_and_fp32(gpu):
             stopped += 1
             assert abs(f32[i] - f64[i]) <= 2e-3 * abs(f64[i]), (i, f32[i], f64[i])
     assert stopped >= 1
+
+
+def test_batched_fp32_reaches_fp64_targets_confirmed_in_fp64(gpu):
+    """Each problem's FP32-policy run stops only once its energy, evaluated in
+    FP64, is at or below the FP64 run's final energy x (1 + 1e-6)."""
+    probs, k = small_batch(), kernels()
+    B = gpu.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"], k["knl_order"],
+                  k["time_stp"], k["time_num"], chunk=256)
+    _, i64, _ = B.run(60, 0.0, 0.0, 0.5, 5)
+    targets = np.array([r["fcn"] for r in i64]) * (1 + 1e-6)
+    a32, i32, r32 = B.run(600, 0.0, 0.0, 0.5, 5, precision="fp32", targets=targets)
+    f, _ = B.objgrd(a32)
+    print("fp32 to fp64 targets: rounds %d, iterations %s, fp32 iterations %s" %
+          (r32, [r["iterations"] for r in i32], [r["fp32_iterations"] for r in i32]), flush=True)
+    for i in range(B.P):
+        assert i32[i]["reached_target"], i
+        assert f[i] <= targets[i] * (1 + 1e-13), (i, f[i], targets[i])
+        assert abs(f[i] - i32[i]["fcn"]) <= 1e-12 * abs(f[i])
+    B.close()
+
+
+def test_graph_replays_match_direct_launches(gpu):
+    """Evaluations replayed from captured hipGraphs equal the directly
+    launched ones bitwise, replay after replay, with eager work and
+    synchronisation in between and with the work tables rebuilt."""
+    probs, k = small_batch(), kernels()
+    B = gpu.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"], k["knl_order"],
+                  k["time_stp"], k["time_num"], chunk=256)
+    alps = [0.05 * G.normal(80 + i, B.sizes[i]).reshape(k["time_num"] - 1, B.n[i], 3) for i in range(B.P)]
+    other = [0.5 * a for a in alps]
+    B.set_graphs(False)
+    f_ref, g_ref = B.objgrd(alps)
+    f_oth, g_oth = B.objgrd(other)
+    B.set_graphs(True)
+    for rep in range(4):
+        f, g = B.objgrd(alps if rep % 2 == 0 else other)
+        fr, gr = (f_ref, g_ref) if rep % 2 == 0 else (f_oth, g_oth)
+        assert np.array_equal(f, fr), rep
+        assert all(np.array_equal(a, b) for a, b in zip(g, gr)), rep
+        # eager work between replays: a single-problem evaluation on its own stream
+        obj, _ = single(gpu, probs[rep % len(probs)], k)
+        obj(alps[rep % len(probs)])
+        obj.close()
+    # a run (which rebuilds the tables as problems finish) is unchanged by graphs
+    B.set_graphs(False)
+    a1, i1, r1 = B.run(12, 0.0, 0.0, 0.5, 5, precision="fp32")
+    B.set_graphs(True)
+    a2, i2, r2 = B.run(12, 0.0, 0.0, 0.5, 5, precision="fp32")
+    print("graph replays: run rounds %d / %d" % (r1, r2), flush=True)
+    assert r1 == r2
+    for x, y, p, q in zip(a1, a2, i1, i2):
+        assert np.array_equal(x, y)
+        assert p["fcn"] == q["fcn"] and p["iterations"] == q["iterations"]
+    B.close()

@@ -128,8 +128,7 @@ def test_fp32_policy_reaches_fp64_energy(gpu):
     """The registration protocol of the metric: the plain FP64 run (200
     iterations) sets E_ref; the FP32 policy (finishing in FP64 if its FP32
     phase stalls) runs until f <= E_ref (1 + 1e-6); its result, re-evaluated
-    in FP64, must be within 1e-6 of E_ref (plus the FP32 energy's own error
-    budget of 1e-8 relative)."""
+    in FP64, must be within 1e-6 of E_ref."""
     print("fp32 vs fp64 registration, circle_square 400", flush=True)
     w = circle_square_args()
     n = w["q0"].shape[0]
@@ -148,7 +147,8 @@ def test_fp32_policy_reaches_fp64_energy(gpu):
           (i64["iterations"], e_ref, i64["seconds"], i32["iterations"], i32["fp32_iterations"], f32,
            i32["seconds"], (f32 - e_ref) / abs(e_ref), i32["reached_target"]), flush=True)
     assert i32["reached_target"]
-    assert f32 <= e_ref * (1 + 1e-6 + 1e-8)
+    # a target met in FP32 is confirmed by an FP64 evaluation before the run stops
+    assert f32 <= e_ref * (1 + 1e-6) * (1 + 1e-13)
 
 
 def test_fp32_phase_hands_over_to_fp64_on_stall(gpu):
