@@ -256,6 +256,14 @@ int lddmm_gemm_bf16(float *C, const uint16_t *A, const uint16_t *B, int M, int N
  * the (1 + maxAdj*V) x N table when elmIfoMat != NULL and returns the row
  * count (negative on error). */
 int lddmm_get_elm_ifo(int32_t *elmIfoMat, const int32_t *elmVtxMat, int vtxPerElm, int elmNum, int ndeNum);
+/* Element varifold of a mesh (the target preprocessing of the examples,
+ * Cpp/examples/circle2square/run_circle2square.m:21-24; the same formulas as
+ * landmarksToVarifoldKernel, Cpp/src/varifold3D.cpp:8-39 and
+ * Cpp/src/varifold2D.cpp:8-32): centres D x E, unit directions (2D: unit
+ * tangent; 3D: unit normal (v1-v0)x(v2-v0)/|.|) D x E, volumes (length or
+ * area) E, from ndeMat D x N and a 0-based V x E element table.  Host layout. */
+int lddmm_element_varifold(double *cenPosMat, double *uniDirMat, double *elmVolVec, const double *ndeMat,
+                           const int32_t *elmVtxMat, int ndeNum, int elmNum, int dimNum);
 /* computeNodePosition (Cpp/src/computeNodePosition.m:1-20): ndeStk D x N x T,
  * vlcStk D x N x (T-1) (either may be NULL).  Host layout. */
 int lddmm_compute_node_position(double *ndeStk, double *vlcStk, const double *iniNdeMat,

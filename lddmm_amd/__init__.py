@@ -19,7 +19,7 @@ import os
 
 import numpy as np
 
-__all__ = ["lib", "LDDMM", "getElmIfo", "computeNodePosition", "multiplyKernel", "dqKernel", "computeKernel",
+__all__ = ["lib", "LDDMM", "getElmIfo", "computeNodePosition", "elementVarifold", "multiplyKernel", "dqKernel", "computeKernel",
            "varifold", "Objective", "run", "LddmmError", "profile_enable", "profile_read", "profile_reset"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -214,6 +214,22 @@ def varifold(q, elm, ifo, tgt_cen, tgt_dir, tgt_vol, cen_type, cen_width, dir_ty
                                 _ch(cen_type), cen_width, _ch(dir_type), dir_width, q.shape[0], elm.shape[0],
                                 ifo.shape[1], len(ta), q.shape[1]))
     return (val.value, dq) if grad else val.value
+
+
+def elementVarifold(ndeMat, elmVtxMat):
+    """Element centres, unit directions and volumes of a mesh (the target
+    preprocessing of Cpp/examples/circle2square/run_circle2square.m:21-24;
+    landmarksToVarifoldKernel, Cpp/src/varifold3D.cpp:8-39).  ndeMat (N, D),
+    elmVtxMat (E, D) 0-based -> (cen (E, D), dir (E, D), vol (E,))."""
+    L = lib()
+    if not hasattr(L.lddmm_element_varifold, "_set"):
+        L.lddmm_element_varifold.argtypes = [_dp, _dp, _dp, _dp, _ip, C.c_int, C.c_int, C.c_int]
+        L.lddmm_element_varifold._set = True
+    q, e = _f64(ndeMat), _i32(elmVtxMat)
+    n, d = q.shape
+    cen, udir, vol = np.empty((len(e), d)), np.empty((len(e), d)), np.empty(len(e))
+    _check(L.lddmm_element_varifold(_p(cen), _p(udir), _p(vol), _p(q), _p(e), n, len(e), d))
+    return cen, udir, vol
 
 
 def computeNodePosition(iniNdeMat, alpStk, knlOrder, knlWidth, timeStp, timeNum):

@@ -15,6 +15,7 @@
 #include "../../include/lddmm.h"
 #include "batch.hpp"
 #include "engine.hpp"
+#include "kernels.hpp"
 #include "mlp.hpp"
 
 using namespace lddmm;
@@ -938,6 +939,34 @@ int lddmm_get_elm_ifo(int32_t *elmIfoMat, const int32_t *elmVtxMat, int vtxPerEl
             }
         }
         return rows;
+    });
+}
+
+int lddmm_element_varifold(double *cenPosMat, double *uniDirMat, double *elmVolVec, const double *ndeMat,
+                           const int32_t *elmVtxMat, int ndeNum, int elmNum, int dimNum)
+{
+    return guarded([&] {
+        require(cenPosMat && uniDirMat && elmVolVec && ndeMat && elmVtxMat, "null argument");
+        require(dimNum == 2 || dimNum == 3, "dimNum must be 2 or 3");
+        require(ndeNum > 0 && elmNum > 0, "ndeNum and elmNum must be positive");
+        for (long long k = 0; k < (long long)elmNum * dimNum; ++k)
+            require(elmVtxMat[k] >= 0 && elmVtxMat[k] < ndeNum, "element table holds a vertex outside [0, ndeNum)");
+        Stream st;
+        auto hq = to_soa(ndeMat, ndeNum, dimNum);
+        auto he = to_soa(elmVtxMat, elmNum, dimNum);
+        DevBuf<double> q(hq.size()), cen((size_t)dimNum * elmNum), dir((size_t)dimNum * elmNum), vol(elmNum);
+        DevBuf<int> e(he.size());
+        q.upload(hq.data(), st.s);
+        e.upload(he.data(), st.s);
+        launch_elm_geometry(dimNum, q.p, ndeNum, e.p, elmNum, cen.p, dir.p, vol.p, st.s);
+        std::vector<double> hc((size_t)dimNum * elmNum), hd((size_t)dimNum * elmNum);
+        cen.download(hc.data(), st.s);
+        dir.download(hd.data(), st.s);
+        vol.download(elmVolVec, st.s);
+        st.sync();
+        from_soa(cenPosMat, hc.data(), elmNum, dimNum);
+        from_soa(uniDirMat, hd.data(), elmNum, dimNum);
+        return 0;
     });
 }
 

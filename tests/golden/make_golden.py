@@ -1,6 +1,6 @@
 """Writes the golden vectors in this folder from the reference's own kernel
 sources, compiled in place by oracle/Makefile (`make -C oracle ref`) into
-oracle/_ref/libref{2,3}d.so.
+oracle/_ref/libref{2,3}d.so, and from the reference's example data files.
 
 Run in the build container (the reference tree is not on the GPU box):
     python tests/golden/make_golden.py
@@ -94,10 +94,36 @@ def varifolds():
     np.savez_compressed(os.path.join(HERE, "varifold.npz"), **out)
 
 
+def example_sphere2cube():
+    """The reference's own sphere->cube example data (Cpp/examples/sphere2cube/
+    sphereData.mat, cubeData.mat; identical copies under CUDA/ and MATLAB/),
+    read with scipy.io.loadmat (plain arrays, nothing executed), plus the
+    compiled reference's varifold value and gradient at the source with the
+    example's matching parameters ('C' 0.1 x 'B', run_sphere2cube.m:11-19)."""
+    import scipy.io as sio
+    src = os.path.join(REF_ROOT, "Cpp", "examples", "sphere2cube")
+    out = {}
+    for name in ("sphereData.mat", "cubeData.mat"):
+        d = sio.loadmat(os.path.join(src, name))
+        out.update({k: np.asarray(v) for k, v in d.items() if not k.startswith("__")})
+    q = np.ascontiguousarray(out["iniNdeMat"].T)
+    e = np.ascontiguousarray(out["elmVtxMat"].T.astype(np.int32) - 1)
+    ifo = oracle.get_elm_ifo(e, q.shape[0])
+    tc = np.ascontiguousarray(out["tgtCenPosMat"].T)
+    tu = np.ascontiguousarray(out["tgtUniDirMat"].T)
+    ta = np.ascontiguousarray(out["tgtElmVolVec"].ravel())
+    val, dq = oracle.varifold(q, e, ifo, tc, tu, ta, "C", 0.1, "B", 0.0, grad=True, L=oracle.ref(3), prefix="ref")
+    out["ref_vf_val"], out["ref_vf_dq"] = np.array(val), dq
+    np.savez_compressed(os.path.join(HERE, "sphere2cube.npz"), **out)
+
+
+REF_ROOT = "/root/reference"
+
 if __name__ == "__main__":
     oracle.build()
     if oracle.ref(2) is None or oracle.ref(3) is None:
         raise SystemExit("oracle/_ref was not built (needs the reference tree)")
     kernels()
     varifolds()
+    example_sphere2cube()
     print("golden vectors written to", HERE)
