@@ -16,8 +16,8 @@
  * column-major (SoA), stacks N x D x (T-1), elements E x V, adjacency
  * N x (1 + maxAdj*V).
  *
- * Kernel orders: -1 Gaussian, 0..4 Matern (Cpp/src/computeKernel.cpp:23-24).
- * 2D Matern orders return LDDMM_ERR_UNSUPPORTED on this GPU path.
+ * Kernel orders: -1 Gaussian, 0..4 Matern (Cpp/src/computeKernel.cpp:23-24),
+ * in 2D and 3D (2D Matern through the Bessel factors x^2 K0(x), x K1(x)).
  */
 #ifndef LDDMM_AMD_H
 #define LDDMM_AMD_H

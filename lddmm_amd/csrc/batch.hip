@@ -560,7 +560,6 @@ Batch::Batch(const BatchSpec &spec, const double *q0, const int *elm, const doub
     if (P <= 0) throw Error(kErrArg, "a batch needs at least one problem");
     if (D != 2 && D != 3) throw Error(kErrArg, "dimension must be 2 or 3");
     if (spec_.T < 2) throw Error(kErrArg, "timeNum must be at least 2");
-    if (D == 2 && spec_.order >= 0) throw Error(kErrUnsupported, "2D Matern kernels are not implemented on the GPU");
     if (spec_.order < -1 || spec_.order > 4) throw Error(kErrArg, "knlOrder must be -1 or 0..4");
     if ((int)spec_.ne.size() != P || (int)spec_.nt.size() != P || (int)spec_.width.size() != P)
         throw Error(kErrArg, "per-problem arrays must have one entry per problem");

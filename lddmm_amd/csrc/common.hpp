@@ -11,6 +11,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <type_traits>
 #include <stdexcept>
 #include <string>
 
@@ -101,6 +102,116 @@ struct KernelFn {
         }
     }
 };
+
+// ---------------------------------------------------------------------------
+// 2D Matern kernels need p0 = x^2 K0(x) and p1 = x K1(x) (modified Bessel
+// functions of the second kind).  FP64 on the device, the oracle's method:
+// the ascending series (A&S 9.6.13, 9.6.11) for 0 < x <= 2,
+//   K0 = -ln(x/2) I0 + sum_k psi(k+1) t^k/(k!)^2,  t = x^2/4,
+//   K1 = 1/x + ln(x/2) I1 - (x/4) sum_k (psi(k+1) + psi(k+2)) t^k/(k!(k+1)!),
+// and for x > 2 Steed's continued fraction for K1/K0 with the
+// Thompson-Barnett normalising sum (Thompson & Barnett, J. Comput. Phys. 64
+// (1986) 490; Temme 1975): K0 = sqrt(pi/(2x)) e^-x / S, K1 = K0 (x + 1/2 - h)/x.
+// Limits p0(0) = 0, p1(0) = 1 and p0 = p1 = 0 beyond x = 700 as in the
+// reference (Cpp/src/polybesselk.cpp:13-17, 54-58).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void bessel_p01(double x, double &p0, double &p1)
+{
+    if (x < 2.220446049250313e-16) {
+        p0 = 0.0;
+        p1 = 1.0;
+        return;
+    }
+    if (x > 700.0) {
+        p0 = 0.0;
+        p1 = 0.0;
+        return;
+    }
+    double k0, k1;
+    if (x <= 2.0) {
+        const double t = 0.25 * x * x, lg = log(0.5 * x);
+        double i0 = 0.0, i1 = 0.0, s0 = 0.0, s1 = 0.0, term0 = 1.0, term1 = 1.0, hk = 0.0;
+        for (int k = 0; k < 40; ++k) {
+            const double psi1 = -0.57721566490153286061 + hk;
+            const double psi2 = psi1 + 1.0 / (double)(k + 1);
+            i0 += term0;
+            i1 += term1;
+            s0 += psi1 * term0;
+            s1 += (psi1 + psi2) * term1;
+            if (term0 < 1e-18 * i0 && k > 2) break;
+            hk += 1.0 / (double)(k + 1);
+            term0 *= t / ((double)(k + 1) * (double)(k + 1));
+            term1 *= t / ((double)(k + 1) * (double)(k + 2));
+        }
+        k0 = -lg * i0 + s0;
+        k1 = 1.0 / x + lg * (0.5 * x * i1) - 0.25 * x * s1;
+    } else {
+        const double a1 = 0.25;
+        double b = 2.0 * (1.0 + x), d = 1.0 / b, h = d, dh = d;
+        double qm1 = 0.0, qc = 1.0, a = -a1, c = a1, q = a1, s = 1.0 + q * dh;
+        for (int i = 2; i < 400; ++i) {
+            a -= 2.0 * (double)(i - 1);
+            c = -a * c / (double)i;
+            const double qn = (qm1 - b * qc) / a;
+            qm1 = qc;
+            qc = qn;
+            q += c * qn;
+            b += 2.0;
+            d = 1.0 / (b + a * d);
+            dh = (b * d - 1.0) * dh;
+            h += dh;
+            const double ds = q * dh;
+            s += ds;
+            if (fabs(ds) < 1e-17 * fabs(s)) break;
+        }
+        h *= a1;
+        k0 = sqrt(1.5707963267948966192 / x) * exp(-x) / s;
+        k1 = k0 * (x + 0.5 - h) / x;
+    }
+    p0 = x * x * k0;
+    p1 = x * k1;
+}
+
+// Kernel value k and gradient factor c (grad_{q_i} k = c (q_i - q_j)) of
+// the 2D Matern family, order p (nu = p + 1), d = |q_i - q_j|/sigma
+// (Cpp/src/computeKernel.cpp:60,83,105,127,149; Cpp/src/dqKernel.cpp:91,
+// 122,153,184; order 0 has no derivative in the reference: c = 0).
+template <int ORDER>
+struct KernelFn2D {
+    __device__ __forceinline__ static void eval(double r2, double inv_w, double inv_w2, double &k, double &c)
+    {
+        const double d = sqrt(r2) * inv_w, d2 = r2 * inv_w2;
+        double p0, p1;
+        bessel_p01(d, p0, p1);
+        if constexpr (ORDER == 0) {
+            k = p1;
+            c = 0.0;
+        } else if constexpr (ORDER == 1) {
+            k = 0.5 * (p0 + 2.0 * p1);
+            c = -p1 * (0.5 * inv_w2);
+        } else if constexpr (ORDER == 2) {
+            k = (4.0 * p0 + (8.0 + d2) * p1) * 0.125;
+            c = -(p0 + 2.0 * p1) * (0.125 * inv_w2);
+        } else if constexpr (ORDER == 3) {
+            k = ((24.0 + d2) * p0 + 8.0 * (6.0 + d2) * p1) * (1.0 / 48.0);
+            c = -(4.0 * p0 + (8.0 + d2) * p1) * (inv_w2 / 48.0);
+        } else {
+            k = (12.0 * (16.0 + d2) * p0 + (384.0 + d2 * (72.0 + d2)) * p1) * (1.0 / 384.0);
+            c = -((24.0 + d2) * p0 + 8.0 * (6.0 + d2) * p1) * (inv_w2 / 384.0);
+        }
+    }
+    __device__ __forceinline__ static double value(double r2, double inv_w, double inv_w2)
+    {
+        double k, c;
+        eval(r2, inv_w, inv_w2, k, c);
+        return k;
+    }
+};
+
+// The kernel family of a dimension: 2D Matern orders through the Bessel
+// factors, Gaussian (both dimensions) and 3D Matern through KernelFn.
+template <int D, int ORDER>
+using KernelFnD = std::conditional_t<(D == 2 && ORDER >= 0), KernelFn2D<ORDER>, KernelFn<ORDER>>;
 
 // Dispatch a runtime order onto a template lambda f(std::integral_constant<int, O>).
 template <typename F>

@@ -124,8 +124,6 @@ void validate(const DeviceProblem &p)
     if (p.n <= 0) throw Error(kErrArg, "ndeNum must be positive");
     if (p.T < 2) throw Error(kErrArg, "timeNum must be at least 2");
     if (p.order < -1 || p.order > 4) throw Error(kErrArg, "knlOrder must be -1 or 0..4");
-    if (p.dim == 2 && p.order >= 0)
-        throw Error(kErrUnsupported, "2D Matern kernels are not implemented on the GPU path");
     if (!(p.width > 0.0)) throw Error(kErrArg, "knlWidth must be positive");
     if (p.ne <= 0 || p.nt < 0) throw Error(kErrArg, "element counts must be positive");
     if (p.ifo_rows < 1) throw Error(kErrArg, "dfmElmIfoNum must be at least 1");

@@ -45,19 +45,20 @@ struct PrecF32 {
 };
 
 // Kernel family in the policy's arithmetic: value k and factor c of
-// grad_{q_i} k = c (q_i - q_j), from the squared distance.
-template <int ORDER, typename real>
+// grad_{q_i} k = c (q_i - q_j), from the squared distance.  D selects the
+// 2D Matern family (Bessel factors, evaluated in FP64 under both policies).
+template <int ORDER, typename real, int D = 3>
 struct KernelFnT;
 
-template <int ORDER>
-struct KernelFnT<ORDER, double> {
+template <int ORDER, int D>
+struct KernelFnT<ORDER, double, D> {
     __device__ __forceinline__ static double value(double r2, double iw, double iw2)
     {
-        return KernelFn<ORDER>::value(r2, iw, iw2);
+        return KernelFnD<D, ORDER>::value(r2, iw, iw2);
     }
     __device__ __forceinline__ static void eval(double r2, double iw, double iw2, double &k, double &c)
     {
-        KernelFn<ORDER>::eval(r2, iw, iw2, k, c);
+        KernelFnD<D, ORDER>::eval(r2, iw, iw2, k, c);
     }
 };
 
@@ -67,8 +68,24 @@ struct KernelFnT<ORDER, double> {
 // coefficients divided by their leading constant) so the compiler hoists
 // them out of the pair loop: the Gaussian costs one multiply and one
 // v_exp_f32 per pair for k, one more multiply for c.
+// 2D Matern under the FP32 policy: the Bessel factors and the polynomial in
+// FP64 from the FP32 squared distance, rounded to FP32.
 template <int ORDER>
-struct KernelFnT<ORDER, float> {
+struct KernelFnT<ORDER, float, 2> {
+    __device__ __forceinline__ static float value(float r2, float iw, float iw2)
+    {
+        return (float)KernelFnD<2, ORDER>::value((double)r2, (double)iw, (double)iw2);
+    }
+    __device__ __forceinline__ static void eval(float r2, float iw, float iw2, float &k, float &c)
+    {
+        double kd, cd;
+        KernelFnD<2, ORDER>::eval((double)r2, (double)iw, (double)iw2, kd, cd);
+        k = (float)kd;
+        c = (float)cd;
+    }
+};
+template <int ORDER>
+struct KernelFnT<ORDER, float, 3> {
     static constexpr float kLog2e = 1.4426950408889634f;
     __device__ __forceinline__ static float value(float r2, float iw, float iw2)
     {
@@ -114,6 +131,11 @@ struct KernelFnT<ORDER, float> {
             }
         }
     }
+};
+
+// the Gaussian is the same function in 2D and 3D
+template <>
+struct KernelFnT<-1, float, 2> : KernelFnT<-1, float, 3> {
 };
 
 constexpr int kWaves = kBlock / 64;

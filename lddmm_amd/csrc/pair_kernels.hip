@@ -36,7 +36,7 @@ __global__ __launch_bounds__(kBlock) void k_dense(const double *__restrict__ qi,
             double df = x[c] - qj[(size_t)c * nj + j];
             r2 += df * df;
         }
-        K[(size_t)j * ni + i] = KernelFn<ORDER>::value(r2, inv_w, inv_w2);
+        K[(size_t)j * ni + i] = KernelFnD<D, ORDER>::value(r2, inv_w, inv_w2);
     }
 }
 
@@ -126,8 +126,6 @@ void check_dim_order(int dim, int order)
 {
     if (dim != 2 && dim != 3) throw Error(kErrArg, "dimension must be 2 or 3");
     if (order < -1 || order > 4) throw Error(kErrArg, "knlOrder must be -1 (Gaussian) or 0..4 (Matern)");
-    if (dim == 2 && order >= 0)
-        throw Error(kErrUnsupported, "2D Matern kernels (Bessel K0/K1) are not implemented on the GPU path");
 }
 
 // f(Dim, Order, Prec)

@@ -56,7 +56,7 @@ struct OpKV {
             real df = (real)(t.x[c] - xj[c]);
             r2 += df * df;
         }
-        real k = KernelFnT<ORDER, real>::value(r2, A.iw, A.iw2);
+        real k = KernelFnT<ORDER, real, D>::value(r2, A.iw, A.iw2);
 #pragma unroll
         for (int c = 0; c < D; ++c) acc.v[c] += k * fj[c];
     }
@@ -128,7 +128,7 @@ struct OpBwd {
             r2 += df[c] * df[c];
         }
         real k, cc;
-        KernelFnT<ORDER, real>::eval(r2, A.iw, A.iw2, k, cc);
+        KernelFnT<ORDER, real, D>::eval(r2, A.iw, A.iw2, k, cc);
         real lr = real(0);
 #pragma unroll
         for (int c = 0; c < D; ++c) lr += t.l[c] * fj[c];
@@ -206,7 +206,7 @@ struct OpDqSelf {
             r2 += df[c] * df[c];
         }
         real k, cc;
-        KernelFnT<ORDER, real>::eval(r2, A.iw, A.iw2, k, cc);
+        KernelFnT<ORDER, real, D>::eval(r2, A.iw, A.iw2, k, cc);
         real lr = real(0);
 #pragma unroll
         for (int c = 0; c < D; ++c) lr += t.l[c] * fj[D + c];
@@ -275,7 +275,7 @@ struct OpDqCross {
             r2 += df[c] * df[c];
         }
         real k, cc;
-        KernelFnT<ORDER, real>::eval(r2, A.iw, A.iw2, k, cc);
+        KernelFnT<ORDER, real, D>::eval(r2, A.iw, A.iw2, k, cc);
         real lr = real(0);
 #pragma unroll
         for (int c = 0; c < D; ++c) lr += t.u[c] * fj[c];

@@ -22,7 +22,7 @@ def pts(seed, n, d, scale=0.5):
     return (scale * G.normal(seed, n * d)).reshape(n, d)
 
 
-CASES = [(2, -1)] + [(3, o) for o in (-1, 0, 1, 2, 3, 4)]
+CASES = [(d, o) for d in (2, 3) for o in (-1, 0, 1, 2, 3, 4)]
 
 
 @pytest.mark.parametrize("dim,order", CASES)
@@ -56,7 +56,7 @@ def test_dq_kernel(gpu, dim, order, n):
         assert rel_err(got, exp) <= TOL_OP
 
 
-@pytest.mark.parametrize("dim,order", [(2, -1), (3, -1), (3, 1), (3, 4)])
+@pytest.mark.parametrize("dim,order", [(2, -1), (2, 2), (2, 4), (3, -1), (3, 1), (3, 4)])
 def test_dq_kernel_cross(gpu, dim, order):
     qi, qj = pts(9, 250, dim), pts(10, 333, dim)
     l, r = pts(11, 250, dim, 1.0), pts(12, 333, dim, 1.0)
@@ -66,7 +66,7 @@ def test_dq_kernel_cross(gpu, dim, order):
     assert rel_err(gi, ei) <= TOL_OP and rel_err(gj, ej) <= TOL_OP
 
 
-@pytest.mark.parametrize("dim,order", [(2, -1), (3, -1), (3, 2)])
+@pytest.mark.parametrize("dim,order", [(2, -1), (2, 0), (2, 3), (3, -1), (3, 2)])
 def test_compute_kernel(gpu, dim, order):
     q, qj = pts(13, 90, dim), pts(14, 41, dim)
     print("compute_kernel dim=%d order=%d" % (dim, order), flush=True)
@@ -86,11 +86,24 @@ def test_golden_kernels_on_gpu(gpu, golden_kernels):
             assert rel_err(gpu.dqKernel(q, l, a, order, w), g[t + "_dq"]) <= TOL_OP
 
 
-def test_unsupported_2d_matern_fails_loudly(gpu):
-    q = pts(1, 10, 2)
-    with pytest.raises(gpu.LddmmError) as ei:
-        gpu.multiplyKernel(q, q, 1, 0.3)
-    assert ei.value.code == -4
+@pytest.mark.parametrize("order", [0, 1, 2, 3, 4])
+def test_2d_matern_over_the_whole_distance_range(gpu, order):
+    """2D Matern kernel values at distances d = |q_i - q_j|/sigma from 0 and
+    below machine epsilon through both sides of the series / continued
+    fraction switch (d = 2) to past the cut-off (d = 700), against the oracle
+    (pinned to the reference's polybesselk.cpp and scipy's k0/k1)."""
+    w = 0.3
+    d = np.concatenate([[0.0, 1e-17, 1e-12, 1e-6], np.logspace(-4, 2.9, 200), [1.999999, 2.0, 2.000001],
+                        [650.0, 699.9, 700.0, 700.1, 800.0]])
+    qi = np.zeros((1, 2))
+    qj = np.stack([d * w, np.zeros_like(d)], axis=1)
+    got = gpu.computeKernel(qi, qj, order, w)
+    exp = oracle.compute_kernel_cross(qi, qj, order, w)
+    print("2d matern order %d: max rel %.2e" % (order, np.max(np.abs(got - exp) / np.maximum(np.abs(exp), 1e-300))),
+          flush=True)
+    # relative agreement down to the subnormal range (e^-700 ~ 1e-304 at the cut-off)
+    assert np.all(np.abs(got - exp) <= 1e-13 * np.abs(exp) + 1e-300)
+    assert np.all(got[d > 700.0] == 0.0) and got[0] == exp[0]
 
 
 @pytest.mark.parametrize("dim", (2, 3))
@@ -125,9 +138,11 @@ def test_varifold_larger_and_without_target(gpu):
 
 
 def make_problem(kind):
-    if kind == "circle_square":
+    if kind.startswith("circle_square"):
         w = G.circle_square(200)
         w["q0"] = w["q0"] * 0.95
+        if kind == "circle_square_matern":
+            w["knl_order"] = 2   # 2D Matern nu = 3 (Bessel factors)
     elif kind == "sphere_cube":
         w = G.sphere_cube(subdivisions=2, cells=10)
     else:
@@ -137,7 +152,7 @@ def make_problem(kind):
     return w
 
 
-@pytest.mark.parametrize("kind", ["circle_square", "sphere_cube", "superquadric"])
+@pytest.mark.parametrize("kind", ["circle_square", "circle_square_matern", "sphere_cube", "superquadric"])
 def test_objgrd_matches_oracle(gpu, kind):
     w = make_problem(kind)
     n, d = w["q0"].shape
@@ -189,7 +204,7 @@ def test_compute_node_position_matches_trajectory(gpu):
         assert rel_err(nde[t + 1], q) <= 1e-12
 
 
-@pytest.mark.parametrize("kind", ["circle_square", "sphere_cube", "superquadric"])
+@pytest.mark.parametrize("kind", ["circle_square", "circle_square_matern", "sphere_cube", "superquadric"])
 def test_objgrd_fp32_policy_error(gpu, kind):
     """FP32 pair arithmetic (FP64 differences and cross-tile sums) against
     the FP64 oracle: relative error of the energy and of the gradient
