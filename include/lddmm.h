@@ -179,6 +179,20 @@ int lddmm_varifold(double *vfdVal, double *dqVfdMat, const double *dfmLmkPosMat,
                    char cenKnlType, double cenKnlWidth, char dirKnlType, double dirKnlWidth,
                    int dfmLmkNum, int dfmElmNum, int dfmElmIfoNum, int tgtElmNum, int dimNum);
 
+/* ---------------------------------------------------------------- l2 landmark matching
+ * The MATLAB variant's alternative matching term l2Dist (MATLAB/src/l2Dist.m:1-5,
+ * used as disHdl by MATLAB/src/objgrd.m:29-34): E = (h/2) sum_t <alpha_t, K alpha_t>
+ * + (w/2) |q_{T-1} - tgt|^2 for landmark targets with known correspondence.
+ * Host layout (D x N). */
+int lddmm_l2_dist(double *fcnVal, double *grdMat, const double *ndeMat, const double *tgtMat, double wgtVal,
+                  int ndeNum, int dimNum);
+int lddmm_objective_create_l2(lddmm_objective **out, int dimNum, const double *iniNdeMat, const double *tgtNdeMat,
+                              double disWgt, int knlOrder, double knlWidth, double timeStp, int timeNum, int ndeNum);
+int lddmm_run_l2(int dimNum, const double *iniNdeMat, const double *iniAlpVec, const double *tgtNdeMat,
+                 double disWgt, int knlOrder, double knlWidth, double timeStp, int timeNum, int itrMax, double tolVal,
+                 double wolfe1, double wolfe2, int memNum, int vbsFlg, int ndeNum, double *optAlpVec,
+                 const lddmm_options *opts, lddmm_result *result);
+
 /* ---------------------------------------------------------------- batches
  * Many independent registrations of uneven sizes in one set of launches
  * (no counterpart in the reference, which registers one pair per MEX call;

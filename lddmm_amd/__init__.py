@@ -20,7 +20,7 @@ import os
 import numpy as np
 
 __all__ = ["lib", "LDDMM", "getElmIfo", "computeNodePosition", "elementVarifold", "multiplyKernel", "dqKernel", "computeKernel",
-           "varifold", "Objective", "run", "LddmmError", "profile_enable", "profile_read", "profile_reset"]
+           "varifold", "Objective", "run", "l2Dist", "run_l2", "LddmmError", "profile_enable", "profile_read", "profile_reset"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblddmm_amd.so")
@@ -244,6 +244,47 @@ def computeNodePosition(iniNdeMat, alpStk, knlOrder, knlWidth, timeStp, timeNum)
     return nde, vlc
 
 
+def l2Dist(ndeMat, tgtMat, wgtVal):
+    """[fcnVal, grdMat] = l2Dist(ndeMat, tgtMat, wgtVal) of MATLAB/src/l2Dist.m:1-5
+    on the GPU: w/2 |q - tgt|^2 and w (q - tgt); arrays (N, D)."""
+    L = lib()
+    if not hasattr(L.lddmm_l2_dist, "_set"):
+        L.lddmm_l2_dist.argtypes = [_dp, _dp, _dp, _dp, C.c_double, C.c_int, C.c_int]
+        L.lddmm_objective_create_l2.argtypes = [C.POINTER(C.c_void_p), C.c_int, _dp, _dp, C.c_double, C.c_int,
+                                                C.c_double, C.c_double, C.c_int, C.c_int]
+        L.lddmm_run_l2.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_int, C.c_double, C.c_double, C.c_int,
+                                   C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, _dp,
+                                   C.POINTER(Options), C.POINTER(Result)]
+        L.lddmm_l2_dist._set = True
+    q, t = _f64(ndeMat), _f64(tgtMat)
+    if q.shape != t.shape:
+        raise ValueError("ndeMat and tgtMat must have the same shape")
+    f = C.c_double()
+    g = np.empty_like(q)
+    _check(L.lddmm_l2_dist(C.byref(f), _p(g), _p(q), _p(t), float(wgtVal), q.shape[0], q.shape[1]))
+    return f.value, g
+
+
+def run_l2(q0, alp0, tgt, dis_wgt, knl_order, knl_width, time_stp, time_num, itr_max, tol, wolfe1, wolfe2, mem,
+           verbose=False, precision="fp64", rel_energy_tol=0.0):
+    """Landmark registration with the l2Dist matching term (MATLAB/src/LDDMM.m
+    with disHdl = @(q) l2Dist(q, tgt, w)).  Returns (alpha (T-1, N, D), info)."""
+    l2Dist(np.zeros((1, 2)), np.zeros((1, 2)), 0.0)   # binds the signatures
+    q0, tgt = _f64(q0), _f64(tgt)
+    n, d = q0.shape
+    T = int(time_num)
+    a0 = np.zeros(d * n * (T - 1)) if alp0 is None else _f64(alp0).reshape(-1)
+    out = np.empty(d * n * (T - 1))
+    opts = Options(PRECISIONS[precision], 0, rel_energy_tol, 0.0, 0.0, 0, 0, 0)
+    res = Result()
+    st = _check(lib().lddmm_run_l2(d, _p(q0), _p(a0), _p(tgt), dis_wgt, knl_order, knl_width, time_stp, T, itr_max,
+                                   tol, wolfe1, wolfe2, mem, int(bool(verbose)), n, _p(out), C.byref(opts),
+                                   C.byref(res)), allow_positive=True)
+    info = dict(fcn=res.fcn, grad_norm=res.grad_norm, seconds=res.seconds, iterations=res.iterations,
+                evaluations=res.evaluations, status=st)
+    return out.reshape(T - 1, n, d), info
+
+
 # ---------------------------------------------------------------- objective
 class Objective:
     """objgrd (Cpp/src/objgrd.cpp:26-90) on the GPU for a fixed problem.
@@ -264,6 +305,22 @@ class Objective:
         self._h = h
         if precision != "fp64":
             self.set_precision(precision)
+
+    @classmethod
+    def l2(cls, q0, tgt, dis_wgt, knl_order, knl_width, time_stp, time_num):
+        """The objective with the l2Dist landmark matching term
+        (MATLAB/src/objgrd.m with disHdl = l2Dist) against targets tgt (N, D)."""
+        l2Dist(np.zeros((1, 2)), np.zeros((1, 2)), 0.0)   # binds the signatures
+        self = cls.__new__(cls)
+        self.q0, self.tl = _f64(q0), _f64(tgt)
+        self.n, self.dim = self.q0.shape
+        self.T = int(time_num)
+        self.varnum = self.dim * self.n * (self.T - 1)
+        h = C.c_void_p()
+        _check(lib().lddmm_objective_create_l2(C.byref(h), self.dim, _p(self.q0), _p(self.tl), dis_wgt, knl_order,
+                                               knl_width, time_stp, self.T, self.n))
+        self._h = h
+        return self
 
     def set_precision(self, precision):
         """'fp64' (the reference's arithmetic) or 'fp32' (FP32 pair terms,

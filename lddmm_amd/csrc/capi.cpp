@@ -116,10 +116,37 @@ void from_soa(T *dst, const T *soa, int n, int d, int slices = 1)
 
 // Host-side problem arrays converted to the device layout and uploaded.
 struct UploadedProblem {
-    DevBuf<double> q0, tc, tu, ta;
+    DevBuf<double> q0, tc, tu, ta, tl;
     DevBuf<int> elm, ifo;
     DeviceProblem p;
 };
+
+// l2 landmark matching problem: source and target landmarks (host layout)
+void upload_l2_problem(UploadedProblem &u, int dim, int n, int T, int order, double width, double h, double w,
+                       const double *q0, const double *tgt, hipStream_t st)
+{
+    require(dim == 2 || dim == 3, "dimNum must be 2 or 3");
+    require(n > 0, "ndeNum must be positive");
+    require(T >= 2, "timeNum must be at least 2");
+    require(q0 && tgt, "null landmark array");
+    DeviceProblem &p = u.p;
+    p.dim = dim;
+    p.n = n;
+    p.T = T;
+    p.order = order;
+    p.width = width;
+    p.h = h;
+    p.w = w;
+    p.ne = p.nt = 0;
+    auto hq = to_soa(q0, n, dim), ht = to_soa(tgt, n, dim);
+    u.q0.alloc(hq.size());
+    u.tl.alloc(ht.size());
+    u.q0.upload(hq.data(), st);
+    u.tl.upload(ht.data(), st);
+    p.q0 = u.q0.p;
+    p.tl = u.tl.p;
+    validate(p);
+}
 
 void check_common(int dim, int n, int ne, int ifo_rows, int nt, int T)
 {
@@ -458,6 +485,79 @@ int lddmm_objective_target_term(lddmm_objective *obj, double *value)
 }
 
 void lddmm_objective_destroy(lddmm_objective *obj) { delete obj; }
+
+// ---------------------------------------------------------------- l2 landmark matching
+int lddmm_l2_dist(double *fcnVal, double *grdMat, const double *ndeMat, const double *tgtMat, double wgtVal,
+                  int ndeNum, int dimNum)
+{
+    return guarded([&] {
+        require(fcnVal && grdMat && ndeMat && tgtMat, "null argument");
+        require(dimNum == 2 || dimNum == 3, "dimNum must be 2 or 3");
+        require(ndeNum > 0, "ndeNum must be positive");
+        Stream st;
+        auto hq = to_soa(ndeMat, ndeNum, dimNum), ht = to_soa(tgtMat, ndeNum, dimNum);
+        const int nb = l2_match_blocks(ndeNum);
+        DevBuf<double> q(hq.size()), t(ht.size()), p(hq.size()), parts(nb), zero(1), f(1), e(1);
+        q.upload(hq.data(), st.s);
+        t.upload(ht.data(), st.s);
+        const double z = 0.0;
+        zero.upload(&z, st.s);
+        e.upload(&z, st.s);
+        launch_l2_match(dimNum, q.p, t.p, ndeNum, wgtVal, p.p, parts.p, st.s);
+        // energy kernel with no flow partials: f = w * (sum parts + 0)
+        launch_energy(e.p, 0, parts.p, nb, zero.p, 0.0, wgtVal, f.p, st.s);
+        p.download(hq.data(), st.s);
+        f.download(fcnVal, st.s);
+        st.sync();
+        from_soa(grdMat, hq.data(), ndeNum, dimNum);
+        for (size_t k = 0; k < hq.size(); ++k) grdMat[k] = -grdMat[k];   // p = -grad
+        return 0;
+    });
+}
+
+int lddmm_objective_create_l2(lddmm_objective **out, int dimNum, const double *iniNdeMat, const double *tgtNdeMat,
+                              double disWgt, int knlOrder, double knlWidth, double timeStp, int timeNum, int ndeNum)
+{
+    return guarded([&] {
+        require(out != nullptr, "null handle pointer");
+        *out = nullptr;
+        auto h = std::make_unique<lddmm_objective>();
+        upload_l2_problem(h->u, dimNum, ndeNum, timeNum, knlOrder, knlWidth, timeStp, disWgt, iniNdeMat, tgtNdeMat,
+                          h->st.s);
+        h->obj = std::make_unique<Objective>(h->u.p, h->st.s);
+        h->alp.alloc(h->u.p.varnum());
+        h->grd.alloc(h->u.p.varnum());
+        h->f.alloc(1);
+        *out = h.release();
+        return 0;
+    });
+}
+
+int lddmm_run_l2(int dimNum, const double *iniNdeMat, const double *iniAlpVec, const double *tgtNdeMat,
+                 double disWgt, int knlOrder, double knlWidth, double timeStp, int timeNum, int itrMax, double tolVal,
+                 double wolfe1, double wolfe2, int memNum, int vbsFlg, int ndeNum, double *optAlpVec,
+                 const lddmm_options *opts, lddmm_result *result)
+{
+    return guarded([&] {
+        require(iniAlpVec && optAlpVec, "null momentum array");
+        LbfgsOptions lo = make_opts(itrMax, tolVal, wolfe1, wolfe2, memNum, vbsFlg, opts);
+        Stream st;
+        UploadedProblem u;
+        u.p.prec = precision_of(opts);
+        upload_l2_problem(u, dimNum, ndeNum, timeNum, knlOrder, knlWidth, timeStp, disWgt, iniNdeMat, tgtNdeMat,
+                          st.s);
+        Objective obj(u.p, st.s, opts ? opts->pair_target_blocks : 0);
+        auto hx = to_soa(iniAlpVec, ndeNum, dimNum, timeNum - 1);
+        DevBuf<double> x(hx.size());
+        x.upload(hx.data(), st.s);
+        LbfgsResult lr = run_lbfgs(obj, x.p, lo, st.s);
+        x.download(hx.data(), st.s);
+        st.sync();
+        from_soa(optAlpVec, hx.data(), ndeNum, dimNum, timeNum - 1);
+        fill_result(result, lr, obj);
+        return lr.status;
+    });
+}
 
 // ---------------------------------------------------------------- batches
 struct lddmm_batch {

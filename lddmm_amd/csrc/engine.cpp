@@ -125,6 +125,11 @@ void validate(const DeviceProblem &p)
     if (p.T < 2) throw Error(kErrArg, "timeNum must be at least 2");
     if (p.order < -1 || p.order > 4) throw Error(kErrArg, "knlOrder must be -1 or 0..4");
     if (!(p.width > 0.0)) throw Error(kErrArg, "knlWidth must be positive");
+    if ((long long)p.dim * p.n * (p.T - 1) >= (1LL << 40)) throw Error(kErrArg, "problem too large");
+    if (p.tl) {
+        if (p.ne != 0 || p.nt != 0) throw Error(kErrArg, "l2 landmark matching takes no elements or target varifold");
+        return;
+    }
     if (p.ne <= 0 || p.nt < 0) throw Error(kErrArg, "element counts must be positive");
     if (p.ifo_rows < 1) throw Error(kErrArg, "dfmElmIfoNum must be at least 1");
     if (p.vp.cen_type != 'G' && p.vp.cen_type != 'C') throw Error(kErrArg, "cenKnlType must be 'G' or 'C'");
@@ -151,7 +156,7 @@ Objective::Objective(const DeviceProblem &p, hipStream_t st, int target_blocks) 
     sp_kv_ = plan_kv(p_.n, p_.n, p_.prec, tb);
     const PairSplit kv_other = plan_kv(p_.n, p_.n, p_.prec == kFP32 ? kFP64 : kFP32, tb);
     sp_bwd_ = sp_kv_;
-    const double fd = (double)p_.ne / (double)(p_.ne + p_.nt);
+    const double fd = p_.ne + p_.nt > 0 ? (double)p_.ne / (double)(p_.ne + p_.nt) : 1.0;
     int tbd = (int)std::max(1.0, tb * fd), tbt = std::max(1, tb - tbd);
     sp_vfd_ = plan_vf(p_.ne, p_.ne, tbd);
     sp_vft_ = plan_vf(p_.ne, std::max(p_.nt, 1), tbt);
@@ -162,6 +167,7 @@ Objective::Objective(const DeviceProblem &p, hipStream_t st, int target_blocks) 
     part = std::max(part, (size_t)sp_tt.splits * std::max(p_.nt, 1));
     nb_fwd_ = fwd_reduce_blocks(p_.n);
     nb_vf_ = vf_finalize_blocks(p_.ne);
+    nb_l2_ = p_.tl ? l2_match_blocks(p_.n) : 0;
     const int nb_tt = vf_value_finalize_blocks(std::max(p_.nt, 1));
 
     slab_.reserve(&nde_, (size_t)D * p_.n * p_.T);
@@ -173,7 +179,7 @@ Objective::Objective(const DeviceProblem &p, hipStream_t st, int target_blocks) 
     slab_.reserve(&dc_, (size_t)D * p_.ne);
     slab_.reserve(&dd_, (size_t)D * p_.ne);
     slab_.reserve(&ep_fwd_, (size_t)(p_.T - 1) * nb_fwd_);
-    slab_.reserve(&ep_vf_, (size_t)std::max(nb_vf_, nb_tt));
+    slab_.reserve(&ep_vf_, (size_t)std::max(std::max(nb_vf_, nb_tt), nb_l2_));
     slab_.reserve(&tt_, 1);
     slab_.reserve(&scratch_, (size_t)kDotBlocks);
     slab_.allocate();
@@ -213,7 +219,7 @@ void Objective::compute_target_target(int target_blocks)
 double Objective::pairs_per_eval() const
 {
     const double n = p_.n, T1 = p_.T - 1;
-    return 2.0 * T1 * n * n + (double)p_.ne * p_.ne + (double)p_.ne * p_.nt;
+    return 2.0 * T1 * n * n + (double)p_.ne * p_.ne + (double)p_.ne * p_.nt;   // l2 matching: ne = nt = 0
 }
 
 void Objective::eval(const double *alp, double *grd, double *f_dev)
@@ -238,29 +244,41 @@ void Objective::eval(const double *alp, double *grd, double *f_dev)
         }
     }
 
-    // varifold at q_{T-1}; adjoint start p = -w grad V (Cpp/src/objgrd.cpp:61-67)
     const double *qT = nde_ + (size_t)(T - 1) * dn;
-    {
-        ProfScope ps(kProfVfGeometry, st_);
-        launch_elm_geometry(D, qT, n, p_.elm, p_.ne, ecen_, edir_, evol_, st_);
-    }
-    {
-        ProfScope ps(kProfVfPairs, st_);
-        launch_vf_pairs(D, p_.prec, p_.vp, ecen_, edir_, evol_, p_.ne, p_.tc, p_.tu, p_.ta, p_.nt, sp_vfd_,
-                        sp_vft_, part_, st_);
-    }
-    {
-        ProfScope ps(kProfVfFinalize, st_);
-        launch_vf_finalize(D, p_.vp, sp_vfd_.splits, sp_vfd_.splits + (p_.nt > 0 ? sp_vft_.splits : 0), p_.ne,
-                           part_, edir_, evol_, dc_, dd_, ep_vf_, st_);
-    }
-    {
-        ProfScope ps(kProfVfGather, st_);
-        launch_vf_gather(D, qT, n, p_.ifo, p_.ifo_rows, dc_, dd_, p_.ne, -p_.w, p_adj_, st_);
-    }
-    {
-        ProfScope ps(kProfEnergy, st_);
-        launch_energy(ep_fwd_, (T - 1) * nb_fwd_, ep_vf_, nb_vf_, tt_, p_.h, p_.w, f_dev, st_);
+    if (p_.tl) {
+        // l2Dist at q_{T-1} (MATLAB/src/objgrd.m:29-34 with disHdl = l2Dist)
+        {
+            ProfScope ps(kProfVfFinalize, st_);
+            launch_l2_match(D, qT, p_.tl, n, p_.w, p_adj_, ep_vf_, st_);
+        }
+        {
+            ProfScope ps(kProfEnergy, st_);
+            launch_energy(ep_fwd_, (T - 1) * nb_fwd_, ep_vf_, nb_l2_, tt_, p_.h, p_.w, f_dev, st_);
+        }
+    } else {
+        // varifold at q_{T-1}; adjoint start p = -w grad V (Cpp/src/objgrd.cpp:61-67)
+        {
+            ProfScope ps(kProfVfGeometry, st_);
+            launch_elm_geometry(D, qT, n, p_.elm, p_.ne, ecen_, edir_, evol_, st_);
+        }
+        {
+            ProfScope ps(kProfVfPairs, st_);
+            launch_vf_pairs(D, p_.prec, p_.vp, ecen_, edir_, evol_, p_.ne, p_.tc, p_.tu, p_.ta, p_.nt, sp_vfd_,
+                            sp_vft_, part_, st_);
+        }
+        {
+            ProfScope ps(kProfVfFinalize, st_);
+            launch_vf_finalize(D, p_.vp, sp_vfd_.splits, sp_vfd_.splits + (p_.nt > 0 ? sp_vft_.splits : 0), p_.ne,
+                               part_, edir_, evol_, dc_, dd_, ep_vf_, st_);
+        }
+        {
+            ProfScope ps(kProfVfGather, st_);
+            launch_vf_gather(D, qT, n, p_.ifo, p_.ifo_rows, dc_, dd_, p_.ne, -p_.w, p_adj_, st_);
+        }
+        {
+            ProfScope ps(kProfEnergy, st_);
+            launch_energy(ep_fwd_, (T - 1) * nb_fwd_, ep_vf_, nb_vf_, tt_, p_.h, p_.w, f_dev, st_);
+        }
     }
 
     // backward adjoint sweep (Cpp/src/objgrd.cpp:69-87)

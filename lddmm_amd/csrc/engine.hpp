@@ -112,6 +112,9 @@ struct DeviceProblem {
     const int *elm = nullptr;     // Ed x V
     const int *ifo = nullptr;     // N x rows
     const double *tc = nullptr, *tu = nullptr, *ta = nullptr;  // Et x D, Et x D, Et
+    // landmark targets N x D: when set, the matching term is l2Dist
+    // (MATLAB/src/l2Dist.m:1-5) and the element / varifold fields are unused
+    const double *tl = nullptr;
     long long varnum() const { return (long long)dim * n * (T - 1); }
 };
 
@@ -144,7 +147,7 @@ private:
     double *nde_ = nullptr, *p_adj_ = nullptr, *part_ = nullptr;
     double *ecen_ = nullptr, *edir_ = nullptr, *evol_ = nullptr, *dc_ = nullptr, *dd_ = nullptr;
     double *ep_fwd_ = nullptr, *ep_vf_ = nullptr, *tt_ = nullptr, *scratch_ = nullptr;
-    int nb_fwd_ = 0, nb_vf_ = 0, target_blocks_ = 0;
+    int nb_fwd_ = 0, nb_vf_ = 0, nb_l2_ = 0, target_blocks_ = 0;
     double tt_host_ = 0.0;
     void compute_target_target(int target_blocks);
 };

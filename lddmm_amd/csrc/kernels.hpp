@@ -81,6 +81,10 @@ void launch_sum(const double *x, long long n, double *scratch, double *out, hipS
 void launch_dots(int nd, const double *const *xs, const double *const *ys, long long n, double *scratch,
                  double *out, hipStream_t st);
 // Energy: out = 0.5*h*sum(fwd parts) + w*(sum(vf parts) + tt[0])
+// l2Dist end point: p = -w (q - tl), parts[b] = per-block |q - tl|^2 / 2
+int l2_match_blocks(int n);
+void launch_l2_match(int dim, const double *q, const double *tl, int n, double w, double *p, double *parts,
+                     hipStream_t st);
 void launch_energy(const double *fwd_parts, int nf, const double *vf_parts, int nv, const double *tt, double h,
                    double w, double *out, hipStream_t st);
 

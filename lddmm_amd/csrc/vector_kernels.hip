@@ -83,6 +83,29 @@ __global__ __launch_bounds__(kBlock) void k_energy(const double *__restrict__ fp
     if (threadIdx.x == 0) out[0] = h * 0.5 * ef + w * (ev + tt[0]);
 }
 
+// l2Dist end point (MATLAB/src/l2Dist.m:1-5): p = -w (q - tl) per vertex,
+// per-block partials of |q - tl|^2 / 2 for the energy kernel (which
+// multiplies them by w)
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_l2_match(const double *__restrict__ q, const double *__restrict__ tl,
+                                                     int n, double w, double *__restrict__ p,
+                                                     double *__restrict__ parts)
+{
+    __shared__ double red[kBlock];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    double s = 0.0;
+    if (i < n) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            const double d = q[(size_t)c * n + i] - tl[(size_t)c * n + i];
+            p[(size_t)c * n + i] = -w * d;
+            s += d * d;
+        }
+    }
+    const double t = block_sum(s, red);
+    if (threadIdx.x == 0) parts[blockIdx.x] = 0.5 * t;
+}
+
 // ---------------------------------------------------------------- L-BFGS
 __global__ void k_copy_neg(double *__restrict__ d, const double *__restrict__ g, long long n)
 {
@@ -182,6 +205,16 @@ void launch_energy(const double *fwd_parts, int nf, const double *vf_parts, int 
                    double w, double *out, hipStream_t st)
 {
     k_energy<<<1, kBlock, 0, st>>>(fwd_parts, nf, vf_parts, nv, tt, h, w, out);
+    LDDMM_LAUNCH_CHECK();
+}
+
+int l2_match_blocks(int n) { return ceil_div(n, kBlock); }
+
+void launch_l2_match(int dim, const double *q, const double *tl, int n, double w, double *p, double *parts,
+                     hipStream_t st)
+{
+    if (dim == 2) k_l2_match<2><<<l2_match_blocks(n), kBlock, 0, st>>>(q, tl, n, w, p, parts);
+    else k_l2_match<3><<<l2_match_blocks(n), kBlock, 0, st>>>(q, tl, n, w, p, parts);
     LDDMM_LAUNCH_CHECK();
 }
 

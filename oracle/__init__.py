@@ -33,7 +33,7 @@ class Problem(C.Structure):
                 ("elmNum", C.c_int), ("ifoRows", C.c_int), ("tgtNum", C.c_int),
                 ("tgtCen", C.c_void_p), ("tgtDir", C.c_void_p), ("tgtVol", C.c_void_p),
                 ("cenType", C.c_char), ("dirType", C.c_char),
-                ("cenWidth", C.c_double), ("dirWidth", C.c_double)]
+                ("cenWidth", C.c_double), ("dirWidth", C.c_double), ("tgtNde", C.c_void_p)]
 
 
 class LbfgsOpts(C.Structure):
@@ -77,6 +77,8 @@ def lib():
                                    C.c_double, C.c_char, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_get_elm_ifo.argtypes = [C.c_void_p, _ip, C.c_int, C.c_int, C.c_int]
         L.orc_objgrd.argtypes = [C.POINTER(C.c_double), _dp, _dp, C.POINTER(Problem), C.c_void_p]
+        L.orc_l2_dist.argtypes = [C.POINTER(C.c_double), _dp, _dp, _dp, C.c_double, C.c_int, C.c_int]
+        L.orc_l2_dist.restype = None
         L.orc_get_direction.argtypes = [_dp, C.c_double, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, _dp, C.c_size_t]
         L.orc_get_direction.restype = None
         L.orc_line_search.argtypes = [OBJFUN, C.c_void_p, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double,
@@ -206,6 +208,15 @@ def varifold_terms(q, elm, tc, tu, ta, cen_type, cen_w, dir_type, dir_w):
     return dd.value, dt.value, tt.value
 
 
+def l2_dist(q, tgt, w):
+    """l2Dist (MATLAB/src/l2Dist.m:1-5): (w/2 |q - tgt|^2, w (q - tgt))."""
+    q, tgt = _f64(q), _f64(tgt)
+    f = C.c_double()
+    g = np.empty_like(q)
+    lib().orc_l2_dist(C.byref(f), g, q, tgt, C.c_double(w), q.shape[0], q.shape[1])
+    return f.value, g
+
+
 def varifold(q, elm, ifo, tc, tu, ta, cen_type, cen_w, dir_type, dir_w, grad=True, L=None, prefix="orc"):
     q, elm, ifo = _f64(q), _i32(elm), _i32(ifo)
     val = C.c_double()
@@ -219,10 +230,13 @@ def varifold(q, elm, ifo, tc, tu, ta, cen_type, cen_w, dir_type, dir_w, grad=Tru
 
 # ---------------------------------------------------------------- objective
 class OracleProblem:
-    """Keeps the numpy arrays of an orc_problem alive."""
+    """Keeps the numpy arrays of an orc_problem alive.  With tgt_nde (N, D)
+    the matching term is l2Dist against those landmarks (the element and
+    target varifold arguments are then ignored and may be empty)."""
 
     def __init__(self, q0, elm, tgt_cen, tgt_dir, tgt_vol, cen_type, cen_w, dir_type, dir_w, dis_wgt,
-                 knl_order, knl_width, time_stp, time_num, ifo=None):
+                 knl_order, knl_width, time_stp, time_num, ifo=None, tgt_nde=None):
+        self.tl = None if tgt_nde is None else _f64(tgt_nde)
         self.q0 = _f64(q0)
         self.elm = _i32(elm)
         self.ifo = _i32(ifo) if ifo is not None else get_elm_ifo(self.elm, self.q0.shape[0])
@@ -238,6 +252,7 @@ class OracleProblem:
         P.tgtCen, P.tgtDir, P.tgtVol = self.tc.ctypes.data, self.tu.ctypes.data, self.ta.ctypes.data
         P.cenType, P.dirType = cen_type.encode(), dir_type.encode()
         P.cenWidth, P.dirWidth = cen_w, dir_w
+        P.tgtNde = self.tl.ctypes.data if self.tl is not None else None
         self.P = P
 
     def objgrd(self, alp, want_traj=False):

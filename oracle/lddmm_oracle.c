@@ -548,6 +548,17 @@ int orc_get_elm_ifo(int *ifo, const int *elmVtx, int vpe, int ne, int nq)
  * Matrix-free: the kernel matrix is recomputed per product instead of the
  * reference's stored knlStk (Cpp/src/assignStructMemory.cpp:89-91); the
  * arithmetic of each product is the same.  ndeStk (D*N*T) may be NULL.    */
+void orc_l2_dist(double *fcn, double *grd, const double *q, const double *tgt, double w, int n, int dim)
+{
+	double s = 0.0;
+	for (size_t k = 0; k < (size_t)n * dim; ++k) {
+		double d = q[k] - tgt[k];
+		s += d * d;
+		grd[k] = w * d;
+	}
+	*fcn = w * 0.5 * s;
+}
+
 int orc_objgrd(double *fcn, double *grd, const double *alp, const orc_problem *P, double *ndeStkOut)
 {
 	int n = P->ndeNum, T = P->timeNum, dim = P->dim;
@@ -573,14 +584,22 @@ int orc_objgrd(double *fcn, double *grd, const double *alp, const orc_problem *P
 		e += s;
 		for (size_t k = 0; k < dn; ++k) qn[k] = qt[k] + h * vlc[k];
 	}
-	double vfd;
-	rc = orc_varifold(&vfd, dqv, nde + (size_t)(T - 1) * dn, P->elmVtx, P->elmIfo, P->tgtCen, P->tgtDir,
-	                  P->tgtVol, P->cenType, P->cenWidth, P->dirType, P->dirWidth, n, P->elmNum,
-	                  P->ifoRows, P->tgtNum, dim);
-	if (rc) goto done;
-	*fcn = h * 0.5 * e + P->disWgt * vfd;
-
-	for (size_t k = 0; k < dn; ++k) pm[k] = -P->disWgt * dqv[k];
+	if (P->tgtNde) {
+		/* MATLAB/src/objgrd.m:29-34 with disHdl = l2Dist: the handle's value and
+		 * gradient already carry the weight */
+		double l2;
+		orc_l2_dist(&l2, dqv, nde + (size_t)(T - 1) * dn, P->tgtNde, P->disWgt, n, dim);
+		*fcn = h * 0.5 * e + l2;
+		for (size_t k = 0; k < dn; ++k) pm[k] = -dqv[k];
+	} else {
+		double vfd;
+		rc = orc_varifold(&vfd, dqv, nde + (size_t)(T - 1) * dn, P->elmVtx, P->elmIfo, P->tgtCen, P->tgtDir,
+		                  P->tgtVol, P->cenType, P->cenWidth, P->dirType, P->dirWidth, n, P->elmNum,
+		                  P->ifoRows, P->tgtNum, dim);
+		if (rc) goto done;
+		*fcn = h * 0.5 * e + P->disWgt * vfd;
+		for (size_t k = 0; k < dn; ++k) pm[k] = -P->disWgt * dqv[k];
+	}
 	for (int t = T - 2; t >= 0; --t) {
 		const double *qt = nde + (size_t)t * dn, *at = alp + (size_t)t * dn;
 		double *gt = grd + (size_t)t * dn;

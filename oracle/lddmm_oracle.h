@@ -27,6 +27,8 @@ typedef struct {
 	const double *tgtCen, *tgtDir, *tgtVol; /* D x Et, D x Et, Et */
 	char cenType, dirType;
 	double cenWidth, dirWidth;
+	const double *tgtNde;    /* D x N landmark targets: when non-null the matching term is
+	                            l2Dist (MATLAB/src/l2Dist.m:1-5) instead of the varifold */
 } orc_problem;
 
 typedef struct {
@@ -69,6 +71,8 @@ int orc_varifold(double *val, double *dq, const double *q, const int *elmVtx, co
 int orc_get_elm_ifo(int *ifo, const int *elmVtx, int vpe, int ne, int nq);
 
 int orc_objgrd(double *fcn, double *grd, const double *alp, const orc_problem *P, double *ndeStkOut);
+/* l2Dist: fcn = w/2 |q - tgt|^2, grd = w (q - tgt) (MATLAB/src/l2Dist.m:1-5) */
+void orc_l2_dist(double *fcn, double *grd, const double *q, const double *tgt, double w, int n, int dim);
 
 void orc_get_direction(double *d, double H0, const double *g, const double *S, const double *Y,
                        int newIdx, int hisNum, int memNum, double *rec, size_t n);

@@ -224,3 +224,46 @@ def test_lddmm_run_decreases_energy():
     assert np.all(np.diff(tr["f"]) < 0)
     f2, _ = P.objgrd(alp)
     assert f2 == f
+
+
+def l2_problem(dim, order, n=11, T=5):
+    q = (0.4 * G.normal(61 + dim, n * dim)).reshape(n, dim)
+    t = q + (0.2 * G.normal(71 + dim, n * dim)).reshape(n, dim)
+    none = np.zeros((0, dim))
+    P = oracle.OracleProblem(q, np.zeros((0, dim), dtype=np.int32), none, none, np.zeros(0), "G", 0.3, "B", 0.0,
+                             7.0, order, 0.45, 1.0 / (T - 1), T, ifo=np.zeros((n, 1), dtype=np.int32), tgt_nde=t)
+    return P, q, t
+
+
+def test_l2_dist_closed_form():
+    q = G.normal(1, 12).reshape(4, 3)
+    t = G.normal(2, 12).reshape(4, 3)
+    f, g = oracle.l2_dist(q, t, 3.0)
+    assert abs(f - 1.5 * np.sum((q - t) ** 2)) <= 1e-15 * f
+    assert np.array_equal(g, 3.0 * (q - t))
+
+
+@pytest.mark.parametrize("dim,order", [(2, -1), (3, -1), (3, 2), (2, 1)])
+def test_l2_objective_matches_finite_differences(dim, order):
+    """The objective with the l2Dist matching term (MATLAB/src/objgrd.m with
+    disHdl = l2Dist): its discrete adjoint is the exact gradient."""
+    P, q, t = l2_problem(dim, order)
+    a = 0.3 * G.normal(81, P.varnum)
+    f, g = P.objgrd(a)
+    for k in range(3):
+        u = G.normal(90 + k, P.varnum)
+        eps = 1e-6
+        fp, _ = P.objgrd(a + eps * u)
+        fm, _ = P.objgrd(a - eps * u)
+        fd = (fp - fm) / (2 * eps)
+        assert abs(np.dot(g, u) - fd) <= 1e-6 * max(1.0, abs(fd)), (k, np.dot(g, u), fd)
+
+
+def test_l2_objective_zero_momentum_closed_form():
+    P, q, t = l2_problem(3, -1)
+    f, g = P.objgrd(np.zeros(P.varnum))
+    assert abs(f - 3.5 * np.sum((q - t) ** 2)) <= 1e-14 * f
+    h = 1.0 / (P.T - 1)
+    expect = h * oracle.multiply_kernel(q, 7.0 * (q - t), -1, 0.45)
+    for s in g.reshape(P.T - 1, P.n, P.dim):
+        assert rel_err(s, expect) <= 1e-13
