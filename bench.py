@@ -356,6 +356,9 @@ def main():
         B = lddmm_amd.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
                             k["knl_order"], k["time_stp"], k["time_num"], precision="fp32", chunk=0)
         chunk, pairs = B.info()
+        # direct launches: rocprofv3's kernel tracing has crashed the host process
+        # during hipGraph capture here; graphs do not change kernel times
+        B.set_graphs(False)
         _, info, rounds = B.run(300, 0.0, 0.0, 0.5, 5, precision="fp32", rel_energy_tol=1e-6)
         print(json.dumps(dict(problems=args.batch, chunk=chunk, pairs_per_eval_total=float(pairs.sum()),
                               seconds=info[0]["seconds"], rounds=rounds,
