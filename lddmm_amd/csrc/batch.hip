@@ -741,12 +741,11 @@ void Batch::build_tables(const std::vector<int> &prec_of)
         const int g = pr == kFP32 ? 1 : 0;
         const BatchDesc &d = desc_h_[p];
         probs.push_back(p);
-        const int tkv = g ? 128 : 64;   // FP32 pair kernels keep two targets per lane
+        // FP32 pair kernels keep two targets per lane
+        const int tkv = g ? 128 : 64, tbwd = g ? 128 : 64;
         for (int s = 0; s < d.splits; ++s) {
-            for (int t = 0; t < ceil_div(d.n, tkv); ++t) {
-                kv[g].push_back(make_int4(p, t, s, 0));
-                bwd[g].push_back(make_int4(p, t, s, 0));
-            }
+            for (int t = 0; t < ceil_div(d.n, tkv); ++t) kv[g].push_back(make_int4(p, t, s, 0));
+            for (int t = 0; t < ceil_div(d.n, tbwd); ++t) bwd[g].push_back(make_int4(p, t, s, 0));
         }
         for (int s = 0; s < d.vsd + d.vst; ++s)
             for (int t = 0; t < ceil_div(d.ne, kTile); ++t) vf[g].push_back(make_int4(p, t, s, 0));

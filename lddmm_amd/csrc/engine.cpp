@@ -154,15 +154,17 @@ Objective::Objective(const DeviceProblem &p, hipStream_t st, int target_blocks) 
     target_blocks_ = tb;
     const int D = p_.dim;
     sp_kv_ = plan_kv(p_.n, p_.n, p_.prec, tb);
-    const PairSplit kv_other = plan_kv(p_.n, p_.n, p_.prec == kFP32 ? kFP64 : kFP32, tb);
-    sp_bwd_ = sp_kv_;
+    sp_bwd_ = plan_bwd(p_.n, p_.n, p_.prec, tb);
+    const int other = p_.prec == kFP32 ? kFP64 : kFP32;
+    const PairSplit kv_other = plan_kv(p_.n, p_.n, other, tb), bwd_other = plan_bwd(p_.n, p_.n, other, tb);
     const double fd = p_.ne + p_.nt > 0 ? (double)p_.ne / (double)(p_.ne + p_.nt) : 1.0;
     int tbd = (int)std::max(1.0, tb * fd), tbt = std::max(1, tb - tbd);
     sp_vfd_ = plan_vf(p_.ne, p_.ne, tbd);
     sp_vft_ = plan_vf(p_.ne, std::max(p_.nt, 1), tbt);
     PairSplit sp_tt = plan_vf(std::max(p_.nt, 1), std::max(p_.nt, 1), tb);
 
-    size_t part = (size_t)std::max(sp_kv_.splits, kv_other.splits) * 2 * D * p_.n;
+    size_t part = (size_t)std::max(sp_kv_.splits, kv_other.splits) * D * p_.n;
+    part = std::max(part, (size_t)std::max(sp_bwd_.splits, bwd_other.splits) * 2 * D * p_.n);
     part = std::max(part, (size_t)(sp_vfd_.splits + sp_vft_.splits) * vf_comps(D) * p_.ne);
     part = std::max(part, (size_t)sp_tt.splits * std::max(p_.nt, 1));
     nb_fwd_ = fwd_reduce_blocks(p_.n);
@@ -194,7 +196,7 @@ void Objective::set_precision(int prec)
     p_.prec = prec;
     // the partial-sum buffer was sized for the plans of both policies
     sp_kv_ = plan_kv(p_.n, p_.n, prec, target_blocks_);
-    sp_bwd_ = sp_kv_;
+    sp_bwd_ = plan_bwd(p_.n, p_.n, prec, target_blocks_);
 }
 
 void Objective::compute_target_target(int target_blocks)

@@ -8,10 +8,11 @@
 namespace lddmm {
 
 // ---- pair sums over points (pair_kernels.hip) ---------------------------
-// Split plans for the point kernels: plan_kv for K.a and the fused backward
-// pass (two targets per lane in FP32), plan_single for the dq kernels (one
-// target per lane).
+// Split plans for the point kernels: plan_kv for K.a and plan_bwd for the
+// fused backward pass (two targets per lane in FP32), plan_single for the dq
+// kernels (one target per lane).
 PairSplit plan_kv(int ni, int nj, int prec, int target_blocks);
+PairSplit plan_bwd(int ni, int nj, int prec, int target_blocks);
 PairSplit plan_single(int ni, int nj, int target_blocks);
 // part[(s*D + c)*ni + i] = sum_{j in chunk s} k(qi_i, qj_j) a_j[c]
 void launch_kv(int dim, int order, int prec, const double *qi, int ni, const double *qj, const double *a, int nj,
@@ -26,7 +27,7 @@ void launch_fwd_reduce(int dim, int splits, int n, const double *part, const dou
 int fwd_reduce_blocks(int n);
 // Fused backward pair pass: part[(s*2D + c)*n + i] for c < D holds
 // sum_j k_ij amp_j[c] (amp = a - p), for c >= D the dq sum of
-// l^T K r with l = amp - a/2, r = a.  Plan: plan_kv.
+// l^T K r with l = amp - a/2, r = a.  Plan: plan_bwd.
 void launch_bwd_pairs(int dim, int order, int prec, const double *q, const double *a, const double *p, int n,
                       double width, const PairSplit &sp, double *part, hipStream_t st);
 // g = h * sum_s gpart;  p -= h * sum_s dpart

@@ -147,10 +147,16 @@ void dispatch_all(int dim, int order, int prec, F &&f)
 }  // namespace
 
 int kv_targets_per_block(int prec) { return prec == kFP32 ? 128 : 64; }
+int bwd_targets_per_block(int prec) { return prec == kFP32 ? 128 : 64; }
 
 PairSplit plan_kv(int ni, int nj, int prec, int target_blocks)
 {
     return make_split_ti(ni, nj, kv_targets_per_block(prec), target_blocks);
+}
+
+PairSplit plan_bwd(int ni, int nj, int prec, int target_blocks)
+{
+    return make_split_ti(ni, nj, bwd_targets_per_block(prec), target_blocks);
 }
 
 PairSplit plan_single(int ni, int nj, int target_blocks) { return make_split_ti(ni, nj, 64, target_blocks); }
