@@ -143,6 +143,7 @@ void upload_l2_problem(UploadedProblem &u, int dim, int n, int T, int order, dou
     u.tl.alloc(ht.size());
     u.q0.upload(hq.data(), st);
     u.tl.upload(ht.data(), st);
+    LDDMM_HIP(hipStreamSynchronize(st));   // host vectors go out of scope
     p.q0 = u.q0.p;
     p.tl = u.tl.p;
     validate(p);

@@ -432,6 +432,7 @@ void Mlp::ensure_rows(int rows)
     act_ = dact_ = part_ = nullptr;
     lossbuf_ = nullptr;
     actb_ = dactb_ = xb_ = nullptr;
+    cap_ = 0;   // stays 0 if an allocation below fails
     long long per_row = 0;
     for (int l = 0; l < layers_; ++l) per_row += dout_[l];
     auto alloc = [&](void **p, size_t bytes) {
