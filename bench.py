@@ -110,7 +110,17 @@ def cpu_baseline(w, ifo, pairs_per_eval):
     vf_pairs_full = ed * ed + ed * et + et * et     # the reference recomputes the target-target term
     t_eval = T1 * (2 * t_mult + t_dq) + t_vf * vf_pairs_full / vf_pairs_sample
     sample_s = t_mult + t_dq + t_vf
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return dict(value=pairs_per_eval / t_eval, unit="pair-interactions/s", cores=1, kind="port",
+                host=dict(cpu=model, logical_cpus=os.cpu_count()),
                 sample=("oracle/lddmm_oracle.c, 1 core: K.alpha + dq at N=%d and the varifold with gradient vs "
                         "%d of %d target triangles (%.1f s of CPU work), scaled to one full evaluation "
                         "= %.1f s" % (n, ets, et, sample_s, t_eval)),
