@@ -350,8 +350,8 @@ def main():
                     help="only the batched FP32 registration of the uneven batch (for kernel traces)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (problems would shard by "
-              "target rows or by independent registrations; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (problems would shard by "
+              "independent registrations, or a single large problem by target rows)", file=sys.stderr)
         sys.exit(2)
 
     if args.precision_report:

@@ -3,7 +3,7 @@
 // kernels (FP32 accumulation, FP32 master weights, Adam).  The network maps
 // 32 features of a source vertex to a 3-vector momentum used as the initial
 // L-BFGS iterate (every time slice of alpha set to it); the zero-momentum
-// run stays the fallback and the judge.  No counterpart in the reference.
+// run stays the fallback and the yardstick.  No counterpart in the reference.
 #pragma once
 
 #include <vector>

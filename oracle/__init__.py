@@ -5,8 +5,9 @@ FP64, single-threaded restatement of the reference's CPU path
 (dnhsieh/LDDMM Cpp/src).  `ref()` loads the reference's own kernel sources
 compiled into `_ref/libref{2,3}d.so` when they were built.
 
-Only tests/, `__graft_entry__.smoke()` and the `cpu_baseline` leg of
-bench.py may import this module; the product package never does.
+Only tests/, the smoke check of the repository's entry module and the
+`cpu_baseline` leg of bench.py may import this module; the product package
+never does.
 
 All arrays use the reference's host layout: point sets D x N column-major
 (numpy shape (N, D), C order), stacks (T-1, N, D), element tables (E, V)
