@@ -55,6 +55,25 @@ def test_short_run_is_identical_to_oracle_run(gpu):
     assert np.max(np.abs(alp.reshape(-1) - ealp)) <= 1e-7 * np.max(np.abs(ealp))
 
 
+@pytest.mark.parametrize("kind", ["sphere_cube_matern", "circle_square_matern2d"])
+def test_short_matern_runs_are_identical_to_oracle_runs(gpu, kind):
+    """6-iteration registrations with the C2 kernel family (3D Matern
+    nu = 3/2, 'G' x 'B' varifold) and with a 2D Matern kernel: the same
+    evaluations, energies and momenta as the oracle's L-BFGS."""
+    if kind == "sphere_cube_matern":
+        w = G.sphere_cube(subdivisions=2, cells=8)
+    else:
+        w = circle_square_args(120, T=6)
+        w["knl_order"] = 2
+    w["time_num"], w["time_stp"] = 6, 0.2
+    print("lbfgs %s, 6 iterations" % kind, flush=True)
+    alp, info, ealp, ef, tr, P = run_both(gpu, w, 6)
+    assert info["status"] == 0 and tr["status"] == 0
+    assert info["evaluations"] == tr["total_evals"]
+    assert abs(info["fcn"] - ef) <= 1e-9 * abs(ef)
+    assert np.max(np.abs(alp.reshape(-1) - ealp)) <= 1e-7 * np.max(np.abs(ealp))
+
+
 def test_tolerance_stop_and_line_search_status(gpu):
     print("lbfgs tolerance stop", flush=True)
     w = circle_square_args(100, T=6)
