@@ -223,8 +223,9 @@ int lddmm_batch_run(lddmm_batch *b, const double *iniAlpStks, double *optAlpStks
 /* chosen source chunk and the pair interactions per evaluation of each problem */
 int lddmm_batch_info(lddmm_batch *b, int *chunk, double *pairsPerEval);
 /* on != 0 (default): each batched evaluation replays a hipGraph captured for
- * the current active set; 0: launches are issued one by one.  The
- * environment variable LDDMM_BATCH_GRAPHS=0 sets the default to off. */
+ * the current active set; 0: launches are issued one by one.  The default is
+ * off when the environment holds ROCPROF* variables (a rocprofv3 run) and is
+ * set by LDDMM_BATCH_GRAPHS=0/1. */
 int lddmm_batch_set_graphs(lddmm_batch *b, int on);
 void lddmm_batch_destroy(lddmm_batch *b);
 

@@ -19,6 +19,8 @@
 #include <map>
 #include <tuple>
 
+#include <unistd.h>
+
 #include "batch.hpp"
 #include "pair_ops.hpp"
 #include "varifold_ops.hpp"
@@ -555,6 +557,11 @@ Batch::Batch(const BatchSpec &spec, const double *q0, const int *elm, const doub
              const double *tu, hipStream_t st)
     : spec_(spec), st_(st), tab_(new Tables)
 {
+    // hipGraph replay by default; off when rocprofv3 drives the process
+    // (its kernel tracing has crashed the host process during stream capture
+    // on this ROCm release) or when LDDMM_BATCH_GRAPHS=0
+    for (char **e = environ; e && *e; ++e)
+        if (std::strncmp(*e, "ROCPROF", 7) == 0) use_graphs_ = false;
     if (const char *e = std::getenv("LDDMM_BATCH_GRAPHS")) use_graphs_ = std::atoi(e) != 0;
     const int P = spec_.P(), D = spec_.dim;
     if (P <= 0) throw Error(kErrArg, "a batch needs at least one problem");
