@@ -271,6 +271,13 @@ int lddmm_gemm_bf16(float *C, const uint16_t *A, const uint16_t *B, int M, int N
  * the (1 + maxAdj*V) x N table when elmIfoMat != NULL and returns the row
  * count (negative on error). */
 int lddmm_get_elm_ifo(int32_t *elmIfoMat, const int32_t *elmVtxMat, int vtxPerElm, int elmNum, int ndeNum);
+/* The spatial order in which objectives and batches hold a problem (no
+ * counterpart in the reference, which keeps the caller's order): perm[s] =
+ * the point in slot s (0-based) of ptsMat (D x N, host layout), such that
+ * every aligned block of 64 or 128 slots is one subtree of a median
+ * bisection and hence spatially compact.  Host-only; inputs and outputs of
+ * every other entry point stay in the caller's order. */
+int lddmm_spatial_order(int32_t *perm, const double *ptsMat, int ptsNum, int dimNum);
 /* Element varifold of a mesh (the target preprocessing of the examples,
  * Cpp/examples/circle2square/run_circle2square.m:21-24; the same formulas as
  * landmarksToVarifoldKernel, Cpp/src/varifold3D.cpp:8-39 and

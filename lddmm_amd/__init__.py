@@ -20,7 +20,7 @@ import os
 import numpy as np
 
 __all__ = ["lib", "LDDMM", "getElmIfo", "computeNodePosition", "elementVarifold", "multiplyKernel", "dqKernel", "computeKernel",
-           "varifold", "Objective", "run", "l2Dist", "run_l2", "LddmmError", "profile_enable", "profile_read", "profile_reset"]
+           "varifold", "Objective", "run", "l2Dist", "run_l2", "spatial_order", "LddmmError", "profile_enable", "profile_read", "profile_reset"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblddmm_amd.so")
@@ -93,6 +93,7 @@ def lib():
         L.lddmm_varifold.argtypes = [_dp, _dp, _dp, _ip, _ip, _dp, _dp, _dp, C.c_char, C.c_double, C.c_char,
                                      C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         L.lddmm_get_elm_ifo.argtypes = [_ip, _ip, C.c_int, C.c_int, C.c_int]
+        L.lddmm_spatial_order.argtypes = [_ip, _dp, C.c_int, C.c_int]
         L.lddmm_compute_node_position.argtypes = [_dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_int,
                                                   C.c_int, C.c_int]
         L.lddmm_device_count.argtypes = [C.POINTER(C.c_int)]
@@ -140,6 +141,17 @@ def getElmIfo(elmVtxMat, ndeNum, one_based=True):
     ifo = np.zeros((ndeNum, rows), dtype=np.int32)
     _check(lib().lddmm_get_elm_ifo(_p(ifo), _p(elm), vpe, ne, ndeNum), allow_positive=True)
     return ifo
+
+
+def spatial_order(pts):
+    """The internal order of a point set (N x D): perm[s] = the point in slot
+    s; every aligned block of 64 or 128 slots is spatially compact.  Every
+    entry point takes and returns the caller's order; this shows the order
+    the library computes in."""
+    q = _f64(pts)
+    perm = np.empty(q.shape[0], dtype=np.int32)
+    _check(lib().lddmm_spatial_order(_p(perm), _p(q), q.shape[0], q.shape[1]))
+    return perm
 
 
 # ---------------------------------------------------------------- operators

@@ -595,16 +595,45 @@ Batch::Batch(const BatchSpec &spec, const double *q0, const int *elm, const doub
     }
     vtot_ = (long long)D * (spec_.T - 1) * ntot_;
 
+    // Each problem is held in a spatial order (spatial_order of its initial
+    // vertices, element centres and target centres; the element table
+    // renumbered), as in Objective; momenta and gradients cross in the
+    // host-layout conversions.  Element tables in the internal numbering,
+    // host layout [E_p][V]:
+    vperm_.resize(ntot_);
+    std::vector<int> selm((size_t)D * etot_), tperm(std::max<long long>(ttot_, 1));
+    for (int p = 0; p < P; ++p) {
+        const int n = spec_.n[p], ne = spec_.ne[p], nt = spec_.nt[p];
+        const double *qp = q0 + (size_t)D * vo_[p];
+        const int *ev = elm + (size_t)D * eo_[p];
+        for (long long k = 0; k < (long long)ne * D; ++k)
+            if (ev[k] < 0 || ev[k] >= n) throw Error(kErrArg, "element table holds a vertex outside its problem");
+        const std::vector<int> vp = spatial_order(n, D, [&](int i, int c) { return qp[(size_t)i * D + c]; });
+        std::vector<int> vinv(n);
+        for (int s = 0; s < n; ++s) {
+            vperm_[vo_[p] + s] = vp[s];
+            vinv[vp[s]] = s;
+        }
+        std::vector<double> cen((size_t)D * ne, 0.0);
+        for (int e = 0; e < ne; ++e)
+            for (int k = 0; k < D; ++k)
+                for (int c = 0; c < D; ++c) cen[(size_t)e * D + c] += qp[(size_t)ev[(size_t)e * D + k] * D + c] / D;
+        const std::vector<int> ep = spatial_order(ne, D, [&](int i, int c) { return cen[(size_t)i * D + c]; });
+        for (int s = 0; s < ne; ++s)
+            for (int k = 0; k < D; ++k) selm[(size_t)D * (eo_[p] + s) + k] = vinv[ev[(size_t)ep[s] * D + k]];
+        const double *tp = nt > 0 ? tc + (size_t)D * to_[p] : nullptr;
+        const std::vector<int> tq = spatial_order(nt, D, [&](int i, int c) { return tp[(size_t)i * D + c]; });
+        for (int s = 0; s < nt; ++s) tperm[to_[p] + s] = tq[s];
+    }
+    elm = selm.data();
+
     // adjacency tables per problem (getElmIfo, Cpp/src/getElmIfo.m), padded to one row count
     std::vector<std::vector<int>> ifos(P);
     for (int p = 0; p < P; ++p) {
         const int n = spec_.n[p], ne = spec_.ne[p];
         const int *ev = elm + (size_t)D * eo_[p];
         std::vector<int> cnt(n, 0);
-        for (long long k = 0; k < (long long)ne * D; ++k) {
-            if (ev[k] < 0 || ev[k] >= n) throw Error(kErrArg, "element table holds a vertex outside its problem");
-            ++cnt[ev[k]];
-        }
+        for (long long k = 0; k < (long long)ne * D; ++k) ++cnt[ev[k]];
         int mx = 0;
         for (int c : cnt) mx = std::max(mx, c);
         rows_ = std::max(rows_, 1 + mx * D);
@@ -634,18 +663,23 @@ Batch::Batch(const BatchSpec &spec, const double *q0, const int *elm, const doub
     std::vector<double> hq((size_t)D * ntot_), htc((size_t)D * std::max<long long>(ttot_, 1)),
         htu((size_t)D * std::max<long long>(ttot_, 1));
     std::vector<int> helm((size_t)D * etot_);
+    std::vector<double> hta(std::max<long long>(ttot_, 1));
     for (int p = 0; p < P; ++p) {
         const int n = spec_.n[p], ne = spec_.ne[p], nt = spec_.nt[p];
         for (int i = 0; i < n; ++i)
-            for (int c = 0; c < D; ++c) hq[(size_t)D * vo_[p] + (size_t)c * n + i] = q0[(size_t)D * (vo_[p] + i) + c];
+            for (int c = 0; c < D; ++c)
+                hq[(size_t)D * vo_[p] + (size_t)c * n + i] = q0[(size_t)D * (vo_[p] + vperm_[vo_[p] + i]) + c];
         for (int e = 0; e < ne; ++e)
             for (int c = 0; c < D; ++c)
                 helm[(size_t)D * eo_[p] + (size_t)c * ne + e] = elm[(size_t)D * (eo_[p] + e) + c];
-        for (int e = 0; e < nt; ++e)
+        for (int e = 0; e < nt; ++e) {
+            const long long src = to_[p] + tperm[to_[p] + e];
             for (int c = 0; c < D; ++c) {
-                htc[(size_t)D * to_[p] + (size_t)c * nt + e] = tc[(size_t)D * (to_[p] + e) + c];
-                htu[(size_t)D * to_[p] + (size_t)c * nt + e] = tu[(size_t)D * (to_[p] + e) + c];
+                htc[(size_t)D * to_[p] + (size_t)c * nt + e] = tc[(size_t)D * src + c];
+                htu[(size_t)D * to_[p] + (size_t)c * nt + e] = tu[(size_t)D * src + c];
             }
+            hta[to_[p] + e] = ta[src];
+        }
     }
 
     chunk_ = spec_.chunk > 0 ? ((spec_.chunk + 63) / 64) * 64 : 1024;
@@ -689,7 +723,7 @@ Batch::Batch(const BatchSpec &spec, const double *q0, const int *elm, const doub
     if (ttot_ > 0) {
         LDDMM_HIP(hipMemcpyAsync(tc_, htc.data(), (size_t)D * ttot_ * sizeof(double), hipMemcpyHostToDevice, st_));
         LDDMM_HIP(hipMemcpyAsync(tu_, htu.data(), (size_t)D * ttot_ * sizeof(double), hipMemcpyHostToDevice, st_));
-        LDDMM_HIP(hipMemcpyAsync(ta_, ta, (size_t)ttot_ * sizeof(double), hipMemcpyHostToDevice, st_));
+        LDDMM_HIP(hipMemcpyAsync(ta_, hta.data(), (size_t)ttot_ * sizeof(double), hipMemcpyHostToDevice, st_));
     }
     LDDMM_HIP(hipMemcpyAsync(desc_, desc_h_.data(), P * sizeof(BatchDesc), hipMemcpyHostToDevice, st_));
     LDDMM_HIP(hipStreamSynchronize(st_));
@@ -1048,7 +1082,8 @@ std::vector<double> Batch::to_device_layout(const double *h) const
         for (int t = 0; t < T1; ++t)
             for (int i = 0; i < n; ++i)
                 for (int c = 0; c < D; ++c)
-                    out[off + (size_t)t * D * n + (size_t)c * n + i] = h[off + (size_t)t * D * n + (size_t)i * D + c];
+                    out[off + (size_t)t * D * n + (size_t)c * n + i] =
+                        h[off + (size_t)t * D * n + (size_t)vperm_[vo_[p] + i] * D + c];
     }
     return out;
 }
@@ -1062,7 +1097,8 @@ void Batch::to_host_layout(const std::vector<double> &dv, double *h) const
         for (int t = 0; t < T1; ++t)
             for (int i = 0; i < n; ++i)
                 for (int c = 0; c < D; ++c)
-                    h[off + (size_t)t * D * n + (size_t)i * D + c] = dv[off + (size_t)t * D * n + (size_t)c * n + i];
+                    h[off + (size_t)t * D * n + (size_t)vperm_[vo_[p] + i] * D + c] =
+                        dv[off + (size_t)t * D * n + (size_t)c * n + i];
     }
 }
 

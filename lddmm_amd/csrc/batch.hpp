@@ -84,7 +84,8 @@ public:
     // f_dev[p], grd (device, concatenated per-problem [T-1][D][N_p] blocks)
     // for the problems with prec_of[p] >= 0 (their pair arithmetic)
     void eval(const double *alp, double *grd, double *f_dev, const std::vector<int> &prec_of);
-    // host-layout helpers
+    // host layout (caller's vertex order) <-> device layout (internal
+    // spatial order of every problem, as in Objective)
     std::vector<double> to_device_layout(const double *host_alp) const;
     void to_host_layout(const std::vector<double> &dev, double *host_alp) const;
     const double *target_terms() const { return tt_; }
@@ -100,6 +101,7 @@ private:
     int chunk_ = 1024, vchunk_ = 1024, rows_ = 1;
     long long ntot_ = 0, etot_ = 0, ttot_ = 0, vtot_ = 0;
     std::vector<long long> vo_, eo_, to_;
+    std::vector<int> vperm_;   // per problem: internal slot -> caller's vertex (local indices)
     std::vector<BatchDesc> desc_h_;
     Slab slab_;
     BatchDesc *desc_ = nullptr;

@@ -455,8 +455,9 @@ int lddmm_objective_trajectory(lddmm_objective *obj, double *ndeStk)
         require(obj && ndeStk, "null argument");
         const DeviceProblem &p = obj->u.p;
         std::vector<double> h((size_t)p.dim * p.n * p.T);
-        LDDMM_HIP(hipMemcpyAsync(h.data(), obj->obj->trajectory(), h.size() * sizeof(double),
-                                 hipMemcpyDeviceToHost, obj->st.s));
+        DevBuf<double> d(h.size());
+        obj->obj->trajectory_to(d.p);
+        d.download(h.data(), obj->st.s);
         obj->st.sync();
         from_soa(ndeStk, h.data(), p.n, p.dim, p.T);
         return 0;
@@ -1004,6 +1005,19 @@ int lddmm_varifold(double *vfdVal, double *dqVfdMat, const double *dfmLmkPosMat,
 }
 
 // ---------------------------------------------------------------- data formats
+int lddmm_spatial_order(int32_t *perm, const double *ptsMat, int ptsNum, int dimNum)
+{
+    return guarded([&] {
+        require(ptsNum >= 0, "negative size");
+        require(ptsNum == 0 || (perm && ptsMat), "null argument");
+        require(dimNum == 2 || dimNum == 3, "dimNum must be 2 or 3");
+        const std::vector<int> v =
+            spatial_order(ptsNum, dimNum, [&](int i, int c) { return ptsMat[(size_t)i * dimNum + c]; });
+        std::copy(v.begin(), v.end(), perm);
+        return 0;
+    });
+}
+
 int lddmm_get_elm_ifo(int32_t *elmIfoMat, const int32_t *elmVtxMat, int vtxPerElm, int elmNum, int ndeNum)
 {
     return guarded([&] {

@@ -184,11 +184,168 @@ Objective::Objective(const DeviceProblem &p, hipStream_t st, int target_blocks) 
     slab_.reserve(&ep_vf_, (size_t)std::max(std::max(nb_vf_, nb_tt), nb_l2_));
     slab_.reserve(&tt_, 1);
     slab_.reserve(&scratch_, (size_t)kDotBlocks);
+    // the problem in the internal order
+    slab_.reserve(&q0_, (size_t)D * p_.n);
+    slab_.reserve(&vperm_, (size_t)p_.n);
+    slab_.reserve(&alp_i_, (size_t)p_.varnum());
+    slab_.reserve(&grd_i_, (size_t)p_.varnum());
+    if (p_.tl) {
+        slab_.reserve(&tl_, (size_t)D * p_.n);
+    } else {
+        slab_.reserve(&elm_, (size_t)D * p_.ne);
+        slab_.reserve(&ifo_, (size_t)p_.ifo_rows * p_.n);
+        if (p_.nt > 0) {
+            slab_.reserve(&tc_, (size_t)D * p_.nt);
+            slab_.reserve(&tu_, (size_t)D * p_.nt);
+            slab_.reserve(&ta_, (size_t)p_.nt);
+        }
+    }
     slab_.allocate();
+    reorder();
     compute_target_target(tb);
 }
 
 Objective::~Objective() = default;
+
+// Reads the problem back, orders vertices, deformed elements and target
+// elements spatially (spatial_order of the initial vertex positions, the
+// initial element centres and the target centres), renumbers the element
+// and adjacency tables, checks every index on the way, and points p_ at
+// the reordered copies.  Every sum of the objective is over the same
+// terms; only their order changes.
+void Objective::reorder()
+{
+    const int D = p_.dim, n = p_.n, ne = p_.ne, nt = p_.nt, rows = p_.ifo_rows;
+    auto down = [&](auto *dst, const auto *src, size_t count) {
+        if (count) LDDMM_HIP(hipMemcpyAsync(dst, src, count * sizeof(*src), hipMemcpyDeviceToHost, st_));
+    };
+    auto up = [&](auto *dst, const auto *src, size_t count) {
+        if (count) LDDMM_HIP(hipMemcpyAsync(dst, src, count * sizeof(*src), hipMemcpyHostToDevice, st_));
+    };
+    std::vector<double> q((size_t)D * n), tl, tc, tu, ta;
+    std::vector<int> elm, ifo;
+    down(q.data(), p_.q0, q.size());
+    if (p_.tl) {
+        tl.resize((size_t)D * n);
+        down(tl.data(), p_.tl, tl.size());
+    } else {
+        elm.resize((size_t)D * ne);
+        ifo.resize((size_t)rows * n);
+        down(elm.data(), p_.elm, elm.size());
+        down(ifo.data(), p_.ifo, ifo.size());
+        if (nt > 0) {
+            tc.resize((size_t)D * nt);
+            tu.resize((size_t)D * nt);
+            ta.resize(nt);
+            down(tc.data(), p_.tc, tc.size());
+            down(tu.data(), p_.tu, tu.size());
+            down(ta.data(), p_.ta, ta.size());
+        }
+    }
+    LDDMM_HIP(hipStreamSynchronize(st_));
+
+    const std::vector<int> vp = spatial_order(n, D, [&](int i, int c) { return q[(size_t)c * n + i]; });
+    std::vector<int> vinv(n);
+    for (int s = 0; s < n; ++s) vinv[vp[s]] = s;
+    std::vector<double> qs((size_t)D * n);
+    for (int c = 0; c < D; ++c)
+        for (int s = 0; s < n; ++s) qs[(size_t)c * n + s] = q[(size_t)c * n + vp[s]];
+    up(q0_, qs.data(), qs.size());
+    up(vperm_, vp.data(), vp.size());
+    p_.q0 = q0_;
+
+    std::vector<double> tls, tcs, tus, tas;
+    std::vector<int> elms, ifos;
+    if (p_.tl) {
+        tls.resize(tl.size());
+        for (int c = 0; c < D; ++c)
+            for (int s = 0; s < n; ++s) tls[(size_t)c * n + s] = tl[(size_t)c * n + vp[s]];
+        up(tl_, tls.data(), tls.size());
+        p_.tl = tl_;
+    } else {
+        for (int v : elm)
+            if (v < 0 || v >= n) throw Error(kErrArg, "dfmElmVtxMat holds a vertex index outside [0, ndeNum)");
+        std::vector<double> cen((size_t)D * ne, 0.0);
+        for (int e = 0; e < ne; ++e)
+            for (int k = 0; k < D; ++k) {
+                const int v = elm[(size_t)k * ne + e];
+                for (int c = 0; c < D; ++c) cen[(size_t)c * ne + e] += q[(size_t)c * n + v] / D;
+            }
+        const std::vector<int> ep = spatial_order(ne, D, [&](int i, int c) { return cen[(size_t)c * ne + i]; });
+        std::vector<int> einv(ne);
+        for (int s = 0; s < ne; ++s) einv[ep[s]] = s;
+        elms.resize(elm.size());
+        for (int k = 0; k < D; ++k)
+            for (int s = 0; s < ne; ++s) elms[(size_t)k * ne + s] = vinv[elm[(size_t)k * ne + ep[s]]];
+        // adjacency rows: count, then per element (element, sign) in 2D or
+        // (element, vertex, vertex) in 3D (Cpp/src/getElmIfo.m)
+        ifos.assign(ifo.size(), 0);
+        for (int s = 0; s < n; ++s) {
+            const int u = vp[s], cnt = ifo[u];
+            if (cnt < 0 || 1 + (long long)cnt * D > rows)
+                throw Error(kErrArg, "dfmElmIfoMat count exceeds its row count");
+            ifos[s] = cnt;
+            for (int a = 0; a < cnt; ++a) {
+                const size_t b = 1 + (size_t)a * D;
+                const int e = ifo[b * n + u];
+                if (e < 0 || e >= ne) throw Error(kErrArg, "dfmElmIfoMat holds an element index outside [0, dfmElmNum)");
+                ifos[b * n + s] = einv[e];
+                if (D == 2) {
+                    ifos[(b + 1) * n + s] = ifo[(b + 1) * n + u];
+                } else {
+                    for (int z = 1; z <= 2; ++z) {
+                        const int v = ifo[(b + z) * n + u];
+                        if (v < 0 || v >= n) throw Error(kErrArg, "dfmElmIfoMat holds a vertex index outside [0, ndeNum)");
+                        ifos[(b + z) * n + s] = vinv[v];
+                    }
+                }
+            }
+        }
+        up(elm_, elms.data(), elms.size());
+        up(ifo_, ifos.data(), ifos.size());
+        p_.elm = elm_;
+        p_.ifo = ifo_;
+        if (nt > 0) {
+            const std::vector<int> tp = spatial_order(nt, D, [&](int i, int c) { return tc[(size_t)c * nt + i]; });
+            tcs.resize(tc.size());
+            tus.resize(tu.size());
+            tas.resize(nt);
+            for (int s = 0; s < nt; ++s) {
+                for (int c = 0; c < D; ++c) {
+                    tcs[(size_t)c * nt + s] = tc[(size_t)c * nt + tp[s]];
+                    tus[(size_t)c * nt + s] = tu[(size_t)c * nt + tp[s]];
+                }
+                tas[s] = ta[tp[s]];
+            }
+            up(tc_, tcs.data(), tcs.size());
+            up(tu_, tus.data(), tus.size());
+            up(ta_, tas.data(), tas.size());
+            p_.tc = tc_;
+            p_.tu = tu_;
+            p_.ta = ta_;
+        }
+    }
+    LDDMM_HIP(hipStreamSynchronize(st_));   // host copies go out of scope
+}
+
+void Objective::to_internal(const double *user, double *internal, int slices)
+{
+    launch_permute(user, internal, vperm_, p_.n, slices * p_.dim, true, st_);
+}
+
+void Objective::to_user(const double *internal, double *user, int slices)
+{
+    launch_permute(internal, user, vperm_, p_.n, slices * p_.dim, false, st_);
+}
+
+void Objective::trajectory_to(double *dst) { to_user(nde_, dst, p_.T); }
+
+void Objective::eval(const double *alp, double *grd, double *f_dev)
+{
+    to_internal(alp, alp_i_, p_.T - 1);
+    eval_internal(alp_i_, grd_i_, f_dev);
+    to_user(grd_i_, grd, p_.T - 1);
+}
 
 void Objective::set_precision(int prec)
 {
@@ -224,7 +381,7 @@ double Objective::pairs_per_eval() const
     return 2.0 * T1 * n * n + (double)p_.ne * p_.ne + (double)p_.ne * p_.nt;   // l2 matching: ne = nt = 0
 }
 
-void Objective::eval(const double *alp, double *grd, double *f_dev)
+void Objective::eval_internal(const double *alp, double *grd, double *f_dev)
 {
     const int D = p_.dim, n = p_.n, T = p_.T;
     const size_t dn = (size_t)D * n;
@@ -332,7 +489,7 @@ void fetch(LbfgsState &s, int first, int count, hipStream_t st)
 
 }  // namespace
 
-LbfgsResult run_lbfgs(Objective &obj, double *x, const LbfgsOptions &o, hipStream_t st)
+LbfgsResult run_lbfgs(Objective &obj, double *x_user, const LbfgsOptions &o, hipStream_t st)
 {
     auto t0 = std::chrono::steady_clock::now();
     if (o.mem < 1) throw Error(kErrArg, "memNum must be at least 1");
@@ -340,6 +497,10 @@ LbfgsResult run_lbfgs(Objective &obj, double *x, const LbfgsOptions &o, hipStrea
     const long long nv = obj.problem().varnum();
     const int m = o.mem;
     LbfgsState s;
+    // the iterate and every L-BFGS vector live in the objective's internal
+    // vertex order; x_user is read at the start and written at the end
+    double *x = nullptr;
+    s.slab.reserve(&x, nv);
     s.slab.reserve(&s.g, nv);
     s.slab.reserve(&s.d, nv);
     s.slab.reserve(&s.xt, nv);
@@ -353,6 +514,7 @@ LbfgsResult run_lbfgs(Objective &obj, double *x, const LbfgsOptions &o, hipStrea
     s.slab.reserve(&s.scratch, (size_t)kMaxDots * kDotBlocks);
     s.slab.allocate();
     LDDMM_HIP(hipHostMalloc((void **)&s.host, sizeof(double) * kNumScal, hipHostMallocDefault));
+    obj.to_internal(x_user, x, obj.problem().T - 1);
 
     LbfgsResult r;
     auto push_trace = [&](double f, double g, double step, int cnt) {
@@ -363,7 +525,7 @@ LbfgsResult run_lbfgs(Objective &obj, double *x, const LbfgsOptions &o, hipStrea
     };
 
     // f, g at the start (Cpp/src/LBFGS.cpp:50-52)
-    obj.eval(x, s.g, s.scal + kF);
+    obj.eval_internal(x, s.g, s.scal + kF);
     {
         const double *xs[1] = {s.g}, *ys[1] = {s.g};
         dots(s, 1, xs, ys, nv, s.scal + kGG, st);
@@ -387,7 +549,7 @@ LbfgsResult run_lbfgs(Objective &obj, double *x, const LbfgsOptions &o, hipStrea
     // whose arithmetic is FP64 throughout)
     auto refine = [&](const char *why) {
         obj.set_precision(kFP64);
-        obj.eval(x, s.g, s.scal + kF);
+        obj.eval_internal(x, s.g, s.scal + kF);
         const double *xs[1] = {s.g}, *ys[1] = {s.g};
         dots(s, 1, xs, ys, nv, s.scal + kGG, st);
         fetch(s, kF, 2, st);
@@ -475,7 +637,7 @@ LbfgsResult run_lbfgs(Objective &obj, double *x, const LbfgsOptions &o, hipStrea
                 ProfScope ps(kProfLbfgsVector, st);
                 vec_try(s.xt, x, rel, s.d, nv, st);
             }
-            obj.eval(s.xt, s.gt, s.scal + kF);
+            obj.eval_internal(s.xt, s.gt, s.scal + kF);
             ++cnt;
             {
                 const double *xs[2] = {s.gt, s.gt}, *ys[2] = {s.gt, s.d};
@@ -573,6 +735,7 @@ LbfgsResult run_lbfgs(Objective &obj, double *x, const LbfgsOptions &o, hipStrea
         }
         if (o.rel_tol > 0.0 && (fprev - fnow) < o.rel_tol * std::fabs(fnow)) break;
     }
+    obj.to_user(x, x_user, obj.problem().T - 1);
     LDDMM_HIP(hipStreamSynchronize(st));
     r.f = fnow;
     r.gnorm = glen;

@@ -3,8 +3,11 @@
 // (Cpp/src/LBFGS.cpp:25-112) and kernel-time accounting.
 #pragma once
 
+#include <algorithm>
 #include <array>
+#include <cfloat>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "kernels.hpp"
@@ -120,16 +123,69 @@ struct DeviceProblem {
 
 void validate(const DeviceProblem &p);
 
+// ---------------------------------------------------------------- spatial order
+// Order of n points (coord(i, c) = coordinate c of point i) such that every
+// aligned block of 64 slots, and of 128 slots, holds the points of one
+// subtree of a median bisection: a node of more than 64 points is split at
+// the median of its widest coordinate into a first part whose size is a
+// multiple of g (g = 128 above 256 points, else 64).  Block offsets are
+// multiples of g at every level, so no aligned block straddles two
+// subtrees, and the pair kernels' target tiles (64 or 128 consecutive
+// slots) are spatially compact.  Returns perm[slot] = point.
+template <class Coord>
+std::vector<int> spatial_order(int n, int D, Coord coord)
+{
+    std::vector<int> idx(n > 0 ? n : 0);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::vector<std::pair<int, int>> todo;
+    if (n > 64) todo.push_back({0, n});
+    while (!todo.empty()) {
+        const int lo = todo.back().first, hi = todo.back().second;
+        todo.pop_back();
+        const int len = hi - lo;
+        double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+        for (int k = lo; k < hi; ++k)
+            for (int c = 0; c < D; ++c) {
+                const double v = coord(idx[k], c);
+                mn[c] = std::min(mn[c], v);
+                mx[c] = std::max(mx[c], v);
+            }
+        int ax = 0;
+        for (int c = 1; c < D; ++c)
+            if (mx[c] - mn[c] > mx[ax] - mn[ax]) ax = c;
+        const int g = len > 256 ? 128 : 64;
+        const int left = g * ((len + 2 * g - 1) / (2 * g));
+        std::nth_element(idx.begin() + lo, idx.begin() + lo + left, idx.begin() + hi, [&](int a, int b) {
+            const double va = coord(a, ax), vb = coord(b, ax);
+            return va < vb || (va == vb && a < b);
+        });
+        if (left > 64) todo.push_back({lo, lo + left});
+        if (hi - lo - left > 64) todo.push_back({lo + left, hi});
+    }
+    return idx;
+}
+
 // E(alpha) and dE/dalpha on the device.  The flow stack, adjoint and all
 // partial sums live in one slab; the constant target-target varifold term
-// is computed once at construction.
+// is computed once at construction.  Internally the problem is held in a
+// spatial order (vertices, deformed elements and target elements each by
+// spatial_order of their initial positions, the element and adjacency
+// tables renumbered to match); eval() and trajectory_to() take and give
+// the caller's vertex order.
 class Objective {
 public:
     Objective(const DeviceProblem &p, hipStream_t st, int target_blocks = 0);
     ~Objective();
-    // f_dev[0] = E(alp); grd = dE/dalp.  alp and grd are (T-1) SoA slices.
+    // f_dev[0] = E(alp); grd = dE/dalp.  alp and grd are (T-1) SoA slices
+    // in the caller's vertex order.
     void eval(const double *alp, double *grd, double *f_dev);
-    const double *trajectory() const { return nde_; }   // T SoA slices of the last eval
+    // the same with alp and grd in the internal vertex order
+    void eval_internal(const double *alp, double *grd, double *f_dev);
+    // stacks of `slices` D x N SoA slices between the two vertex orders
+    void to_internal(const double *user, double *internal, int slices);
+    void to_user(const double *internal, double *user, int slices);
+    // T SoA slices of the last evaluation's flow, in the caller's order
+    void trajectory_to(double *dst);
     double target_target() const { return tt_host_; }
     const DeviceProblem &problem() const { return p_; }
     // switch the pair arithmetic (kFP64 / kFP32) of later evaluations
@@ -147,8 +203,13 @@ private:
     double *nde_ = nullptr, *p_adj_ = nullptr, *part_ = nullptr;
     double *ecen_ = nullptr, *edir_ = nullptr, *evol_ = nullptr, *dc_ = nullptr, *dd_ = nullptr;
     double *ep_fwd_ = nullptr, *ep_vf_ = nullptr, *tt_ = nullptr, *scratch_ = nullptr;
+    // the problem in the internal order, and the vertex permutation
+    double *q0_ = nullptr, *tc_ = nullptr, *tu_ = nullptr, *ta_ = nullptr, *tl_ = nullptr;
+    int *elm_ = nullptr, *ifo_ = nullptr, *vperm_ = nullptr;
+    double *alp_i_ = nullptr, *grd_i_ = nullptr;
     int nb_fwd_ = 0, nb_vf_ = 0, nb_l2_ = 0, target_blocks_ = 0;
     double tt_host_ = 0.0;
+    void reorder();
     void compute_target_target(int target_blocks);
 };
 

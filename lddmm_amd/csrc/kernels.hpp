@@ -88,5 +88,8 @@ void launch_l2_match(int dim, const double *q, const double *tl, int n, double w
                      hipStream_t st);
 void launch_energy(const double *fwd_parts, int nf, const double *vf_parts, int nv, const double *tt, double h,
                    double w, double *out, hipStream_t st);
+// Vertex permutation of `rows` stacked length-n rows (perm[slot] = vertex):
+// gather dst[r*n + s] = src[r*n + perm[s]]; scatter dst[r*n + perm[s]] = src[r*n + s]
+void launch_permute(const double *src, double *dst, const int *perm, int n, int rows, bool gather, hipStream_t st);
 
 }  // namespace lddmm

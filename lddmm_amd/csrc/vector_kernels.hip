@@ -8,6 +8,8 @@
 // cublasDdot / ddot order is not specified).  Scalars stay in device
 // memory; the two-loop coefficients are formed on the device from them, so
 // a direction costs no host round trip.
+#include <algorithm>
+
 #include "kernels.hpp"
 #include "vector_ops.hpp"
 
@@ -175,6 +177,18 @@ __global__ void k_accept(double *__restrict__ x, double *__restrict__ g, const d
 
 inline int nblk(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
+// rows x n permutation between the caller's and the internal vertex order
+__global__ void k_permute(const double *__restrict__ src, double *__restrict__ dst, const int *__restrict__ perm,
+                          int n, bool gather)
+{
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const size_t row = (size_t)blockIdx.y * n;
+    const int v = perm[s];
+    if (gather) dst[row + s] = src[row + v];
+    else dst[row + v] = src[row + s];
+}
+
 }  // namespace
 
 void launch_dots(int nd, const double *const *xs, const double *const *ys, long long n, double *scratch,
@@ -206,6 +220,16 @@ void launch_energy(const double *fwd_parts, int nf, const double *vf_parts, int 
 {
     k_energy<<<1, kBlock, 0, st>>>(fwd_parts, nf, vf_parts, nv, tt, h, w, out);
     LDDMM_LAUNCH_CHECK();
+}
+
+void launch_permute(const double *src, double *dst, const int *perm, int n, int rows, bool gather, hipStream_t st)
+{
+    for (int r0 = 0; r0 < rows && n > 0; r0 += 65535) {
+        const int nr = std::min(rows - r0, 65535);
+        const size_t off = (size_t)r0 * n;
+        k_permute<<<dim3(ceil_div(n, kBlock), nr), kBlock, 0, st>>>(src + off, dst + off, perm, n, gather);
+        LDDMM_LAUNCH_CHECK();
+    }
 }
 
 int l2_match_blocks(int n) { return ceil_div(n, kBlock); }

@@ -1,0 +1,123 @@
+"""The caller's vertex order at the boundary.  Objectives and batches hold
+every problem in their own spatial order; relabelling the vertices,
+elements and target elements of a problem must therefore leave energies
+unchanged and permute gradients, trajectories and optimised momenta in the
+same way, through every entry point (host arrays, device arrays, the
+registration driver, the batch)."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_err
+from lddmm_amd import geometry as G
+
+pytestmark = pytest.mark.gpu
+
+K = dict(G.BATCH_KERNELS)
+K.update(time_num=6, time_stp=0.2)
+
+
+def relabel(p, seed):
+    """New vertex k is old vertex vp[k]; element and target rows shuffled,
+    each element's vertex order (its orientation) kept."""
+    rng = np.random.default_rng(seed)
+    n = len(p["q0"])
+    vp = rng.permutation(n)
+    inv = np.empty(n, dtype=np.int64)
+    inv[vp] = np.arange(n)
+    ep = rng.permutation(len(p["elm"]))
+    tp = rng.permutation(len(p["tgt_vol"]))
+    q = dict(p)
+    q["q0"] = p["q0"][vp]
+    q["elm"] = inv[p["elm"][ep]].astype(np.int32)
+    q["tgt_cen"], q["tgt_dir"], q["tgt_vol"] = p["tgt_cen"][tp], p["tgt_dir"][tp], p["tgt_vol"][tp]
+    return q, vp
+
+
+def objective(gpu, p, precision="fp64"):
+    ifo = oracle.get_elm_ifo(p["elm"], len(p["q0"]))
+    return gpu.Objective(p["q0"], p["elm"], ifo, p["tgt_cen"], p["tgt_vol"], p["tgt_dir"], K["cen_type"],
+                         K["cen_width"], K["dir_type"], K["dir_width"], K["dis_wgt"], K["knl_order"], p["knl_width"],
+                         K["time_stp"], K["time_num"], precision=precision)
+
+
+@pytest.fixture(scope="module")
+def problem():
+    return G.batch_problems(1, seed=17, nmin=1500, nmax=1600)[0]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_objective_is_invariant_under_relabelling(gpu, problem, precision):
+    print("relabelled objective, %s" % precision, flush=True)
+    q, vp = relabel(problem, 3)
+    n, T = len(problem["q0"]), K["time_num"]
+    alp = 0.05 * G.normal(5, (T - 1) * n * 3).reshape(T - 1, n, 3)
+    o1, o2 = objective(gpu, problem, precision), objective(gpu, q, precision)
+    f1, g1 = o1(alp)
+    f2, g2 = o2(alp[:, vp, :])
+    tol = (1e-13, 1e-12) if precision == "fp64" else (1e-6, 1e-4)
+    assert abs(f2 - f1) <= tol[0] * abs(f1)
+    assert rel_err(g2.reshape(T - 1, n, 3), g1.reshape(T - 1, n, 3)[:, vp, :]) <= tol[1]
+    if precision == "fp64":
+        tr1, tr2 = o1.trajectory(), o2.trajectory()
+        assert np.max(np.abs(tr2 - tr1[:, vp, :])) <= 1e-13
+        assert np.array_equal(tr1[0], problem["q0"]) and np.array_equal(tr2[0], q["q0"])
+    o1.close()
+    o2.close()
+
+
+def test_device_entry_takes_the_callers_order(gpu, problem):
+    """lddmm_objgrd_dev on device arrays (device layout, caller's order)
+    gives the host entry's result bitwise."""
+    n, T = len(problem["q0"]), K["time_num"]
+    alp = 0.05 * G.normal(6, (T - 1) * n * 3).reshape(T - 1, n, 3)
+    obj = objective(gpu, problem)
+    f, g = obj(alp)
+    d_alp = gpu.DeviceArray(np.ascontiguousarray(alp.transpose(0, 2, 1)))
+    d_grd = gpu.DeviceArray(nbytes=alp.nbytes)
+    d_f = gpu.DeviceArray(nbytes=8)
+    obj.eval_device(d_f.ptr, d_grd.ptr, d_alp.ptr)
+    obj.synchronize()
+    gd = d_grd.get().reshape(T - 1, 3, n).transpose(0, 2, 1)
+    assert d_f.get()[0] == f and np.array_equal(gd.reshape(-1), g)
+    obj.close()
+
+
+def test_registration_is_invariant_under_relabelling(gpu, problem):
+    print("relabelled registration", flush=True)
+    q, vp = relabel(problem, 4)
+    out = []
+    for p in (problem, q):
+        ifo = gpu.getElmIfo(p["elm"], len(p["q0"]), one_based=False)
+        a, info = gpu.run(p["q0"], None, p["elm"], ifo, p["tgt_cen"], p["tgt_vol"], p["tgt_dir"], K["cen_type"],
+                          K["cen_width"], K["dir_type"], K["dir_width"], K["dis_wgt"], K["knl_order"],
+                          p["knl_width"], K["time_stp"], K["time_num"], 6, 0.0, 0.0, 0.5, 5)
+        out.append((a, info))
+    (a1, i1), (a2, i2) = out
+    assert i1["iterations"] == i2["iterations"] == 6
+    assert abs(i2["fcn"] - i1["fcn"]) <= 1e-11 * abs(i1["fcn"])
+    assert rel_err(a2, a1[:, vp, :]) <= 1e-9
+
+
+def test_batch_is_invariant_under_relabelling(gpu):
+    print("relabelled batch", flush=True)
+    probs = G.batch_problems(3, seed=23, nmin=300, nmax=2000)
+    rel = [relabel(p, 10 + i) for i, p in enumerate(probs)]
+    T = K["time_num"]
+    args = (K["cen_type"], K["cen_width"], K["dir_type"], K["dir_width"], K["dis_wgt"], K["knl_order"],
+            K["time_stp"], T)
+    B1 = gpu.Batch(probs, *args, chunk=256)
+    B2 = gpu.Batch([r[0] for r in rel], *args, chunk=256)
+    alps = [0.05 * G.normal(40 + i, B1.sizes[i]).reshape(T - 1, B1.n[i], 3) for i in range(B1.P)]
+    f1, g1 = B1.objgrd(alps)
+    f2, g2 = B2.objgrd([a[:, vp, :] for a, (_, vp) in zip(alps, rel)])
+    for i, (_, vp) in enumerate(rel):
+        assert abs(f2[i] - f1[i]) <= 1e-13 * abs(f1[i])
+        assert rel_err(g2[i], g1[i][:, vp, :]) <= 1e-12
+    a1, i1, _ = B1.run(4, 0.0, 0.0, 0.5, 5)
+    a2, i2, _ = B2.run(4, 0.0, 0.0, 0.5, 5)
+    for i, (_, vp) in enumerate(rel):
+        assert abs(i2[i]["fcn"] - i1[i]["fcn"]) <= 1e-11 * abs(i1[i]["fcn"])
+        assert rel_err(a2[i], a1[i][:, vp, :]) <= 1e-9
+    B1.close()
+    B2.close()
