@@ -278,6 +278,12 @@ int lddmm_get_elm_ifo(int32_t *elmIfoMat, const int32_t *elmVtxMat, int vtxPerEl
  * bisection and hence spatially compact.  Host-only; inputs and outputs of
  * every other entry point stay in the caller's order. */
 int lddmm_spatial_order(int32_t *perm, const double *ptsMat, int ptsNum, int dimNum);
+/* Exact culling in the FP32 policy (no counterpart in the reference): a
+ * pair of source and target tiles whose Gaussian kernel values (deformation
+ * kernel, or the varifold's 'G' position kernel) are all exactly 0 in FP32
+ * is skipped.  On by default; off computes every pair and gives bitwise the
+ * same results (for checking).  Process-wide, read at every launch. */
+int lddmm_set_pair_culling(int on);
 /* Element varifold of a mesh (the target preprocessing of the examples,
  * Cpp/examples/circle2square/run_circle2square.m:21-24; the same formulas as
  * landmarksToVarifoldKernel, Cpp/src/varifold3D.cpp:8-39 and

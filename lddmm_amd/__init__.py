@@ -20,7 +20,7 @@ import os
 import numpy as np
 
 __all__ = ["lib", "LDDMM", "getElmIfo", "computeNodePosition", "elementVarifold", "multiplyKernel", "dqKernel", "computeKernel",
-           "varifold", "Objective", "run", "l2Dist", "run_l2", "spatial_order", "LddmmError", "profile_enable", "profile_read", "profile_reset"]
+           "varifold", "Objective", "run", "l2Dist", "run_l2", "spatial_order", "set_pair_culling", "LddmmError", "profile_enable", "profile_read", "profile_reset"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblddmm_amd.so")
@@ -94,6 +94,7 @@ def lib():
                                      C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         L.lddmm_get_elm_ifo.argtypes = [_ip, _ip, C.c_int, C.c_int, C.c_int]
         L.lddmm_spatial_order.argtypes = [_ip, _dp, C.c_int, C.c_int]
+        L.lddmm_set_pair_culling.argtypes = [C.c_int]
         L.lddmm_compute_node_position.argtypes = [_dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_int,
                                                   C.c_int, C.c_int]
         L.lddmm_device_count.argtypes = [C.POINTER(C.c_int)]
@@ -152,6 +153,13 @@ def spatial_order(pts):
     perm = np.empty(q.shape[0], dtype=np.int32)
     _check(lib().lddmm_spatial_order(_p(perm), _p(q), q.shape[0], q.shape[1]))
     return perm
+
+
+def set_pair_culling(on):
+    """Exact culling of FP32 Gaussian tile pairs whose terms are all exactly
+    zero (default on); off computes every pair, with bitwise the same
+    results."""
+    _check(lib().lddmm_set_pair_culling(int(bool(on))))
 
 
 # ---------------------------------------------------------------- operators

@@ -45,6 +45,7 @@ struct BatchView {
     int rows, T;
     double h, w;
     double *ecen, *edir, *evol, *dc, *dd, *efwd, *evf;
+    int cull;   // exact culling of FP32 Gaussian tile pairs
 };
 
 template <int NC>
@@ -81,7 +82,8 @@ __global__ __launch_bounds__(kBlock) void kb_kv(BatchView v, int t, const int4 *
     const BatchDesc d = v.desc[it.x];
     const double *q = v.nde + (size_t)v.T * D * d.vo + (size_t)t * D * d.n;
     const double *a = v.alp + (size_t)(v.T - 1) * D * d.vo + (size_t)t * D * d.n;
-    typename Op::Args A{q, q, a, d.n, d.n, (real)d.iw, (real)d.iw2};
+    typename Op::Args A{q,        q, a, d.n, d.n, (real)d.iw, (real)d.iw2, cull_r2(v.cull != 0),
+                        Op::KF::kappa(d.iw, d.iw2)};
     const int j0 = it.z * d.chunk, j1 = min(d.n, j0 + d.chunk);
     pairs_block<Op>(A, d.n, it.y, j0, j1, v.part + d.po + (size_t)it.z * D * d.n, smem);
 }
@@ -96,7 +98,9 @@ __global__ __launch_bounds__(kBlock) void kb_bwd(BatchView v, int t, const int4 
     const BatchDesc d = v.desc[it.x];
     const double *q = v.nde + (size_t)v.T * D * d.vo + (size_t)t * D * d.n;
     const double *a = v.alp + (size_t)(v.T - 1) * D * d.vo + (size_t)t * D * d.n;
-    typename Op::Args A{q, a, v.padj + (size_t)D * d.vo, d.n, (real)d.iw, (real)d.iw2};
+    const double kap = Op::KF::kappa(d.iw, d.iw2);
+    typename Op::Args A{q,   a, v.padj + (size_t)D * d.vo, d.n, (real)d.iw, (real)d.iw2, cull_r2(v.cull != 0),
+                        kap, Op::KF::cmul(d.iw, d.iw2) / kap};
     const int j0 = it.z * d.chunk, j1 = min(d.n, j0 + d.chunk);
     pairs_block<Op>(A, d.n, it.y, j0, j1, v.part + d.po + (size_t)it.z * 2 * D * d.n, smem);
 }
@@ -205,6 +209,8 @@ __global__ __launch_bounds__(kBlock) void kb_vf(BatchView v, double ics2, double
     A.nj = tgt ? d.nt : d.ne;
     A.ics2 = (real)ics2;
     A.ids2 = (real)ids2;
+    A.cut2 = cull_r2(v.cull != 0);
+    A.kap = cen_kappa<CEN>(ics2);
     const int j0 = (tgt ? it.z - d.vsd : it.z) * d.vchunk, j1 = min(A.nj, j0 + d.vchunk);
     pairs_block<Op>(A, d.ne, it.y, j0, j1, v.vpart + d.vpo + (size_t)it.z * (2 * D + 2) * d.ne, smem);
 }
@@ -859,8 +865,9 @@ void Batch::eval(const double *alp, double *grd, double *f_dev, const std::vecto
         launch_eval(alp, grd, f_dev);
         return;
     }
+    const int cull = pair_culling() ? 1 : 0;
     for (const GraphEntry &e : graphs_)
-        if (e.alp == alp && e.grd == grd && e.f == f_dev) {
+        if (e.alp == alp && e.grd == grd && e.f == f_dev && e.cull == cull) {
             LDDMM_HIP(hipGraphLaunch(e.exec, st_));
             return;
         }
@@ -878,7 +885,7 @@ void Batch::eval(const double *alp, double *grd, double *f_dev, const std::vecto
     const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     LDDMM_HIP(ie);
-    graphs_.push_back(GraphEntry{alp, grd, f_dev, exec});
+    graphs_.push_back(GraphEntry{alp, grd, f_dev, cull, exec});
     LDDMM_HIP(hipGraphLaunch(exec, st_));
 }
 
@@ -900,7 +907,7 @@ void Batch::launch_eval(const double *alp, double *grd, double *f_dev)
     const int D = spec_.dim, T = spec_.T;
     const Tables &tb = *tab_;
     BatchView v{desc_, nde_, padj_, part_, vpart_, alp, grd, q0_, tc_, tu_, ta_, elm_, ifo_, rows_, T,
-                spec_.h, spec_.w, ecen_, edir_, evol_, dc_, dd_, efwd_, evf_};
+                spec_.h, spec_.w, ecen_, edir_, evol_, dc_, dd_, efwd_, evf_, pair_culling() ? 1 : 0};
 
     // q_0 of the listed problems
     if (D == 2) kb_init_flow<2><<<tb.nvtile, kTile, 0, st_>>>(v, tb.vtile.p);

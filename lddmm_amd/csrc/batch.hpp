@@ -117,6 +117,7 @@ private:
     struct GraphEntry {
         const double *alp;
         double *grd, *f;
+        int cull;   // pair_culling() at capture
         hipGraphExec_t exec;
     };
     std::vector<GraphEntry> graphs_;

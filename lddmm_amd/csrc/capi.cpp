@@ -1018,6 +1018,12 @@ int lddmm_spatial_order(int32_t *perm, const double *ptsMat, int ptsNum, int dim
     });
 }
 
+int lddmm_set_pair_culling(int on)
+{
+    set_pair_culling(on != 0);
+    return 0;
+}
+
 int lddmm_get_elm_ifo(int32_t *elmIfoMat, const int32_t *elmVtxMat, int vtxPerElm, int elmNum, int ndeNum)
 {
     return guarded([&] {

@@ -8,6 +8,11 @@
 namespace lddmm {
 
 // ---- pair sums over points (pair_kernels.hip) ---------------------------
+// Exact culling of FP32 Gaussian tile pairs (pair_driver.hpp); on by
+// default, read at every launch (a batch keys its captured graphs on it).
+// Off gives bitwise the same results at the full pair count.
+void set_pair_culling(bool on);
+bool pair_culling();
 // Split plans for the point kernels: plan_kv for K.a and plan_bwd for the
 // fused backward pass (two targets per lane in FP32), plan_single for the dq
 // kernels (one target per lane).

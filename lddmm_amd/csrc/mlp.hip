@@ -178,6 +178,8 @@ struct OpMoments {
     static constexpr int D = D_, F = D_ + 1, OUT = 1 + 2 * D_, R = 1;
     using real = double;
     using lds_t = double;
+    using crd_t = double;
+    static constexpr bool kCull = false;
     struct Args {
         const double *x, *nrm;
         int ni;
@@ -186,7 +188,7 @@ struct OpMoments {
         double is2, is;
     };
     struct Tgt {
-        double x[D], n[D];
+        double n[D];
     };
     struct Acc {
         double m0, m1[D], mu[D];
@@ -200,11 +202,11 @@ struct OpMoments {
         }
         f[D] = A.a[j];
     }
-    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t)
+    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t, double (&x)[D])
     {
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-            t.x[c] = i >= 0 ? A.x[(size_t)c * A.ni + i] : 0.0;
+            x[c] = i >= 0 ? A.x[(size_t)c * A.ni + i] : 0.0;
             t.n[c] = i >= 0 ? A.nrm[(size_t)c * A.ni + i] : 0.0;
         }
     }
@@ -214,13 +216,13 @@ struct OpMoments {
 #pragma unroll
         for (int c = 0; c < D; ++c) a.m1[c] = a.mu[c] = 0.0;
     }
-    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const double (&xj)[D],
-                                                const lds_t (&fj)[F], Acc &acc)
+    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const crd_t (&xi)[D],
+                                                const crd_t (&xj)[D], const lds_t (&fj)[F], Acc &acc)
     {
         double df[D], r2 = 0.0, un = 0.0;
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-            df[c] = xj[c] - t.x[c];
+            df[c] = xj[c] - xi[c];
             r2 += df[c] * df[c];
             un += fj[c] * t.n[c];
         }
@@ -232,7 +234,7 @@ struct OpMoments {
             acc.mu[c] += w * un * fj[c];
         }
     }
-    __device__ __forceinline__ static void flush(const Acc &a, double (&tot)[OUT])
+    __device__ __forceinline__ static void flush(const Args &, const Acc &a, double (&tot)[OUT])
     {
         tot[0] += a.m0;
 #pragma unroll

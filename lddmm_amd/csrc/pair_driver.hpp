@@ -4,12 +4,12 @@
 // every wave holds targets base + l + 64 r, r < R) and one chunk of the
 // sources.  The chunk is staged through LDS 256 sources at a time, one
 // 16-byte aligned record per source: the per-source fields in the policy's
-// storage type, then the coordinates as FP64, so one base address and
+// storage type, then the coordinates (crd_t, below), so one base address and
 // immediate offsets reach every field (wide ds_read_b128/b64 loads, no
-// per-field address arithmetic in the pair loop).  Wave w consumes sources
-// t = w, w+4, ... of each tile, so all 64 lanes of a wave read the same LDS
-// words (broadcast, conflict-free) and every source is read once per wave
-// for R targets.  Per tile, sums are
+// per-field address arithmetic in the pair loop).  Wave w consumes the w-th
+// contiguous quarter of each tile, so all 64 lanes of a wave read the same
+// LDS words (broadcast, conflict-free) and every source is read once per
+// wave for R targets.  Per tile, sums are
 // carried in the policy's arithmetic type and flushed to FP64 totals
 // (two-level accumulation: at most 64 pairs are summed in FP32 before the
 // FP64 add); at the end the four waves' totals are combined in LDS in a
@@ -17,24 +17,51 @@
 // target), which a following kernel reduces over the splits in a fixed
 // order.  No atomics: results are bitwise repeatable.
 //
+// Coordinates reach the pair term as `crd_t`.  FP64 operators use the FP64
+// positions.  FP32 operators use FP32 offsets from an anchor a, the FP64
+// centroid of the block's targets: targets are converted once, sources
+// once per block when they are staged, so a pair forms its D differences
+// with D FP32 subtractions (instead of D FP64 subtractions and D
+// conversions).  The rounding error of a difference is then
+// 2^-25 (|x_i - a| + |x_j - a| + |x_i - x_j|) per component instead of
+// 2^-25 |x_i - x_j|; with the spatially ordered problems of Objective and
+// Batch (compact 64/128-point tiles) |x_i - a| is at most a tile radius.
+// The offsets are scaled by the kernel's kappa (KernelFnT) in FP64 before
+// the rounding, which changes none of these bounds.
+//
 // An operator Op supplies
 //   D, F, OUT, R                       dimensions, LDS fields, outputs, targets/lane
 //   Args                               kernel arguments
-//   lds_t, Tgt, Acc                    LDS field type, per-target registers, per-tile sums
-//   load_src(args, j, x[D], f[F])      source j's coordinates and fields
-//   load_tgt(args, i, Tgt&)            target i (i < 0: padding lane)
-//   pair(args, Tgt&, xj[D], fj[F], Acc&)    one pair term
-//   zero(Acc&), flush(const Acc&, double tot[OUT])
+//   lds_t, crd_t, Tgt, Acc             LDS field type, coordinate type, per-target
+//                                      registers, per-tile sums
+//   load_src(args, j, x[D], f[F])      source j's FP64 coordinates and fields
+//   load_tgt(args, i, Tgt&, x[D])      target i's registers and FP64 coordinates
+//                                      (i < 0: padding lane, coordinates 0)
+//   pair(args, Tgt&, xi[D], xj[D], fj[F], Acc&)   one pair term
+//   zero(Acc&), flush(args, const Acc&, double tot[OUT])
+//   crd_scale(args)                    FP32 only: kappa of the staged offsets
+//   kCull, args.cut2                   exact culling (below)
+//
+// Exact culling (FP32 Gaussian kernels, kCull): a kernel value
+// exp2(-r2) (scaled coordinates, KernelFnT) is exactly 0 in FP32 once the
+// exponent is below -160, and then every term of the pair is +-0 and leaves the tile sums
+// bitwise unchanged.  Per block the driver takes the target tile's radius
+// about the anchor (rt) and per wave's range of staged sources the smallest
+// distance of a source from the anchor (ms); when ms - rt exceeds that range (with a
+// 1e-4 relative margin for the FP32 rounding of both bounds and of r2:
+// args.cut2 = 160 (1 + 1e-4)), no pair of the two tiles can
+// reach it and the wave skips its range.  With the spatially ordered problems
+// this removes the far tiles of narrow kernels (the varifold's Gaussian
+// of width 0.1 on unit-sized shapes skips most of its pairs).
 #pragma once
 
 #include "common.hpp"
 
 namespace lddmm {
 
-// Precision policies.  Coordinates are always FP64 in HBM and LDS; each
-// coordinate difference is formed in FP64 and then rounded to the
-// arithmetic type, so FP32 pair terms see |q_i - q_j| to FP32 relative
-// accuracy instead of |q| to FP32 accuracy.
+// Precision policies.  Coordinates are FP64 in HBM; the pair arithmetic
+// and the coordinates staged in LDS are in `real` (FP32 as anchored
+// offsets, above).
 struct PrecF64 {
     using real = double;
     static constexpr bool fp32 = false;
@@ -44,90 +71,119 @@ struct PrecF32 {
     static constexpr bool fp32 = true;
 };
 
-// Kernel family in the policy's arithmetic: value k and factor c of
-// grad_{q_i} k = c (q_i - q_j), from the squared distance.  D selects the
-// 2D Matern family (Bessel factors, evaluated in FP64 under both policies).
+// Kernel family in the policy's arithmetic.  The pair terms see the squared
+// distance r2 of coordinates scaled by kappa(iw, iw2) (the driver scales the
+// staged FP32 offsets; FP64 coordinates are unscaled, kappa = 1), and
+//   value(r2, iw, iw2) = k,   eval(r2, iw, iw2, k, cf)
+// with the gradient factor c of grad_{q_i} k = c (q_i - q_j) equal to
+// cmul(iw, iw2) * cf.  Operators carry cmul / kappa into the FP64 flush of
+// their sums of cf (q_i - q_j)_scaled, so neither the width nor the scale
+// costs an instruction per pair.  D selects the 2D Matern family (Bessel
+// factors).
 template <int ORDER, typename real, int D = 3>
 struct KernelFnT;
 
+// FP64: the reference's formulas (kappa = cmul = 1)
 template <int ORDER, int D>
 struct KernelFnT<ORDER, double, D> {
+    __host__ __device__ static double kappa(double, double) { return 1.0; }
+    __host__ __device__ static double cmul(double, double) { return 1.0; }
     __device__ __forceinline__ static double value(double r2, double iw, double iw2)
     {
         return KernelFnD<D, ORDER>::value(r2, iw, iw2);
     }
-    __device__ __forceinline__ static void eval(double r2, double iw, double iw2, double &k, double &c)
+    __device__ __forceinline__ static void eval(double r2, double iw, double iw2, double &k, double &cf)
     {
-        KernelFnD<D, ORDER>::eval(r2, iw, iw2, k, c);
+        KernelFnD<D, ORDER>::eval(r2, iw, iw2, k, cf);
     }
 };
 
-// FP32: exp via v_exp_f32 on a base-2 argument, sqrt via v_sqrt_f32 (both
-// ~1 ulp).  Every width-dependent factor is written as a product of
-// per-launch constants (-iw2 log2(e), -2 iw2, the Matern polynomial
-// coefficients divided by their leading constant) so the compiler hoists
-// them out of the pair loop: the Gaussian costs one multiply and one
-// v_exp_f32 per pair for k, one more multiply for c.
-// 2D Matern under the FP32 policy: the Bessel factors and the polynomial in
-// FP64 from the FP32 squared distance, rounded to FP32.
+// FP32, 2D Matern: the Bessel factors and the polynomial in FP64 from the
+// FP32 squared distance (unscaled), rounded to FP32.
 template <int ORDER>
 struct KernelFnT<ORDER, float, 2> {
+    __host__ __device__ static double kappa(double, double) { return 1.0; }
+    __host__ __device__ static double cmul(double, double) { return 1.0; }
     __device__ __forceinline__ static float value(float r2, float iw, float iw2)
     {
         return (float)KernelFnD<2, ORDER>::value((double)r2, (double)iw, (double)iw2);
     }
-    __device__ __forceinline__ static void eval(float r2, float iw, float iw2, float &k, float &c)
+    __device__ __forceinline__ static void eval(float r2, float iw, float iw2, float &k, float &cf)
     {
         double kd, cd;
         KernelFnD<2, ORDER>::eval((double)r2, (double)iw, (double)iw2, kd, cd);
         k = (float)kd;
-        c = (float)cd;
+        cf = (float)cd;
     }
 };
+
+// FP32, Gaussian and 3D Matern: exp via v_exp_f32 on a base-2 argument,
+// sqrt via v_sqrt_f32 (both ~1 ulp).  Gaussian: kappa^2 = iw2 log2 e, so
+// k = exp2(-r2) and cf = k (cmul = -2 iw2).  Matern order p:
+// kappa = iw log2 e, s = sqrt(r2) = d log2 e with d = |q_i - q_j| iw,
+// e = exp2(-s) = e^-d, k = P_p(d) e and cf = P_{p-1}(d) e with the
+// polynomials of Cpp/src/computeKernel.cpp:170-234 and
+// Cpp/src/dqKernel.cpp:215-291 written in s (coefficients times ln 2^m).
 template <int ORDER>
 struct KernelFnT<ORDER, float, 3> {
-    static constexpr float kLog2e = 1.4426950408889634f;
-    __device__ __forceinline__ static float value(float r2, float iw, float iw2)
+    static constexpr double kLog2e = 1.4426950408889634, kLn2 = 0.6931471805599453;
+    static constexpr float L1 = (float)kLn2, L2 = (float)(kLn2 * kLn2), L3 = (float)(kLn2 * kLn2 * kLn2),
+                           L4 = (float)(kLn2 * kLn2 * kLn2 * kLn2);
+    __host__ __device__ static double kappa(double iw, double iw2)
     {
-        if constexpr (ORDER == -1) {
-            return __builtin_amdgcn_exp2f(r2 * (iw2 * -kLog2e));
-        } else {
-            const float s = __builtin_amdgcn_sqrtf(r2);
-            const float d = s * iw;
-            const float e = __builtin_amdgcn_exp2f(s * (iw * -kLog2e));
-            if constexpr (ORDER == 0) return e;
-            else if constexpr (ORDER == 1) return (1.0f + d) * e;
-            else if constexpr (ORDER == 2) return (1.0f + d * (1.0f + d * (1.0f / 3.0f))) * e;
-            else if constexpr (ORDER == 3)
-                return (1.0f + d * (1.0f + d * (6.0f / 15.0f + d * (1.0f / 15.0f)))) * e;
-            else
-                return (1.0f + d * (1.0f + d * (45.0f / 105.0f + d * (10.0f / 105.0f + d * (1.0f / 105.0f))))) * e;
+        return ORDER == -1 ? sqrt(iw2 * kLog2e) : iw * kLog2e;
+    }
+    __host__ __device__ static double cmul(double, double iw2)
+    {
+        switch (ORDER) {
+        case -1: return -2.0 * iw2;
+        case 0: return 0.0;   // order 0 has no dq (Cpp/src/dqKernel.cpp:39)
+        case 1: return -iw2;
+        case 2: return -iw2 / 3.0;
+        case 3: return -iw2 * 3.0 / 15.0;
+        default: return -iw2 * 15.0 / 105.0;
         }
     }
-    __device__ __forceinline__ static void eval(float r2, float iw, float iw2, float &k, float &c)
+    // P_1 .. P_4 of d = s ln 2
+    __device__ __forceinline__ static float poly(int p, float s)
+    {
+        switch (p) {
+        case 1: return 1.0f + s * L1;
+        case 2: return 1.0f + s * (L1 + s * (L2 * (1.0f / 3.0f)));
+        case 3: return 1.0f + s * (L1 + s * (L2 * (6.0f / 15.0f) + s * (L3 * (1.0f / 15.0f))));
+        default:
+            return 1.0f +
+                   s * (L1 + s * (L2 * (45.0f / 105.0f) + s * (L3 * (10.0f / 105.0f) + s * (L4 * (1.0f / 105.0f)))));
+        }
+    }
+    __device__ __forceinline__ static float value(float r2, float, float)
     {
         if constexpr (ORDER == -1) {
-            k = __builtin_amdgcn_exp2f(r2 * (iw2 * -kLog2e));
-            c = k * (-2.0f * iw2);
+            return __builtin_amdgcn_exp2f(-r2);
         } else {
             const float s = __builtin_amdgcn_sqrtf(r2);
-            const float d = s * iw;
-            const float e = __builtin_amdgcn_exp2f(s * (iw * -kLog2e));
+            const float e = __builtin_amdgcn_exp2f(-s);
+            if constexpr (ORDER == 0) return e;
+            else return poly(ORDER, s) * e;
+        }
+    }
+    __device__ __forceinline__ static void eval(float r2, float, float, float &k, float &cf)
+    {
+        if constexpr (ORDER == -1) {
+            k = __builtin_amdgcn_exp2f(-r2);
+            cf = k;
+        } else {
+            const float s = __builtin_amdgcn_sqrtf(r2);
+            const float e = __builtin_amdgcn_exp2f(-s);
             if constexpr (ORDER == 0) {
                 k = e;
-                c = 0.0f;
+                cf = 0.0f;
             } else if constexpr (ORDER == 1) {
-                k = (1.0f + d) * e;
-                c = e * -iw2;
-            } else if constexpr (ORDER == 2) {
-                k = (1.0f + d * (1.0f + d * (1.0f / 3.0f))) * e;
-                c = (1.0f + d) * e * (iw2 * (-1.0f / 3.0f));
-            } else if constexpr (ORDER == 3) {
-                k = (1.0f + d * (1.0f + d * (6.0f / 15.0f + d * (1.0f / 15.0f)))) * e;
-                c = (1.0f + d * (1.0f + d * (1.0f / 3.0f))) * e * (iw2 * (-3.0f / 15.0f));
+                k = poly(1, s) * e;
+                cf = e;
             } else {
-                k = (1.0f + d * (1.0f + d * (45.0f / 105.0f + d * (10.0f / 105.0f + d * (1.0f / 105.0f))))) * e;
-                c = (1.0f + d * (1.0f + d * (6.0f / 15.0f + d * (1.0f / 15.0f)))) * e * (iw2 * (-15.0f / 105.0f));
+                k = poly(ORDER, s) * e;
+                cf = poly(ORDER - 1, s) * e;
             }
         }
     }
@@ -140,40 +196,103 @@ struct KernelFnT<-1, float, 2> : KernelFnT<-1, float, 3> {
 
 constexpr int kWaves = kBlock / 64;
 
+// args.cut2 of a culling operator (FP32 Gaussian on coordinates scaled so
+// that k = exp2(-r2)): the exponent below which the FP32 exp2 is exactly 0,
+// with a 1e-4 relative margin; +inf turns culling off
+__host__ __device__ inline float cull_r2(bool on) { return on ? (float)(160.0 * (1.0 + 1e-4)) : __builtin_inff(); }
+
+__device__ __forceinline__ float wave_max(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ float wave_min(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
 // LDS bytes of one block: the staging tiles and the final wave reduction
 // share one buffer.
 template <class Op>
 struct PairSmem {
     // record of one source: fields at 0, coordinates at `xoff`
+    static constexpr size_t csize = sizeof(typename Op::crd_t);
     static constexpr size_t fbytes = sizeof(typename Op::lds_t) * (Op::F > 0 ? Op::F : 1);
-    static constexpr size_t xoff = (fbytes + 7) & ~size_t(7);
-    static constexpr size_t rec = (xoff + sizeof(double) * Op::D + 15) & ~size_t(15);
+    static constexpr size_t xoff = (fbytes + csize - 1) & ~(csize - 1);
+    static constexpr size_t rec = (xoff + csize * Op::D + 15) & ~size_t(15);
     static constexpr size_t stage = rec * kTileJ;
+    static constexpr size_t cull = Op::kCull ? sizeof(float) * kTileJ : 0;   // source distances
     static constexpr size_t red = sizeof(double) * kWaves * Op::OUT * 64 * Op::R;
-    static constexpr size_t bytes = stage > red ? stage : red;
+    static constexpr size_t bytes = stage + cull > red ? stage + cull : red;
 };
 
 // One block's work: targets [tile*TI, tile*TI + TI) against sources
-// [j0, j1); writes out[c*ni + i] for its targets.
+// [j0, j1); writes out[c*ni + i] for its targets.  Each staged tile of
+// sources is split into four contiguous ranges, one per wave (with the
+// spatial order a range is a compact patch, which the culling test uses).
 template <class Op>
 __device__ __forceinline__ void pairs_block(const typename Op::Args &args, int ni, int tile, int j0, int j1,
                                             double *__restrict__ out, char *smem)
 {
     constexpr int D = Op::D, F = Op::F, OUT = Op::OUT, R = Op::R, TI = 64 * R;
     using lds_t = typename Op::lds_t;
+    using crd_t = typename Op::crd_t;
+    constexpr bool kAnchor = sizeof(crd_t) < sizeof(double);
     using SM = PairSmem<Op>;
     auto red = reinterpret_cast<double (*)[OUT][TI]>(smem);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the wave index is uniform: as a scalar it keeps the source loop's
+    // counter and bounds on the scalar unit
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
     typename Op::Tgt tg[R];
-    double tot[R][OUT];
+    double xt[R][D];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int i = tile * TI + r * 64 + lane;
-        Op::load_tgt(args, i < ni ? i : -1, tg[r]);
+        Op::load_tgt(args, i < ni ? i : -1, tg[r], xt[r]);   // padding lanes: coordinates 0
+    }
+    // anchor of the FP32 coordinates: the centroid of the tile's targets
+    // (every wave holds all of them and forms the same sum)
+    double anc[D];
+    if constexpr (kAnchor) {
+        const double inv = 1.0 / (double)min(TI, ni - tile * TI);
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            double v = 0.0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v += xt[r][c];
+            anc[c] = wave_sum(v) * inv;
+        }
+    }
+    double kap = 1.0;
+    if constexpr (kAnchor) kap = Op::crd_scale(args);
+    auto to_crd = [&](double v, int c) -> crd_t {
+        if constexpr (kAnchor) return (crd_t)((v - anc[c]) * kap);
+        else return v;
+    };
+
+    crd_t xi[R][D];
+    double tot[R][OUT];
+    float rt2 = 0.0f;   // squared target tile radius about the anchor (culling)
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        float d2 = 0.0f;
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            xi[r][c] = to_crd(xt[r][c], c);
+            d2 += (float)xi[r][c] * (float)xi[r][c];
+        }
+        if (tile * TI + r * 64 + lane < ni) rt2 = fmaxf(rt2, d2);
 #pragma unroll
         for (int c = 0; c < OUT; ++c) tot[r][c] = 0.0;
     }
+    float rt = 0.0f;
+    if constexpr (Op::kCull) rt = __builtin_sqrtf(wave_max(rt2));
+    float *s2buf = reinterpret_cast<float *>(smem + SM::stage);   // squared source distances to the anchor
 
     for (int jt = j0; jt < j1; jt += kTileJ) {
         const int cnt = min(kTileJ, j1 - jt);
@@ -185,27 +304,41 @@ __device__ __forceinline__ void pairs_block(const typename Op::Args &args, int n
             char *rec = smem + SM::rec * threadIdx.x;
 #pragma unroll
             for (int c = 0; c < F; ++c) reinterpret_cast<lds_t *>(rec)[c] = f[c];
+            float s2 = 0.0f;
 #pragma unroll
-            for (int c = 0; c < D; ++c) reinterpret_cast<double *>(rec + SM::xoff)[c] = x[c];
+            for (int c = 0; c < D; ++c) {
+                const crd_t v = to_crd(x[c], c);
+                reinterpret_cast<crd_t *>(rec + SM::xoff)[c] = v;
+                s2 += (float)v * (float)v;
+            }
+            if constexpr (Op::kCull) s2buf[threadIdx.x] = s2;
         }
         __syncthreads();
+        const int share = (cnt + kWaves - 1) / kWaves;   // <= 64
+        const int t0 = wave * share, t1 = min(t0 + share, cnt);
+        if (t0 >= t1) continue;
+        if constexpr (Op::kCull) {
+            const float ms2 = wave_min(t0 + lane < t1 ? s2buf[t0 + lane] : __builtin_inff());
+            const float gap = __builtin_sqrtf(ms2) - rt;
+            if (__builtin_amdgcn_readfirstlane((int)(gap > 0.0f && gap * gap > (float)args.cut2))) continue;
+        }
         typename Op::Acc acc[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) Op::zero(acc[r]);
 #pragma unroll 2
-        for (int t = wave; t < cnt; t += kWaves) {
-            double xj[D];
+        for (int t = t0; t < t1; ++t) {
+            crd_t xj[D];
             lds_t fj[F > 0 ? F : 1];
             const char *rec = smem + SM::rec * t;
 #pragma unroll
             for (int c = 0; c < F; ++c) fj[c] = reinterpret_cast<const lds_t *>(rec)[c];
 #pragma unroll
-            for (int c = 0; c < D; ++c) xj[c] = reinterpret_cast<const double *>(rec + SM::xoff)[c];
+            for (int c = 0; c < D; ++c) xj[c] = reinterpret_cast<const crd_t *>(rec + SM::xoff)[c];
 #pragma unroll
-            for (int r = 0; r < R; ++r) Op::pair(args, tg[r], xj, fj, acc[r]);
+            for (int r = 0; r < R; ++r) Op::pair(args, tg[r], xi[r], xj, fj, acc[r]);
         }
 #pragma unroll
-        for (int r = 0; r < R; ++r) Op::flush(acc[r], tot[r]);
+        for (int r = 0; r < R; ++r) Op::flush(args, acc[r], tot[r]);
     }
 
     // fixed-order combination of the four waves' totals

@@ -6,6 +6,8 @@
 // (Cpp/src/multiplyKernel.cpp:22-564), dqKernel (Cpp/src/dqKernel.cpp:31-591),
 // computeKernel (Cpp/src/computeKernel.cpp:21-470) and the Euler / adjoint
 // steps of objgrd (Cpp/src/objgrd.cpp:45-87).
+#include <atomic>
+
 #include "kernels.hpp"
 #include "pair_driver.hpp"
 #include "pair_ops.hpp"
@@ -161,6 +163,13 @@ PairSplit plan_bwd(int ni, int nj, int prec, int target_blocks)
 
 PairSplit plan_single(int ni, int nj, int target_blocks) { return make_split_ti(ni, nj, 64, target_blocks); }
 
+namespace {
+std::atomic<bool> g_pair_culling{true};
+}
+
+void set_pair_culling(bool on) { g_pair_culling.store(on); }
+bool pair_culling() { return g_pair_culling.load(); }
+
 void launch_kv(int dim, int order, int prec, const double *qi, int ni, const double *qj, const double *a, int nj,
                double width, const PairSplit &sp, double *part, hipStream_t st)
 {
@@ -169,7 +178,8 @@ void launch_kv(int dim, int order, int prec, const double *qi, int ni, const dou
         constexpr int R = decltype(Pc)::fp32 ? 2 : 1;
         using Op = OpKV<Dc(), Oc(), decltype(Pc), R>;
         using real = typename Op::real;
-        typename Op::Args A{qi, qj, a, ni, nj, (real)(1.0 / width), (real)(1.0 / (width * width))};
+        const double iw = 1.0 / width, iw2 = 1.0 / (width * width);
+        typename Op::Args A{qi, qj, a, ni, nj, (real)iw, (real)iw2, cull_r2(pair_culling()), Op::KF::kappa(iw, iw2)};
         launch_pairs<Op>(A, ni, nj, sp, 0, part, st);
     });
 }
@@ -201,7 +211,9 @@ void launch_bwd_pairs(int dim, int order, int prec, const double *q, const doubl
         constexpr int R = decltype(Pc)::fp32 ? 2 : 1;
         using Op = OpBwd<Dc(), Oc(), decltype(Pc), R>;
         using real = typename Op::real;
-        typename Op::Args A{q, a, p, n, (real)(1.0 / width), (real)(1.0 / (width * width))};
+        const double iw = 1.0 / width, iw2 = 1.0 / (width * width), kap = Op::KF::kappa(iw, iw2);
+        typename Op::Args A{q,   a,         p, n, (real)iw, (real)iw2, cull_r2(pair_culling()),
+                            kap, Op::KF::cmul(iw, iw2) / kap};
         launch_pairs<Op>(A, n, n, sp, 0, part, st);
     });
 }
@@ -221,7 +233,8 @@ void launch_dq_self(int dim, int order, int prec, const double *q, const double 
     dispatch_all(dim, order, prec, [&](auto Dc, auto Oc, auto Pc) {
         using Op = OpDqSelf<Dc(), Oc(), decltype(Pc)>;
         using real = typename Op::real;
-        typename Op::Args A{q, l, r, n, (real)(1.0 / width), (real)(1.0 / (width * width))};
+        const double iw = 1.0 / width, iw2 = 1.0 / (width * width), kap = Op::KF::kappa(iw, iw2);
+        typename Op::Args A{q, l, r, n, (real)iw, (real)iw2, kap, Op::KF::cmul(iw, iw2) / kap};
         launch_pairs<Op>(A, n, n, sp, 0, part, st);
     });
 }
@@ -233,7 +246,8 @@ void launch_dq_cross(int dim, int order, int prec, const double *qi, const doubl
     dispatch_all(dim, order, prec, [&](auto Dc, auto Oc, auto Pc) {
         using Op = OpDqCross<Dc(), Oc(), decltype(Pc)>;
         using real = typename Op::real;
-        typename Op::Args A{qi, u, qj, v, ni, nj, (real)(1.0 / width), (real)(1.0 / (width * width))};
+        const double iw = 1.0 / width, iw2 = 1.0 / (width * width), kap = Op::KF::kappa(iw, iw2);
+        typename Op::Args A{qi, u, qj, v, ni, nj, (real)iw, (real)iw2, kap, Op::KF::cmul(iw, iw2) / kap};
         launch_pairs<Op>(A, ni, nj, sp, 0, part, st);
     });
 }

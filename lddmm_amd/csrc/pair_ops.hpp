@@ -18,13 +18,18 @@ struct OpKV {
     static constexpr int D = D_, F = D_, OUT = D_, R = R_;
     using real = typename P::real;
     using lds_t = real;
+    using crd_t = real;
+    static constexpr bool kCull = P::fp32 && ORDER == -1;
+    using KF = KernelFnT<ORDER, real, D_>;
     struct Args {
         const double *qi, *qj, *a;
         int ni, nj;
         real iw, iw2;
+        float cut2;   // culling range (cull_r2); +inf: none
+        double kap;   // KF::kappa
     };
+    __device__ __forceinline__ static double crd_scale(const Args &A) { return A.kap; }
     struct Tgt {
-        double x[D];
     };
     struct Acc {
         real v[D];
@@ -37,30 +42,30 @@ struct OpKV {
             f[c] = (real)A.a[(size_t)c * A.nj + j];
         }
     }
-    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t)
+    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &, double (&x)[D])
     {
 #pragma unroll
-        for (int c = 0; c < D; ++c) t.x[c] = i >= 0 ? A.qi[(size_t)c * A.ni + i] : 0.0;
+        for (int c = 0; c < D; ++c) x[c] = i >= 0 ? A.qi[(size_t)c * A.ni + i] : 0.0;
     }
     __device__ __forceinline__ static void zero(Acc &a)
     {
 #pragma unroll
         for (int c = 0; c < D; ++c) a.v[c] = real(0);
     }
-    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const double (&xj)[D],
-                                                const lds_t (&fj)[F], Acc &acc)
+    __device__ __forceinline__ static void pair(const Args &A, const Tgt &, const crd_t (&xi)[D],
+                                                const crd_t (&xj)[D], const lds_t (&fj)[F], Acc &acc)
     {
         real r2 = real(0);
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-            real df = (real)(t.x[c] - xj[c]);
+            real df = xi[c] - xj[c];
             r2 += df * df;
         }
-        real k = KernelFnT<ORDER, real, D>::value(r2, A.iw, A.iw2);
+        real k = KF::value(r2, A.iw, A.iw2);
 #pragma unroll
         for (int c = 0; c < D; ++c) acc.v[c] += k * fj[c];
     }
-    __device__ __forceinline__ static void flush(const Acc &a, double (&tot)[OUT])
+    __device__ __forceinline__ static void flush(const Args &, const Acc &a, double (&tot)[OUT])
     {
 #pragma unroll
         for (int c = 0; c < D; ++c) tot[c] += (double)a.v[c];
@@ -78,13 +83,18 @@ struct OpBwd {
     static constexpr int D = D_, F = 3 * D_, OUT = 2 * D_, R = R_;
     using real = typename P::real;
     using lds_t = real;
+    using crd_t = real;
+    static constexpr bool kCull = P::fp32 && ORDER == -1;
+    using KF = KernelFnT<ORDER, real, D_>;
     struct Args {
         const double *q, *a, *p;
         int n;
         real iw, iw2;
+        float cut2;        // culling range (cull_r2); +inf: none
+        double kap, pds;   // KF::kappa; dq flush factor KF::cmul / kappa
     };
+    __device__ __forceinline__ static double crd_scale(const Args &A) { return A.kap; }
     struct Tgt {
-        double x[D];
         real a[D], l[D];
     };
     struct Acc {
@@ -102,13 +112,13 @@ struct OpBwd {
             f[2 * D + c] = (real)(amp - 0.5 * aj);
         }
     }
-    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t)
+    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t, double (&x)[D])
     {
 #pragma unroll
         for (int c = 0; c < D; ++c) {
             const size_t o = (size_t)c * A.n + (i >= 0 ? i : 0);
             double ai = i >= 0 ? A.a[o] : 0.0, amp = ai - (i >= 0 ? A.p[o] : 0.0);
-            t.x[c] = i >= 0 ? A.q[o] : 0.0;
+            x[c] = i >= 0 ? A.q[o] : 0.0;
             t.a[c] = (real)ai;
             t.l[c] = (real)(amp - 0.5 * ai);
         }
@@ -118,35 +128,36 @@ struct OpBwd {
 #pragma unroll
         for (int c = 0; c < D; ++c) a.g[c] = a.pd[c] = real(0);
     }
-    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const double (&xj)[D],
-                                                const lds_t (&fj)[F], Acc &acc)
+    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const crd_t (&xi)[D],
+                                                const crd_t (&xj)[D], const lds_t (&fj)[F], Acc &acc)
     {
         real df[D], r2 = real(0);
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-            df[c] = (real)(t.x[c] - xj[c]);
+            df[c] = xi[c] - xj[c];
             r2 += df[c] * df[c];
         }
-        real k, cc;
-        KernelFnT<ORDER, real, D>::eval(r2, A.iw, A.iw2, k, cc);
+        real k, cf;
+        KF::eval(r2, A.iw, A.iw2, k, cf);
         real lr = real(0);
 #pragma unroll
         for (int c = 0; c < D; ++c) lr += t.l[c] * fj[c];
 #pragma unroll
         for (int c = 0; c < D; ++c) lr += fj[2 * D + c] * t.a[c];
-        const real f = lr * cc;
+        const real f = lr * cf;
 #pragma unroll
         for (int c = 0; c < D; ++c) {
             acc.g[c] += k * fj[D + c];
             acc.pd[c] += f * df[c];
         }
     }
-    __device__ __forceinline__ static void flush(const Acc &a, double (&tot)[OUT])
+    __device__ __forceinline__ static void flush(const Args &A, const Acc &a, double (&tot)[OUT])
     {
 #pragma unroll
         for (int c = 0; c < D; ++c) {
             tot[c] += (double)a.g[c];
-            tot[D + c] += (double)a.pd[c];
+            if constexpr (P::fp32) tot[D + c] += (double)a.pd[c] * A.pds;
+            else tot[D + c] += a.pd[c];
         }
     }
 };
@@ -159,13 +170,17 @@ struct OpDqSelf {
     static constexpr int D = D_, F = 2 * D_, OUT = D_, R = 1;
     using real = typename P::real;
     using lds_t = real;
+    using crd_t = real;
+    static constexpr bool kCull = false;
+    using KF = KernelFnT<ORDER, real, D_>;
     struct Args {
         const double *q, *l, *r;
         int n;
         real iw, iw2;
+        double kap, pds;   // KF::kappa; flush factor KF::cmul / kappa
     };
+    __device__ __forceinline__ static double crd_scale(const Args &A) { return A.kap; }
     struct Tgt {
-        double x[D];
         real l[D], r[D];
     };
     struct Acc {
@@ -181,12 +196,12 @@ struct OpDqSelf {
             f[D + c] = (real)A.r[o];
         }
     }
-    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t)
+    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t, double (&x)[D])
     {
 #pragma unroll
         for (int c = 0; c < D; ++c) {
             const size_t o = (size_t)c * A.n + (i >= 0 ? i : 0);
-            t.x[c] = i >= 0 ? A.q[o] : 0.0;
+            x[c] = i >= 0 ? A.q[o] : 0.0;
             t.l[c] = i >= 0 ? (real)A.l[o] : real(0);
             t.r[c] = i >= 0 ? (real)A.r[o] : real(0);
         }
@@ -196,30 +211,33 @@ struct OpDqSelf {
 #pragma unroll
         for (int c = 0; c < D; ++c) a.pd[c] = real(0);
     }
-    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const double (&xj)[D],
-                                                const lds_t (&fj)[F], Acc &acc)
+    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const crd_t (&xi)[D],
+                                                const crd_t (&xj)[D], const lds_t (&fj)[F], Acc &acc)
     {
         real df[D], r2 = real(0);
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-            df[c] = (real)(t.x[c] - xj[c]);
+            df[c] = xi[c] - xj[c];
             r2 += df[c] * df[c];
         }
-        real k, cc;
-        KernelFnT<ORDER, real, D>::eval(r2, A.iw, A.iw2, k, cc);
+        real k, cf;
+        KF::eval(r2, A.iw, A.iw2, k, cf);
         real lr = real(0);
 #pragma unroll
         for (int c = 0; c < D; ++c) lr += t.l[c] * fj[D + c];
 #pragma unroll
         for (int c = 0; c < D; ++c) lr += fj[c] * t.r[c];
-        const real f = lr * cc;
+        const real f = lr * cf;
 #pragma unroll
         for (int c = 0; c < D; ++c) acc.pd[c] += f * df[c];
     }
-    __device__ __forceinline__ static void flush(const Acc &a, double (&tot)[OUT])
+    __device__ __forceinline__ static void flush(const Args &A, const Acc &a, double (&tot)[OUT])
     {
 #pragma unroll
-        for (int c = 0; c < D; ++c) tot[c] += (double)a.pd[c];
+        for (int c = 0; c < D; ++c) {
+            if constexpr (P::fp32) tot[c] += (double)a.pd[c] * A.pds;
+            else tot[c] += a.pd[c];
+        }
     }
 };
 
@@ -232,13 +250,17 @@ struct OpDqCross {
     static constexpr int D = D_, F = D_, OUT = D_, R = 1;
     using real = typename P::real;
     using lds_t = real;
+    using crd_t = real;
+    static constexpr bool kCull = false;
+    using KF = KernelFnT<ORDER, real, D_>;
     struct Args {
         const double *qi, *u, *qj, *v;
         int ni, nj;
         real iw, iw2;
+        double kap, pds;   // KF::kappa; flush factor KF::cmul / kappa
     };
+    __device__ __forceinline__ static double crd_scale(const Args &A) { return A.kap; }
     struct Tgt {
-        double x[D];
         real u[D];
     };
     struct Acc {
@@ -252,11 +274,11 @@ struct OpDqCross {
             f[c] = (real)A.v[(size_t)c * A.nj + j];
         }
     }
-    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t)
+    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t, double (&x)[D])
     {
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-            t.x[c] = i >= 0 ? A.qi[(size_t)c * A.ni + i] : 0.0;
+            x[c] = i >= 0 ? A.qi[(size_t)c * A.ni + i] : 0.0;
             t.u[c] = i >= 0 ? (real)A.u[(size_t)c * A.ni + i] : real(0);
         }
     }
@@ -265,28 +287,31 @@ struct OpDqCross {
 #pragma unroll
         for (int c = 0; c < D; ++c) a.pd[c] = real(0);
     }
-    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const double (&xj)[D],
-                                                const lds_t (&fj)[F], Acc &acc)
+    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const crd_t (&xi)[D],
+                                                const crd_t (&xj)[D], const lds_t (&fj)[F], Acc &acc)
     {
         real df[D], r2 = real(0);
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-            df[c] = (real)(t.x[c] - xj[c]);
+            df[c] = xi[c] - xj[c];
             r2 += df[c] * df[c];
         }
-        real k, cc;
-        KernelFnT<ORDER, real, D>::eval(r2, A.iw, A.iw2, k, cc);
+        real k, cf;
+        KF::eval(r2, A.iw, A.iw2, k, cf);
         real lr = real(0);
 #pragma unroll
         for (int c = 0; c < D; ++c) lr += t.u[c] * fj[c];
-        const real f = lr * cc;
+        const real f = lr * cf;
 #pragma unroll
         for (int c = 0; c < D; ++c) acc.pd[c] += f * df[c];
     }
-    __device__ __forceinline__ static void flush(const Acc &a, double (&tot)[OUT])
+    __device__ __forceinline__ static void flush(const Args &A, const Acc &a, double (&tot)[OUT])
     {
 #pragma unroll
-        for (int c = 0; c < D; ++c) tot[c] += (double)a.pd[c];
+        for (int c = 0; c < D; ++c) {
+            if constexpr (P::fp32) tot[c] += (double)a.pd[c] * A.pds;
+            else tot[c] += a.pd[c];
+        }
     }
 };
 

@@ -24,6 +24,22 @@ __device__ __forceinline__ real cen_value(real r2, real inv_cs2)
     }
 }
 
+// FP32 position kernels on coordinates scaled by cen_kappa: 'G'
+// exp2(-r2) (scaled by sqrt(ics2 log2 e)), 'C' 1 / (1 + r2) (scaled by
+// sqrt(ics2)) with v_rcp_f32 (~1 ulp)
+template <int CEN>
+__device__ __forceinline__ float cen_value_scaled(float r2)
+{
+    if constexpr (CEN == kCenG) return __builtin_amdgcn_exp2f(-r2);
+    else return __builtin_amdgcn_rcpf(1.0f + r2);
+}
+
+template <int CEN>
+__host__ __device__ inline double cen_kappa(double ics2)
+{
+    return CEN == kCenG ? sqrt(ics2 * 1.4426950408889634) : sqrt(ics2);
+}
+
 template <typename real>
 __device__ __forceinline__ real ex(real x)
 {
@@ -56,15 +72,19 @@ struct OpVf {
     static constexpr int D = D_, F = D_ + 1, OUT = 2 * D_ + 2, R = 1;
     using real = typename P::real;
     using lds_t = real;
+    using crd_t = real;
+    static constexpr bool kCull = P::fp32 && CEN == kCenG;
     struct Args {
         const double *ci, *ui;   // targets: deformed element centres / directions
         int ni;
         const double *cj, *uj, *aj;   // sources
         int nj;
         real ics2, ids2;
+        float cut2;   // culling range (cull_r2); +inf: none
+        double kap;   // FP32: cen_kappa of the staged offsets
     };
+    __device__ __forceinline__ static double crd_scale(const Args &A) { return A.kap; }
     struct Tgt {
-        double c[D];
         real u[D];
     };
     struct Acc {
@@ -79,11 +99,11 @@ struct OpVf {
         }
         f[D] = (real)A.aj[j];
     }
-    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t)
+    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t, double (&x)[D])
     {
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-            t.c[c] = i >= 0 ? A.ci[(size_t)c * A.ni + i] : 0.0;
+            x[c] = i >= 0 ? A.ci[(size_t)c * A.ni + i] : 0.0;
             t.u[c] = i >= 0 ? (real)A.ui[(size_t)c * A.ni + i] : real(0);
         }
     }
@@ -93,40 +113,64 @@ struct OpVf {
 #pragma unroll
         for (int c = 0; c < D; ++c) a.g[c] = a.u[c] = real(0);
     }
-    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const double (&xj)[D],
-                                                const lds_t (&fj)[F], Acc &acc)
+    // FP32 with Binet: the factor 2 of dku = 2 ang moves to the flush
+    static constexpr bool kBinet32 = P::fp32 && DIR == kDirB;
+    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const crd_t (&xi)[D],
+                                                const crd_t (&xj)[D], const lds_t (&fj)[F], Acc &acc)
     {
         real df[D], r2 = real(0), ang = real(0);
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-            df[c] = (real)(t.c[c] - xj[c]);
+            df[c] = xi[c] - xj[c];
             r2 += df[c] * df[c];
             ang += t.u[c] * fj[c];
         }
-        const real kc = cen_value<CEN, real>(r2, A.ics2);
-        real dku;
-        const real ku = dir_value<DIR, real>(ang, A.ids2, dku);
+        real kc;
+        if constexpr (P::fp32) kc = cen_value_scaled<CEN>(r2);
+        else kc = cen_value<CEN, real>(r2, A.ics2);
         const real w = kc * fj[D];
-        const real wk = w * ku;
-        const real gw = (CEN == kCenG) ? wk : wk * kc;
-        const real wd = w * dku;
-        acc.e += wk;
-        // Binet: wk - wd ang = w ang^2 - 2 w ang^2 = -wk, so S = -E
-        if constexpr (DIR != kDirB) acc.s += wk - wd * ang;
+        if constexpr (kBinet32) {
+            // ku = ang^2: wk = (w ang) ang, u-sum of (w ang) u_j (times 2 at the flush)
+            const real wa = w * ang;
+            const real wk = wa * ang;
+            const real gw = (CEN == kCenG) ? wk : wk * kc;
+            acc.e += wk;
 #pragma unroll
-        for (int c = 0; c < D; ++c) {
-            acc.g[c] += gw * df[c];
-            acc.u[c] += wd * fj[c];
+            for (int c = 0; c < D; ++c) {
+                acc.g[c] += gw * df[c];
+                acc.u[c] += wa * fj[c];
+            }
+        } else {
+            real dku;
+            const real ku = dir_value<DIR, real>(ang, A.ids2, dku);
+            const real wk = w * ku;
+            const real gw = (CEN == kCenG) ? wk : wk * kc;
+            const real wd = w * dku;
+            acc.e += wk;
+            // Binet: wk - wd ang = w ang^2 - 2 w ang^2 = -wk, so S = -E
+            if constexpr (DIR != kDirB) acc.s += wk - wd * ang;
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                acc.g[c] += gw * df[c];
+                acc.u[c] += wd * fj[c];
+            }
         }
     }
-    __device__ __forceinline__ static void flush(const Acc &a, double (&tot)[OUT])
+    __device__ __forceinline__ static void flush(const Args &A, const Acc &a, double (&tot)[OUT])
     {
         tot[0] += (double)a.e;
         tot[1] += DIR == kDirB ? -(double)a.e : (double)a.s;
+        // FP32: the g-sum is over scaled differences (1 / kappa)
+        const double gs = P::fp32 ? 1.0 / A.kap : 1.0, us = kBinet32 ? 2.0 : 1.0;
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-            tot[2 + c] += (double)a.g[c];
-            tot[2 + D + c] += (double)a.u[c];
+            if constexpr (P::fp32) {
+                tot[2 + c] += (double)a.g[c] * gs;
+                tot[2 + D + c] += (double)a.u[c] * us;
+            } else {
+                tot[2 + c] += a.g[c];
+                tot[2 + D + c] += a.u[c];
+            }
         }
     }
 };
@@ -137,13 +181,15 @@ struct OpVfValue {
     static constexpr int D = D_, F = D_ + 1, OUT = 1, R = 1;
     using real = double;
     using lds_t = double;
+    using crd_t = double;
+    static constexpr bool kCull = false;
     struct Args {
         const double *c, *u, *a;
         int n;
         double ics2, ids2;
     };
     struct Tgt {
-        double c[D], u[D];
+        double u[D];
     };
     struct Acc {
         double e;
@@ -157,29 +203,29 @@ struct OpVfValue {
         }
         f[D] = A.a[j];
     }
-    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t)
+    __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t, double (&x)[D])
     {
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-            t.c[c] = i >= 0 ? A.c[(size_t)c * A.n + i] : 0.0;
+            x[c] = i >= 0 ? A.c[(size_t)c * A.n + i] : 0.0;
             t.u[c] = i >= 0 ? A.u[(size_t)c * A.n + i] : 0.0;
         }
     }
     __device__ __forceinline__ static void zero(Acc &a) { a.e = 0.0; }
-    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const double (&xj)[D],
-                                                const lds_t (&fj)[F], Acc &acc)
+    __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const crd_t (&xi)[D],
+                                                const crd_t (&xj)[D], const lds_t (&fj)[F], Acc &acc)
     {
         double r2 = 0.0, ang = 0.0;
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-            double df = t.c[c] - xj[c];
+            double df = xi[c] - xj[c];
             r2 += df * df;
             ang += t.u[c] * fj[c];
         }
         double dku;
         acc.e += cen_value<CEN, double>(r2, A.ics2) * dir_value<DIR, double>(ang, A.ids2, dku) * fj[D];
     }
-    __device__ __forceinline__ static void flush(const Acc &a, double (&tot)[OUT]) { tot[0] += a.e; }
+    __device__ __forceinline__ static void flush(const Args &, const Acc &a, double (&tot)[OUT]) { tot[0] += a.e; }
 };
 
 }  // namespace ops

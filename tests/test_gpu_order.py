@@ -1,9 +1,10 @@
-"""The caller's vertex order at the boundary.  Objectives and batches hold
+"""The spatial order and what rests on it.  Objectives and batches hold
 every problem in their own spatial order; relabelling the vertices,
 elements and target elements of a problem must therefore leave energies
 unchanged and permute gradients, trajectories and optimised momenta in the
 same way, through every entry point (host arrays, device arrays, the
-registration driver, the batch)."""
+registration driver, the batch).  The FP32 policy's exact culling of far
+tile pairs must give bitwise the results of computing every pair."""
 import numpy as np
 import pytest
 
@@ -121,3 +122,57 @@ def test_batch_is_invariant_under_relabelling(gpu):
         assert rel_err(a2[i], a1[i][:, vp, :]) <= 1e-9
     B1.close()
     B2.close()
+
+
+def two_clusters(seed):
+    """Two copies of a batch problem 6 apart: most tile pairs across the
+    gap are beyond the FP32 Gaussian's range and are culled."""
+    p = G.batch_problems(1, seed=seed, nmin=1200, nmax=1300)[0]
+    n = len(p["q0"])
+    shift = np.array([6.0, 0.0, 0.0])
+    return dict(q0=np.concatenate([p["q0"], p["q0"] + shift]),
+                elm=np.concatenate([p["elm"], p["elm"] + n]).astype(np.int32),
+                tgt_cen=np.concatenate([p["tgt_cen"], p["tgt_cen"] + shift]),
+                tgt_dir=np.concatenate([p["tgt_dir"], p["tgt_dir"]]),
+                tgt_vol=np.concatenate([p["tgt_vol"], p["tgt_vol"]]), knl_width=p["knl_width"])
+
+
+def test_culling_is_exact(gpu, problem):
+    """FP32 objective with and without exact culling: bitwise equal, on a
+    single shape (the varifold's width-0.1 Gaussian culls its far tiles) and
+    on two shapes far apart (the deformation kernel culls too)."""
+    print("culling on/off, objective", flush=True)
+    for p in (problem, two_clusters(8)):
+        n, T = len(p["q0"]), K["time_num"]
+        alp = 0.05 * G.normal(7, (T - 1) * n * 3).reshape(T - 1, n, 3)
+        obj = objective(gpu, p, "fp32")
+        try:
+            gpu.set_pair_culling(False)
+            f0, g0 = obj(alp)
+        finally:
+            gpu.set_pair_culling(True)
+        f1, g1 = obj(alp)
+        assert f0 == f1 and np.array_equal(g0, g1)
+        f64, g64 = objective(gpu, p)(alp)
+        assert abs(f1 - f64) <= 1e-6 * abs(f64) and rel_err(g1, g64) <= 1e-4
+        obj.close()
+
+
+def test_culling_is_exact_in_a_batch(gpu):
+    print("culling on/off, batch", flush=True)
+    probs = G.batch_problems(3, seed=29, nmin=400, nmax=2500) + [two_clusters(9)]
+    T = K["time_num"]
+    B = gpu.Batch(probs, K["cen_type"], K["cen_width"], K["dir_type"], K["dir_width"], K["dis_wgt"],
+                  K["knl_order"], K["time_stp"], T, precision="fp32", chunk=256)
+    alps = [0.05 * G.normal(60 + i, B.sizes[i]).reshape(T - 1, B.n[i], 3) for i in range(B.P)]
+    f1, g1 = B.objgrd(alps)
+    try:
+        gpu.set_pair_culling(False)
+        f0, g0 = B.objgrd(alps)
+    finally:
+        gpu.set_pair_culling(True)
+    f2, g2 = B.objgrd(alps)
+    assert np.array_equal(f0, f1) and np.array_equal(f1, f2)
+    for a, b, c in zip(g0, g1, g2):
+        assert np.array_equal(a, b) and np.array_equal(b, c)
+    B.close()
