@@ -52,12 +52,12 @@ TRANS_PER_PAIR = {("fwd_pairs", -1): 1, ("fwd_pairs", 1): 2, ("bwd_pairs", -1): 
 # copy kernel of known size), valu_busy = AMD's VALUBusy, valu_issue = share of SIMD
 # issue slots at 4 cycles per wave64 VALU instruction.
 ROCPROF_SPHERE_CUBE = {
-    ("fwd_pairs", "f64"): dict(traffic=5.635e6, valu_busy=0.75, valu_issue=0.72),
-    ("bwd_pairs", "f64"): dict(traffic=1.234e7, valu_busy=0.75, valu_issue=0.73),
-    ("vf_pairs", "f64"): dict(traffic=1.322e7, valu_busy=0.78, valu_issue=0.79),
-    ("fwd_pairs", "f32"): dict(traffic=8.055e6, valu_busy=0.80, valu_issue=0.74),
-    ("bwd_pairs", "f32"): dict(traffic=1.724e7, valu_busy=0.85, valu_issue=0.81),
-    ("vf_pairs", "f32"): dict(traffic=1.321e7, valu_busy=1.04, valu_issue=1.03),
+    ("fwd_pairs", "f64"): dict(traffic=5.635e6, valu_busy=0.76, valu_issue=0.72),
+    ("bwd_pairs", "f64"): dict(traffic=1.235e7, valu_busy=0.75, valu_issue=0.73),
+    ("vf_pairs", "f64"): dict(traffic=1.322e7, valu_busy=0.78, valu_issue=0.80),
+    ("fwd_pairs", "f32"): dict(traffic=8.066e6, valu_busy=0.83, valu_issue=0.75),
+    ("bwd_pairs", "f32"): dict(traffic=1.727e7, valu_busy=0.87, valu_issue=0.82),
+    ("vf_pairs", "f32"): dict(traffic=1.323e7, valu_busy=0.75, valu_issue=0.82),
 }
 
 
@@ -230,6 +230,42 @@ def batch_bench(count, nmin, nmax, itr, seed=4, train_count=64):
                 speedup_batched_fp32_warm_vs_plain_fp64=t_seq / t_warm)
 
 
+def culling_report(count=32, seed=4, nmin=500, nmax=5000, reps=5):
+    """Exact culling of far FP32 Gaussian tile pairs (bitwise the same results):
+    mean time per launch group of the pair kernels with culling off and on,
+    on the sphere->cube FP32 evaluation and on the uneven batch (all problems
+    active, seeded momentum), from the library's HIP-event accounting."""
+    def groups(fn):
+        out = {}
+        for on in (False, True):
+            lddmm_amd.set_pair_culling(on)
+            fn()
+            lddmm_amd.profile_reset()
+            lddmm_amd.profile_enable(True)
+            for _ in range(reps):
+                fn()
+            lddmm_amd.profile_enable(False)
+            r = lddmm_amd.profile_read()
+            out["on" if on else "off"] = {k: r[k][0] / r[k][1] for k in ("fwd_pairs", "bwd_pairs", "vf_pairs")
+                                          if r.get(k, (0, 0))[1]}
+        lddmm_amd.set_pair_culling(True)
+        return out
+    w, _ = workload("sphere_cube")
+    n, T = len(w["q0"]), w["time_num"]
+    obj, _ = make_objective(w)
+    obj.set_precision("fp32")
+    alp = 0.02 * G.normal(1, (T - 1) * n * 3)
+    c2 = groups(lambda: obj(alp))
+    probs = G.batch_problems(count, seed, nmin, nmax)
+    k = G.BATCH_KERNELS
+    B = lddmm_amd.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
+                        k["knl_order"], k["time_stp"], k["time_num"], precision="fp32")
+    alps = [0.02 * G.normal(10 + i, B.sizes[i]).reshape(k["time_num"] - 1, B.n[i], 3) for i in range(B.P)]
+    batch = groups(lambda: B.objgrd(alps))
+    return dict(report="exact culling, ms per launch group (fwd/bwd: one Euler step; vf: DD + DT of one "
+                       "evaluation)", sphere_cube_fp32=c2, batch_fp32=dict(problems=count, seed=seed, **batch))
+
+
 def precision_report(freq=100, seed=31, itr=10):
     """FP32 policy vs FP64 on the single large problem: a seeded superquadric
     source (geodesic frequency 100: 100,002 vertices, 200,000 triangles) and an
@@ -348,6 +384,8 @@ def main():
                     help="only the full-size batched configuration (256 problems, N up to 20,000)")
     ap.add_argument("--batch-profile", action="store_true",
                     help="only the batched FP32 registration of the uneven batch (for kernel traces)")
+    ap.add_argument("--culling-report", action="store_true",
+                    help="only the pair-kernel times with exact culling off and on")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: there is no multi-GPU path (problems would shard by "
@@ -360,14 +398,18 @@ def main():
     if args.c4_full:
         print(json.dumps(c4_report()), flush=True)
         return
+    if args.culling_report:
+        print(json.dumps(culling_report()), flush=True)
+        return
     if args.batch_profile:
         probs = G.batch_problems(args.batch, 4, args.batch_nmin, args.batch_nmax)
         k = G.BATCH_KERNELS
         B = lddmm_amd.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
                             k["knl_order"], k["time_stp"], k["time_num"], precision="fp32", chunk=0)
         chunk, pairs = B.info()
-        # direct launches: rocprofv3's kernel tracing has crashed the host process
-        # during hipGraph capture here; graphs do not change kernel times
+        # direct launches (as whenever rocprofv3 drives the process): its kernel
+        # tracing has crashed the host process during hipGraph capture here;
+        # graphs do not change kernel times
         B.set_graphs(False)
         _, info, rounds = B.run(300, 0.0, 0.0, 0.5, 5, precision="fp32", rel_energy_tol=1e-6)
         print(json.dumps(dict(problems=args.batch, chunk=chunk, pairs_per_eval_total=float(pairs.sum()),
