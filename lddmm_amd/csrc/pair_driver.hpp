@@ -370,7 +370,9 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const typename Op::Args args, 
 }
 
 // Split plan: target tiles of TI, source chunks (multiples of 64) so that
-// the grid is about `target_blocks` workgroups.
+// the grid is about `target_blocks` workgroups, with at least one staged
+// tile (256 sources) per chunk: below that the per-block staging, flush and
+// split-reduction work outweighs the finer balance (small problems).
 inline PairSplit make_split_ti(int ni, int nj, int ti, int target_blocks)
 {
     PairSplit s;
@@ -379,7 +381,7 @@ inline PairSplit make_split_ti(int ni, int nj, int ti, int target_blocks)
     if (want < 1) want = 1;
     int chunk = ceil_div(std::max(nj, 1), want);
     chunk = ((chunk + 63) / 64) * 64;
-    if (chunk < 64) chunk = 64;
+    if (chunk < kTileJ) chunk = kTileJ;
     s.chunk = chunk;
     s.splits = nj > 0 ? ceil_div(nj, chunk) : 1;
     return s;
