@@ -231,15 +231,15 @@ def batch_bench(count, nmin, nmax, itr, seed=4, train_count=64):
 
 
 def culling_report(count=32, seed=4, nmin=500, nmax=5000, reps=5):
-    """Exact culling of far FP32 Gaussian tile pairs (bitwise the same results):
+    """Culling of far FP32 Gaussian tile pairs (kernel values below 2^-64):
     mean time per launch group of the pair kernels with culling off and on,
     on the sphere->cube FP32 evaluation and on the uneven batch (all problems
     active, seeded momentum), from the library's HIP-event accounting."""
     def groups(fn):
-        out = {}
+        out, res = {}, {}
         for on in (False, True):
             lddmm_amd.set_pair_culling(on)
-            fn()
+            res[on] = fn()
             lddmm_amd.profile_reset()
             lddmm_amd.profile_enable(True)
             for _ in range(reps):
@@ -249,6 +249,14 @@ def culling_report(count=32, seed=4, nmin=500, nmax=5000, reps=5):
             out["on" if on else "off"] = {k: r[k][0] / r[k][1] for k in ("fwd_pairs", "bwd_pairs", "vf_pairs")
                                           if r.get(k, (0, 0))[1]}
         lddmm_amd.set_pair_culling(True)
+        # results with culling on against every pair computed (an objective
+        # returns one energy and gradient, a batch one per problem)
+        def split(r):
+            f, g = r
+            return np.atleast_1d(np.asarray(f, dtype=float)), (list(g) if isinstance(g, list) else [g])
+        (f0, g0), (f1, g1) = split(res[False]), split(res[True])
+        out["max_rel_energy_diff"] = float(np.max(np.abs(f1 - f0) / np.abs(f0)))
+        out["max_rel_grad_diff"] = max(float(np.max(np.abs(a - b)) / np.max(np.abs(b))) for a, b in zip(g1, g0))
         return out
     w, _ = workload("sphere_cube")
     n, T = len(w["q0"]), w["time_num"]
@@ -262,7 +270,7 @@ def culling_report(count=32, seed=4, nmin=500, nmax=5000, reps=5):
                         k["knl_order"], k["time_stp"], k["time_num"], precision="fp32")
     alps = [0.02 * G.normal(10 + i, B.sizes[i]).reshape(k["time_num"] - 1, B.n[i], 3) for i in range(B.P)]
     batch = groups(lambda: B.objgrd(alps))
-    return dict(report="exact culling, ms per launch group (fwd/bwd: one Euler step; vf: DD + DT of one "
+    return dict(report="culling below 2^-64, ms per launch group (fwd/bwd: one Euler step; vf: DD + DT of one "
                        "evaluation)", sphere_cube_fp32=c2, batch_fp32=dict(problems=count, seed=seed, **batch))
 
 
@@ -328,15 +336,16 @@ def c4_report(count=256, nmin=500, nmax=20000, itr=300, seed=4):
     finishing in FP64 on a stall), and its results are re-evaluated in FP64."""
     probs = G.batch_problems(count, seed, nmin, nmax)
     k = G.BATCH_KERNELS
-    seq = []
+    seq, a_plain = [], []
     t0 = time.perf_counter()
     for i, p in enumerate(probs):
         ifo = lddmm_amd.getElmIfo(p["elm"], len(p["q0"]), one_based=False)
-        _, info = lddmm_amd.run(p["q0"], None, p["elm"], ifo, p["tgt_cen"], p["tgt_vol"], p["tgt_dir"],
+        a, info = lddmm_amd.run(p["q0"], None, p["elm"], ifo, p["tgt_cen"], p["tgt_vol"], p["tgt_dir"],
                                 k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
                                 k["knl_order"], p["knl_width"], k["time_stp"], k["time_num"], itr, 0.0, 0.0, 0.5, 5,
                                 rel_energy_tol=1e-6)
         seq.append(info)
+        a_plain.append(a)
         if i % 16 == 15:
             print("plain FP64: %d of %d problems, %.1f s" % (i + 1, count, time.perf_counter() - t0),
                   file=sys.stderr, flush=True)
@@ -351,8 +360,15 @@ def c4_report(count=256, nmin=500, nmax=20000, itr=300, seed=4):
     B64 = lddmm_amd.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
                           k["knl_order"], k["time_stp"], k["time_num"], precision="fp64", chunk=chunk)
     f64, _ = B64.objgrd(a32)
+    # consistency: the plain runs' energies re-evaluated by the batched FP64 objective
+    f_plain, _ = B64.objgrd(a_plain)
     B64.close()
     rel = (f64 - e_ref) / np.abs(e_ref)
+    missed = [dict(problem=i, n=int(B.n[i]), knl_width=float(probs[i]["knl_width"]), plain_energy=float(e_ref[i]),
+                   plain_iterations=int(seq[i]["iterations"]), plain_status=int(seq[i]["status"]),
+                   fp32_energy_fp64=float(f64[i]), fp32_iterations=int(i32[i]["iterations"]),
+                   fp32_status=int(i32[i]["status"]), fp32_phase_iterations=int(i32[i]["fp32_iterations"]))
+              for i in range(count) if not i32[i]["reached_target"]]
     return dict(problems=count, n_range=[int(min(B.n)), int(max(B.n))], chunk=chunk,
                 pairs_per_eval_total=float(pairs.sum()),
                 plain_fp64_sequential=dict(seconds=t_seq, registrations_per_s=count / t_seq,
@@ -362,7 +378,8 @@ def c4_report(count=256, nmin=500, nmax=20000, itr=300, seed=4):
                                   max_iterations=int(max(r["iterations"] for r in i32)),
                                   reached_target=int(sum(r["reached_target"] for r in i32)),
                                   within_1e6=int(np.sum(rel <= 1e-6 + 1e-8)),
-                                  max_rel_energy_vs_plain=float(rel.max())),
+                                  max_rel_energy_vs_plain=float(rel.max()), missed=missed),
+                plain_energy_check=dict(max_rel_diff_batched_fp64=float(np.max(np.abs(f_plain - e_ref) / np.abs(e_ref)))),
                 speedup_batched_fp32_vs_plain_fp64=t_seq / t32)
 
 
@@ -385,7 +402,7 @@ def main():
     ap.add_argument("--batch-profile", action="store_true",
                     help="only the batched FP32 registration of the uneven batch (for kernel traces)")
     ap.add_argument("--culling-report", action="store_true",
-                    help="only the pair-kernel times with exact culling off and on")
+                    help="only the pair-kernel times with culling off and on")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: there is no multi-GPU path (problems would shard by "

@@ -278,12 +278,22 @@ int lddmm_get_elm_ifo(int32_t *elmIfoMat, const int32_t *elmVtxMat, int vtxPerEl
  * bisection and hence spatially compact.  Host-only; inputs and outputs of
  * every other entry point stay in the caller's order. */
 int lddmm_spatial_order(int32_t *perm, const double *ptsMat, int ptsNum, int dimNum);
-/* Exact culling in the FP32 policy (no counterpart in the reference): a
- * pair of source and target tiles whose Gaussian kernel values (deformation
- * kernel, or the varifold's 'G' position kernel) are all exactly 0 in FP32
- * is skipped.  On by default; off computes every pair and gives bitwise the
- * same results (for checking).  Process-wide, read at every launch. */
+/* Culling in the FP32 policy (no counterpart in the reference): a range of
+ * sources whose Gaussian kernel values (deformation kernel, or the
+ * varifold's 'G' position kernel) with a tile of targets are all below
+ * 2^-64 is skipped; the terms left out are below 2^-64 times the terms'
+ * other factors, far below the FP32 rounding of the sums.  On by default;
+ * off computes every pair (results agree to FP64 rounding, for checking).
+ * Process-wide, read at every launch. */
 int lddmm_set_pair_culling(int on);
+/* Expanded form of the FP32 Gaussian pair terms (no counterpart in the
+ * reference): a pair forms r^2 - |x_i|^2 = |x_j|^2 - 2 x_i.x_j with D
+ * multiply-adds and the factor exp2(-|x_i|^2) of the Gaussian is applied
+ * once per target and range of sources; a target tile whose radius is too
+ * large for that factor keeps the direct form.  On by default; off uses the
+ * direct form everywhere (results differ by FP32 rounding).  Process-wide,
+ * read at every launch. */
+int lddmm_set_pair_expansion(int on);
 /* Element varifold of a mesh (the target preprocessing of the examples,
  * Cpp/examples/circle2square/run_circle2square.m:21-24; the same formulas as
  * landmarksToVarifoldKernel, Cpp/src/varifold3D.cpp:8-39 and

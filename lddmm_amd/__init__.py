@@ -20,7 +20,7 @@ import os
 import numpy as np
 
 __all__ = ["lib", "LDDMM", "getElmIfo", "computeNodePosition", "elementVarifold", "multiplyKernel", "dqKernel", "computeKernel",
-           "varifold", "Objective", "run", "l2Dist", "run_l2", "spatial_order", "set_pair_culling", "LddmmError", "profile_enable", "profile_read", "profile_reset"]
+           "varifold", "Objective", "run", "l2Dist", "run_l2", "spatial_order", "set_pair_culling", "set_pair_expansion", "LddmmError", "profile_enable", "profile_read", "profile_reset"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblddmm_amd.so")
@@ -95,6 +95,7 @@ def lib():
         L.lddmm_get_elm_ifo.argtypes = [_ip, _ip, C.c_int, C.c_int, C.c_int]
         L.lddmm_spatial_order.argtypes = [_ip, _dp, C.c_int, C.c_int]
         L.lddmm_set_pair_culling.argtypes = [C.c_int]
+        L.lddmm_set_pair_expansion.argtypes = [C.c_int]
         L.lddmm_compute_node_position.argtypes = [_dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_int,
                                                   C.c_int, C.c_int]
         L.lddmm_device_count.argtypes = [C.POINTER(C.c_int)]
@@ -156,10 +157,17 @@ def spatial_order(pts):
 
 
 def set_pair_culling(on):
-    """Exact culling of FP32 Gaussian tile pairs whose terms are all exactly
-    zero (default on); off computes every pair, with bitwise the same
-    results."""
+    """Culling of FP32 Gaussian tile pairs whose kernel values are all below
+    2^-64 (default on); off computes every pair, with results that agree to
+    FP64 rounding."""
     _check(lib().lddmm_set_pair_culling(int(bool(on))))
+
+
+def set_pair_expansion(on):
+    """Expanded form of the FP32 Gaussian pair terms (default on): r^2 as
+    |x_i|^2 + (|x_j|^2 - 2 x_i.x_j) with the factor exp2(-|x_i|^2) applied
+    per target; off uses the direct form (results differ by FP32 rounding)."""
+    _check(lib().lddmm_set_pair_expansion(int(bool(on))))
 
 
 # ---------------------------------------------------------------- operators

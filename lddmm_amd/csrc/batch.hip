@@ -45,7 +45,8 @@ struct BatchView {
     int rows, T;
     double h, w;
     double *ecen, *edir, *evol, *dc, *dd, *efwd, *evf;
-    int cull;   // exact culling of FP32 Gaussian tile pairs
+    int cull;     // culling of FP32 Gaussian tile pairs (kernel values < 2^-64)
+    int expand;   // expanded form of the FP32 Gaussian pair terms
 };
 
 template <int NC>
@@ -82,7 +83,15 @@ __global__ __launch_bounds__(kBlock) void kb_kv(BatchView v, int t, const int4 *
     const BatchDesc d = v.desc[it.x];
     const double *q = v.nde + (size_t)v.T * D * d.vo + (size_t)t * D * d.n;
     const double *a = v.alp + (size_t)(v.T - 1) * D * d.vo + (size_t)t * D * d.n;
-    typename Op::Args A{q,        q, a, d.n, d.n, (real)d.iw, (real)d.iw2, cull_r2(v.cull != 0),
+    typename Op::Args A{q,
+                        q,
+                        a,
+                        d.n,
+                        d.n,
+                        (real)d.iw,
+                        (real)d.iw2,
+                        cull_r2(v.cull != 0),
+                        expand_r2(v.expand != 0),
                         Op::KF::kappa(d.iw, d.iw2)};
     const int j0 = it.z * d.chunk, j1 = min(d.n, j0 + d.chunk);
     pairs_block<Op>(A, d.n, it.y, j0, j1, v.part + d.po + (size_t)it.z * D * d.n, smem);
@@ -99,8 +108,16 @@ __global__ __launch_bounds__(kBlock) void kb_bwd(BatchView v, int t, const int4 
     const double *q = v.nde + (size_t)v.T * D * d.vo + (size_t)t * D * d.n;
     const double *a = v.alp + (size_t)(v.T - 1) * D * d.vo + (size_t)t * D * d.n;
     const double kap = Op::KF::kappa(d.iw, d.iw2);
-    typename Op::Args A{q,   a, v.padj + (size_t)D * d.vo, d.n, (real)d.iw, (real)d.iw2, cull_r2(v.cull != 0),
-                        kap, Op::KF::cmul(d.iw, d.iw2) / kap};
+    typename Op::Args A{q,
+                        a,
+                        v.padj + (size_t)D * d.vo,
+                        d.n,
+                        (real)d.iw,
+                        (real)d.iw2,
+                        cull_r2(v.cull != 0),
+                        expand_r2(v.expand != 0),
+                        kap,
+                        Op::KF::cmul(d.iw, d.iw2) / kap};
     const int j0 = it.z * d.chunk, j1 = min(d.n, j0 + d.chunk);
     pairs_block<Op>(A, d.n, it.y, j0, j1, v.part + d.po + (size_t)it.z * 2 * D * d.n, smem);
 }
@@ -210,6 +227,7 @@ __global__ __launch_bounds__(kBlock) void kb_vf(BatchView v, double ics2, double
     A.ics2 = (real)ics2;
     A.ids2 = (real)ids2;
     A.cut2 = cull_r2(v.cull != 0);
+    A.xr2 = expand_r2(v.expand != 0);
     A.kap = cen_kappa<CEN>(ics2);
     const int j0 = (tgt ? it.z - d.vsd : it.z) * d.vchunk, j1 = min(A.nj, j0 + d.vchunk);
     pairs_block<Op>(A, d.ne, it.y, j0, j1, v.vpart + d.vpo + (size_t)it.z * (2 * D + 2) * d.ne, smem);
@@ -788,8 +806,8 @@ void Batch::build_tables(const std::vector<int> &prec_of)
         const int g = pr == kFP32 ? 1 : 0;
         const BatchDesc &d = desc_h_[p];
         probs.push_back(p);
-        // FP32 pair kernels keep two targets per lane
-        const int tkv = g ? 128 : 64, tbwd = g ? 128 : 64;
+        // FP32 pair kernels keep kKvR32 / kBwdR32 targets per lane
+        const int tkv = g ? 64 * kKvR32 : 64, tbwd = g ? 64 * kBwdR32 : 64;
         for (int s = 0; s < d.splits; ++s) {
             for (int t = 0; t < ceil_div(d.n, tkv); ++t) kv[g].push_back(make_int4(p, t, s, 0));
             for (int t = 0; t < ceil_div(d.n, tbwd); ++t) bwd[g].push_back(make_int4(p, t, s, 0));
@@ -865,9 +883,9 @@ void Batch::eval(const double *alp, double *grd, double *f_dev, const std::vecto
         launch_eval(alp, grd, f_dev);
         return;
     }
-    const int cull = pair_culling() ? 1 : 0;
+    const int cull = pair_culling() ? 1 : 0, expand = pair_expansion() ? 1 : 0;
     for (const GraphEntry &e : graphs_)
-        if (e.alp == alp && e.grd == grd && e.f == f_dev && e.cull == cull) {
+        if (e.alp == alp && e.grd == grd && e.f == f_dev && e.cull == cull && e.expand == expand) {
             LDDMM_HIP(hipGraphLaunch(e.exec, st_));
             return;
         }
@@ -885,7 +903,7 @@ void Batch::eval(const double *alp, double *grd, double *f_dev, const std::vecto
     const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     LDDMM_HIP(ie);
-    graphs_.push_back(GraphEntry{alp, grd, f_dev, cull, exec});
+    graphs_.push_back(GraphEntry{alp, grd, f_dev, cull, expand, exec});
     LDDMM_HIP(hipGraphLaunch(exec, st_));
 }
 
@@ -907,7 +925,8 @@ void Batch::launch_eval(const double *alp, double *grd, double *f_dev)
     const int D = spec_.dim, T = spec_.T;
     const Tables &tb = *tab_;
     BatchView v{desc_, nde_, padj_, part_, vpart_, alp, grd, q0_, tc_, tu_, ta_, elm_, ifo_, rows_, T,
-                spec_.h, spec_.w, ecen_, edir_, evol_, dc_, dd_, efwd_, evf_, pair_culling() ? 1 : 0};
+                spec_.h, spec_.w, ecen_, edir_, evol_, dc_, dd_, efwd_, evf_, pair_culling() ? 1 : 0,
+                pair_expansion() ? 1 : 0};
 
     // q_0 of the listed problems
     if (D == 2) kb_init_flow<2><<<tb.nvtile, kTile, 0, st_>>>(v, tb.vtile.p);
@@ -931,7 +950,7 @@ void Batch::launch_eval(const double *alp, double *grd, double *f_dev)
             each_prec([&](int g, auto Pc) {
                 if (!tb.nkv[g]) return;
                 dispatch([&](auto Dc, auto Oc) {
-                    constexpr int R = decltype(Pc)::fp32 ? 2 : 1;
+                    constexpr int R = decltype(Pc)::fp32 ? kKvR32 : 1;
                     kb_kv<Dc(), Oc(), decltype(Pc), R><<<tb.nkv[g], kBlock, 0, st_>>>(v, t, tb.kv[g].p);
                 });
                 LDDMM_LAUNCH_CHECK();
@@ -983,7 +1002,7 @@ void Batch::launch_eval(const double *alp, double *grd, double *f_dev)
             each_prec([&](int g, auto Pc) {
                 if (!tb.nbwd[g]) return;
                 dispatch([&](auto Dc, auto Oc) {
-                    constexpr int R = decltype(Pc)::fp32 ? 2 : 1;
+                    constexpr int R = decltype(Pc)::fp32 ? kBwdR32 : 1;
                     kb_bwd<Dc(), Oc(), decltype(Pc), R><<<tb.nbwd[g], kBlock, 0, st_>>>(v, t, tb.bwd[g].p);
                 });
                 LDDMM_LAUNCH_CHECK();
@@ -1042,12 +1061,13 @@ void Batch::autotune()
         const Tables &tb = *tab_;
         const int g = spec_.prec == kFP32 ? 1 : 0;
         BatchView v{desc_, nde_, padj_, part_, vpart_, alp.p, nullptr, q0_, tc_, tu_, ta_, elm_, ifo_, rows_,
-                    spec_.T, spec_.h, spec_.w, ecen_, edir_, evol_, dc_, dd_, efwd_, evf_};
+                    spec_.T, spec_.h, spec_.w, ecen_, edir_, evol_, dc_, dd_, efwd_, evf_, pair_culling() ? 1 : 0,
+                    pair_expansion() ? 1 : 0};
         auto launch = [&] {
             dispatch_order(spec_.order, [&](auto O) {
                 if (g) {
-                    if (D == 2) kb_kv<2, O(), PrecF32, 2><<<tb.nkv[1], kBlock, 0, st_>>>(v, 0, tb.kv[1].p);
-                    else kb_kv<3, O(), PrecF32, 2><<<tb.nkv[1], kBlock, 0, st_>>>(v, 0, tb.kv[1].p);
+                    if (D == 2) kb_kv<2, O(), PrecF32, kKvR32><<<tb.nkv[1], kBlock, 0, st_>>>(v, 0, tb.kv[1].p);
+                    else kb_kv<3, O(), PrecF32, kKvR32><<<tb.nkv[1], kBlock, 0, st_>>>(v, 0, tb.kv[1].p);
                 } else {
                     if (D == 2) kb_kv<2, O(), PrecF64, 1><<<tb.nkv[0], kBlock, 0, st_>>>(v, 0, tb.kv[0].p);
                     else kb_kv<3, O(), PrecF64, 1><<<tb.nkv[0], kBlock, 0, st_>>>(v, 0, tb.kv[0].p);

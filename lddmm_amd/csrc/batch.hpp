@@ -117,7 +117,7 @@ private:
     struct GraphEntry {
         const double *alp;
         double *grd, *f;
-        int cull;   // pair_culling() at capture
+        int cull, expand;   // pair_culling(), pair_expansion() at capture
         hipGraphExec_t exec;
     };
     std::vector<GraphEntry> graphs_;

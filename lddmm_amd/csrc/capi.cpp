@@ -1024,6 +1024,12 @@ int lddmm_set_pair_culling(int on)
     return 0;
 }
 
+int lddmm_set_pair_expansion(int on)
+{
+    set_pair_expansion(on != 0);
+    return 0;
+}
+
 int lddmm_get_elm_ifo(int32_t *elmIfoMat, const int32_t *elmVtxMat, int vtxPerElm, int elmNum, int ndeNum)
 {
     return guarded([&] {

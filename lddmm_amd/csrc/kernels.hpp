@@ -8,11 +8,17 @@
 namespace lddmm {
 
 // ---- pair sums over points (pair_kernels.hip) ---------------------------
-// Exact culling of FP32 Gaussian tile pairs (pair_driver.hpp); on by
-// default, read at every launch (a batch keys its captured graphs on it).
-// Off gives bitwise the same results at the full pair count.
+// Culling of FP32 Gaussian tile pairs whose kernel values are all below
+// 2^-64 (pair_driver.hpp); on by default, read at every launch (a batch keys
+// its captured graphs on it).  Off computes every pair; the results agree to
+// FP64 rounding.
 void set_pair_culling(bool on);
 bool pair_culling();
+// The expanded form of the FP32 Gaussian pair terms (pair_driver.hpp); on
+// by default, read at every launch like the culling switch.  Off gives the
+// direct form (results differ by FP32 rounding).
+void set_pair_expansion(bool on);
+bool pair_expansion();
 // Split plans for the point kernels: plan_kv for K.a and plan_bwd for the
 // fused backward pass (two targets per lane in FP32), plan_single for the dq
 // kernels (one target per lane).

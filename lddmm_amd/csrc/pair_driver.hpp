@@ -40,19 +40,41 @@
 //   pair(args, Tgt&, xi[D], xj[D], fj[F], Acc&)   one pair term
 //   zero(Acc&), flush(args, const Acc&, double tot[OUT])
 //   crd_scale(args)                    FP32 only: kappa of the staged offsets
-//   kCull, args.cut2                   exact culling (below)
+//   kCull, args.cut2                   culling (below)
 //
-// Exact culling (FP32 Gaussian kernels, kCull): a kernel value
-// exp2(-r2) (scaled coordinates, KernelFnT) is exactly 0 in FP32 once the
-// exponent is below -160, and then every term of the pair is +-0 and leaves the tile sums
-// bitwise unchanged.  Per block the driver takes the target tile's radius
-// about the anchor (rt) and per wave's range of staged sources the smallest
-// distance of a source from the anchor (ms); when ms - rt exceeds that range (with a
-// 1e-4 relative margin for the FP32 rounding of both bounds and of r2:
-// args.cut2 = 160 (1 + 1e-4)), no pair of the two tiles can
-// reach it and the wave skips its range.  With the spatially ordered problems
-// this removes the far tiles of narrow kernels (the varifold's Gaussian
-// of width 0.1 on unit-sized shapes skips most of its pairs).
+// Culling (FP32 Gaussian kernels, kCull): a wave skips its range of staged
+// sources when every kernel value exp2(-r2) (scaled coordinates, KernelFnT)
+// between them and the block's targets is below 2^-64, i.e. r2 > 64.  Per
+// block the driver takes the target tile's radius about the anchor (rt) and
+// per wave's range the smallest distance of a source from the anchor (ms);
+// when ms - rt exceeds sqrt(64) (with a 1e-4 relative margin for the FP32
+// rounding of both bounds and of r2: args.cut2 = 64 (1 + 1e-4)), no pair of
+// the two can reach it.  Every term of a pair is a kernel value times factors
+// of the problem's own scale, so a skipped range changes a target's sum by
+// less than 2^-64 times the sum of its terms' factors: far below the FP32
+// rounding of the sums (2^-24 relative), and in practice below the FP64
+// rounding of the totals (culled and full sums agree to the last bits of
+// FP64).  With the spatially ordered problems this removes the far tiles of
+// narrow kernels (the varifold's Gaussian of width 0.1 on unit-sized shapes
+// skips most of its pairs).
+//
+// Expanded form (FP32 Gaussian operators with kExpand): with n_i = |x_i|^2
+// and t = |x_j|^2 - 2 x_i.x_j, r2 = n_i + t and exp2(-r2) =
+// exp2(-n_i) exp2(-t).  The staged record carries |x_j|^2 after the
+// coordinates, the lanes hold m = -2 x_i, a pair forms t with D FMAs
+// (instead of D subtractions and D multiply-adds) and the operator
+// applies exp2(-n_i) once per flush:
+//   pair_x(args, Tgt&, m[D], xj[D+1], fj[F], Acc&)
+//   flush_x(args, const Acc&, double tot[OUT], xi[D], exp2(-n_i))
+// Differences x_i - x_j in the pair terms become x_i sum_j w - sum_j w x_j,
+// completed at the flush.  The FP32 rounding of t is relative to
+// |x_j|^2 + 2 |x_i||x_j| rather than to r2, so the expanded form is taken
+// only by blocks of small target radius, rt^2 <= args.xr2 = 4 (the kernel
+// falls to 1/16 over that radius): there its error stays within a small
+// factor of the direct form's, and exp2(-t) <= exp2(n_i) <= 16.  Wider
+// tiles (small problems, narrow kernels) keep the direct form above.
+// Culling in the expanded form bounds t >= ms (ms - 2 rt) instead (then
+// exp2(-t) < 2^-64, and the factor exp2(-n_i) <= 1 keeps the bound).
 #pragma once
 
 #include "common.hpp"
@@ -197,9 +219,20 @@ struct KernelFnT<-1, float, 2> : KernelFnT<-1, float, 3> {
 constexpr int kWaves = kBlock / 64;
 
 // args.cut2 of a culling operator (FP32 Gaussian on coordinates scaled so
-// that k = exp2(-r2)): the exponent below which the FP32 exp2 is exactly 0,
-// with a 1e-4 relative margin; +inf turns culling off
-__host__ __device__ inline float cull_r2(bool on) { return on ? (float)(160.0 * (1.0 + 1e-4)) : __builtin_inff(); }
+// that k = exp2(-r2)): skipped pairs have k < 2^-64; a 1e-4 relative margin
+// covers the FP32 rounding of the bounds; +inf turns culling off
+__host__ __device__ inline float cull_r2(bool on) { return on ? (float)(64.0 * (1.0 + 1e-4)) : __builtin_inff(); }
+
+// args.xr2 of an expanding operator: the largest squared target radius
+// (scaled units) of a block that takes the expanded form; -1: never
+__host__ __device__ inline float expand_r2(bool on) { return on ? 4.0f : -1.0f; }
+
+template <class Op, class = void>
+struct ExpandOf : std::false_type {
+};
+template <class Op>
+struct ExpandOf<Op, std::void_t<decltype(Op::kExpand)>> : std::integral_constant<bool, Op::kExpand> {
+};
 
 __device__ __forceinline__ float wave_max(float v)
 {
@@ -223,7 +256,9 @@ struct PairSmem {
     static constexpr size_t csize = sizeof(typename Op::crd_t);
     static constexpr size_t fbytes = sizeof(typename Op::lds_t) * (Op::F > 0 ? Op::F : 1);
     static constexpr size_t xoff = (fbytes + csize - 1) & ~(csize - 1);
-    static constexpr size_t rec = (xoff + csize * Op::D + 15) & ~size_t(15);
+    // coordinates, and |x_j|^2 for the expanded form
+    static constexpr int DX = Op::D + (ExpandOf<Op>::value ? 1 : 0);
+    static constexpr size_t rec = (xoff + csize * DX + 15) & ~size_t(15);
     static constexpr size_t stage = rec * kTileJ;
     static constexpr size_t cull = Op::kCull ? sizeof(float) * kTileJ : 0;   // source distances
     static constexpr size_t red = sizeof(double) * kWaves * Op::OUT * 64 * Op::R;
@@ -275,70 +310,107 @@ __device__ __forceinline__ void pairs_block(const typename Op::Args &args, int n
         else return v;
     };
 
+    constexpr bool kExp = ExpandOf<Op>::value;
     crd_t xi[R][D];
+    float xm[R][D], sc[R];   // expanded form: -2 x_i and exp2(-|x_i|^2)
     double tot[R][OUT];
     float rt2 = 0.0f;   // squared target tile radius about the anchor (culling)
 #pragma unroll
     for (int r = 0; r < R; ++r) {
+        const bool real_lane = tile * TI + r * 64 + lane < ni;
         float d2 = 0.0f;
 #pragma unroll
         for (int c = 0; c < D; ++c) {
             xi[r][c] = to_crd(xt[r][c], c);
             d2 += (float)xi[r][c] * (float)xi[r][c];
         }
-        if (tile * TI + r * 64 + lane < ni) rt2 = fmaxf(rt2, d2);
+        if (real_lane) rt2 = fmaxf(rt2, d2);
+#pragma unroll
+        for (int c = 0; c < D; ++c) xm[r][c] = real_lane ? -2.0f * (float)xi[r][c] : 0.0f;
+        sc[r] = real_lane ? __builtin_amdgcn_exp2f(-d2) : 1.0f;
 #pragma unroll
         for (int c = 0; c < OUT; ++c) tot[r][c] = 0.0;
     }
-    float rt = 0.0f;
-    if constexpr (Op::kCull) rt = __builtin_sqrtf(wave_max(rt2));
+    float rt = 0.0f, rtw2 = 0.0f;
+    if constexpr (Op::kCull || kExp) {
+        rtw2 = wave_max(rt2);
+        rt = __builtin_sqrtf(rtw2);
+    }
     float *s2buf = reinterpret_cast<float *>(smem + SM::stage);   // squared source distances to the anchor
 
-    for (int jt = j0; jt < j1; jt += kTileJ) {
-        const int cnt = min(kTileJ, j1 - jt);
-        __syncthreads();
-        if ((int)threadIdx.x < cnt) {
-            double x[D];
-            lds_t f[F > 0 ? F : 1];
-            Op::load_src(args, jt + threadIdx.x, x, f);
-            char *rec = smem + SM::rec * threadIdx.x;
+    // one sweep over the chunk's sources in the direct (E false) or the
+    // expanded form
+    auto sweep = [&](auto form) {
+        constexpr bool E = decltype(form)::value;
+        using Ox = std::conditional_t<E, Op, Op>;   // dependent: pair_x/flush_x only when E
+        for (int jt = j0; jt < j1; jt += kTileJ) {
+            const int cnt = min(kTileJ, j1 - jt);
+            __syncthreads();
+            if ((int)threadIdx.x < cnt) {
+                double x[D];
+                lds_t f[F > 0 ? F : 1];
+                Op::load_src(args, jt + threadIdx.x, x, f);
+                char *rec = smem + SM::rec * threadIdx.x;
 #pragma unroll
-            for (int c = 0; c < F; ++c) reinterpret_cast<lds_t *>(rec)[c] = f[c];
-            float s2 = 0.0f;
+                for (int c = 0; c < F; ++c) reinterpret_cast<lds_t *>(rec)[c] = f[c];
+                float s2 = 0.0f;
 #pragma unroll
-            for (int c = 0; c < D; ++c) {
-                const crd_t v = to_crd(x[c], c);
-                reinterpret_cast<crd_t *>(rec + SM::xoff)[c] = v;
-                s2 += (float)v * (float)v;
+                for (int c = 0; c < D; ++c) {
+                    const crd_t v = to_crd(x[c], c);
+                    reinterpret_cast<crd_t *>(rec + SM::xoff)[c] = v;
+                    s2 += (float)v * (float)v;
+                }
+                if constexpr (E) reinterpret_cast<crd_t *>(rec + SM::xoff)[D] = s2;
+                if constexpr (Op::kCull) s2buf[threadIdx.x] = s2;
             }
-            if constexpr (Op::kCull) s2buf[threadIdx.x] = s2;
-        }
-        __syncthreads();
-        const int share = (cnt + kWaves - 1) / kWaves;   // <= 64
-        const int t0 = wave * share, t1 = min(t0 + share, cnt);
-        if (t0 >= t1) continue;
-        if constexpr (Op::kCull) {
-            const float ms2 = wave_min(t0 + lane < t1 ? s2buf[t0 + lane] : __builtin_inff());
-            const float gap = __builtin_sqrtf(ms2) - rt;
-            if (__builtin_amdgcn_readfirstlane((int)(gap > 0.0f && gap * gap > (float)args.cut2))) continue;
-        }
-        typename Op::Acc acc[R];
+            __syncthreads();
+            const int share = (cnt + kWaves - 1) / kWaves;   // <= 64
+            const int t0 = wave * share, t1 = min(t0 + share, cnt);
+            if (t0 >= t1) continue;
+            if constexpr (Op::kCull) {
+                const float ms2 = wave_min(t0 + lane < t1 ? s2buf[t0 + lane] : __builtin_inff());
+                const float ms = __builtin_sqrtf(ms2);
+                bool skip;
+                if constexpr (E) {
+                    const float gap = ms - 2.0f * rt;
+                    skip = gap > 0.0f && ms * gap > (float)args.cut2;
+                } else {
+                    const float gap = ms - rt;
+                    skip = gap > 0.0f && gap * gap > (float)args.cut2;
+                }
+                if (__builtin_amdgcn_readfirstlane((int)skip)) continue;
+            }
+            typename Op::Acc acc[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) Op::zero(acc[r]);
+            for (int r = 0; r < R; ++r) Op::zero(acc[r]);
 #pragma unroll 2
-        for (int t = t0; t < t1; ++t) {
-            crd_t xj[D];
-            lds_t fj[F > 0 ? F : 1];
-            const char *rec = smem + SM::rec * t;
+            for (int t = t0; t < t1; ++t) {
+                crd_t xj[E ? D + 1 : D];
+                lds_t fj[F > 0 ? F : 1];
+                const char *rec = smem + SM::rec * t;
 #pragma unroll
-            for (int c = 0; c < F; ++c) fj[c] = reinterpret_cast<const lds_t *>(rec)[c];
+                for (int c = 0; c < F; ++c) fj[c] = reinterpret_cast<const lds_t *>(rec)[c];
 #pragma unroll
-            for (int c = 0; c < D; ++c) xj[c] = reinterpret_cast<const crd_t *>(rec + SM::xoff)[c];
+                for (int c = 0; c < (E ? D + 1 : D); ++c) xj[c] = reinterpret_cast<const crd_t *>(rec + SM::xoff)[c];
 #pragma unroll
-            for (int r = 0; r < R; ++r) Op::pair(args, tg[r], xi[r], xj, fj, acc[r]);
+                for (int r = 0; r < R; ++r) {
+                    if constexpr (E) Ox::pair_x(args, tg[r], xm[r], xj, fj, acc[r]);
+                    else Op::pair(args, tg[r], xi[r], xj, fj, acc[r]);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if constexpr (E) Ox::flush_x(args, acc[r], tot[r], xi[r], sc[r]);
+                else Op::flush(args, acc[r], tot[r]);
+            }
         }
-#pragma unroll
-        for (int r = 0; r < R; ++r) Op::flush(args, acc[r], tot[r]);
+    };
+    if constexpr (kExp) {
+        // rtw2 is the same in every wave (all hold the block's targets)
+        if (__builtin_amdgcn_readfirstlane((int)(rtw2 <= args.xr2))) sweep(std::true_type{});
+        else sweep(std::false_type{});
+    } else {
+        sweep(std::false_type{});
     }
 
     // fixed-order combination of the four waves' totals

@@ -148,8 +148,8 @@ void dispatch_all(int dim, int order, int prec, F &&f)
 
 }  // namespace
 
-int kv_targets_per_block(int prec) { return prec == kFP32 ? 128 : 64; }
-int bwd_targets_per_block(int prec) { return prec == kFP32 ? 128 : 64; }
+int kv_targets_per_block(int prec) { return prec == kFP32 ? 64 * kKvR32 : 64; }
+int bwd_targets_per_block(int prec) { return prec == kFP32 ? 64 * kBwdR32 : 64; }
 
 PairSplit plan_kv(int ni, int nj, int prec, int target_blocks)
 {
@@ -164,22 +164,33 @@ PairSplit plan_bwd(int ni, int nj, int prec, int target_blocks)
 PairSplit plan_single(int ni, int nj, int target_blocks) { return make_split_ti(ni, nj, 64, target_blocks); }
 
 namespace {
-std::atomic<bool> g_pair_culling{true};
+std::atomic<bool> g_pair_culling{true}, g_pair_expansion{true};
 }
 
 void set_pair_culling(bool on) { g_pair_culling.store(on); }
 bool pair_culling() { return g_pair_culling.load(); }
+void set_pair_expansion(bool on) { g_pair_expansion.store(on); }
+bool pair_expansion() { return g_pair_expansion.load(); }
 
 void launch_kv(int dim, int order, int prec, const double *qi, int ni, const double *qj, const double *a, int nj,
                double width, const PairSplit &sp, double *part, hipStream_t st)
 {
     if (ni <= 0) return;
     dispatch_all(dim, order, prec, [&](auto Dc, auto Oc, auto Pc) {
-        constexpr int R = decltype(Pc)::fp32 ? 2 : 1;
+        constexpr int R = decltype(Pc)::fp32 ? kKvR32 : 1;
         using Op = OpKV<Dc(), Oc(), decltype(Pc), R>;
         using real = typename Op::real;
         const double iw = 1.0 / width, iw2 = 1.0 / (width * width);
-        typename Op::Args A{qi, qj, a, ni, nj, (real)iw, (real)iw2, cull_r2(pair_culling()), Op::KF::kappa(iw, iw2)};
+        typename Op::Args A{qi,
+                            qj,
+                            a,
+                            ni,
+                            nj,
+                            (real)iw,
+                            (real)iw2,
+                            cull_r2(pair_culling()),
+                            expand_r2(pair_expansion()),
+                            Op::KF::kappa(iw, iw2)};
         launch_pairs<Op>(A, ni, nj, sp, 0, part, st);
     });
 }
@@ -208,11 +219,11 @@ void launch_bwd_pairs(int dim, int order, int prec, const double *q, const doubl
                       double width, const PairSplit &sp, double *part, hipStream_t st)
 {
     dispatch_all(dim, order, prec, [&](auto Dc, auto Oc, auto Pc) {
-        constexpr int R = decltype(Pc)::fp32 ? 2 : 1;
+        constexpr int R = decltype(Pc)::fp32 ? kBwdR32 : 1;
         using Op = OpBwd<Dc(), Oc(), decltype(Pc), R>;
         using real = typename Op::real;
         const double iw = 1.0 / width, iw2 = 1.0 / (width * width), kap = Op::KF::kappa(iw, iw2);
-        typename Op::Args A{q,   a,         p, n, (real)iw, (real)iw2, cull_r2(pair_culling()),
+        typename Op::Args A{q,   a,   p, n, (real)iw, (real)iw2, cull_r2(pair_culling()), expand_r2(pair_expansion()),
                             kap, Op::KF::cmul(iw, iw2) / kap};
         launch_pairs<Op>(A, n, n, sp, 0, part, st);
     });

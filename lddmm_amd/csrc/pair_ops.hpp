@@ -10,6 +10,11 @@
 namespace lddmm {
 namespace ops {
 
+// targets per lane of the FP32 point-kernel launches (FP64: one); the split
+// plans and the batched work items use 64 times these per block.  Four for
+// K.a measured slower than two (larger tiles, fewer resident waves).
+constexpr int kKvR32 = 2, kBwdR32 = 2;
+
 // ---------------------------------------------------------------------------
 // v_i = sum_j k(qi_i, qj_j) a_j   (forward flow, multiplyKernel)
 // ---------------------------------------------------------------------------
@@ -20,12 +25,14 @@ struct OpKV {
     using lds_t = real;
     using crd_t = real;
     static constexpr bool kCull = P::fp32 && ORDER == -1;
+    static constexpr bool kExpand = kCull;   // expanded form (pair_driver.hpp)
     using KF = KernelFnT<ORDER, real, D_>;
     struct Args {
         const double *qi, *qj, *a;
         int ni, nj;
         real iw, iw2;
         float cut2;   // culling range (cull_r2); +inf: none
+        float xr2;    // expanded-form radius bound (expand_r2)
         double kap;   // KF::kappa
     };
     __device__ __forceinline__ static double crd_scale(const Args &A) { return A.kap; }
@@ -70,46 +77,66 @@ struct OpKV {
 #pragma unroll
         for (int c = 0; c < D; ++c) tot[c] += (double)a.v[c];
     }
+    // expanded form: exp2(-t), t = |x_j|^2 - 2 x_i.x_j
+    __device__ __forceinline__ static void pair_x(const Args &, const Tgt &, const float (&m)[D],
+                                                  const float (&xj)[D + 1], const lds_t (&fj)[F], Acc &acc)
+    {
+        float t = xj[D];
+#pragma unroll
+        for (int c = 0; c < D; ++c) t = __builtin_fmaf(m[c], xj[c], t);
+        const float k = __builtin_amdgcn_exp2f(-t);
+#pragma unroll
+        for (int c = 0; c < D; ++c) acc.v[c] += k * fj[c];
+    }
+    __device__ __forceinline__ static void flush_x(const Args &, const Acc &a, double (&tot)[OUT],
+                                                   const crd_t (&)[D], float sc)
+    {
+#pragma unroll
+        for (int c = 0; c < D; ++c) tot[c] += (double)a.v[c] * (double)sc;
+    }
 };
 
 // ---------------------------------------------------------------------------
 // Fused backward pass of the adjoint (Cpp/src/objgrd.cpp:76-84): one kernel
 // evaluation per pair feeds K(q) amp and the dq sum of l^T K(q) r with
 // amp = a - p, l = amp - a/2, r = a.  Outputs: [K amp (D), dq (D)].
+// The dq weight of a pair, l_i.a_j + l_j.a_i, equals -p_i.a_j + a_i.amp_j,
+// so a source carries a_j and amp_j (2D fields) and a target a_i and -p_i.
 // R_ targets per lane (two in FP32, where registers allow).
 // ---------------------------------------------------------------------------
 template <int D_, int ORDER, class P, int R_ = 1>
 struct OpBwd {
-    static constexpr int D = D_, F = 3 * D_, OUT = 2 * D_, R = R_;
+    static constexpr int D = D_, F = 2 * D_, OUT = 2 * D_, R = R_;
     using real = typename P::real;
     using lds_t = real;
     using crd_t = real;
     static constexpr bool kCull = P::fp32 && ORDER == -1;
+    static constexpr bool kExpand = kCull;   // expanded form (pair_driver.hpp)
     using KF = KernelFnT<ORDER, real, D_>;
     struct Args {
         const double *q, *a, *p;
         int n;
         real iw, iw2;
         float cut2;        // culling range (cull_r2); +inf: none
+        float xr2;         // expanded-form radius bound (expand_r2)
         double kap, pds;   // KF::kappa; dq flush factor KF::cmul / kappa
     };
     __device__ __forceinline__ static double crd_scale(const Args &A) { return A.kap; }
     struct Tgt {
-        real a[D], l[D];
+        real a[D], np[D];   // a_i, -p_i
     };
     struct Acc {
-        real g[D], pd[D];
+        real g[D], pd[D], sf;   // sf: expanded form's sum of the dq weights
     };
     __device__ __forceinline__ static void load_src(const Args &A, int j, double (&x)[D], lds_t (&f)[F])
     {
 #pragma unroll
         for (int c = 0; c < D; ++c) {
             const size_t o = (size_t)c * A.n + j;
-            double aj = A.a[o], amp = aj - A.p[o];
+            const double aj = A.a[o];
             x[c] = A.q[o];
             f[c] = (real)aj;
-            f[D + c] = (real)amp;
-            f[2 * D + c] = (real)(amp - 0.5 * aj);
+            f[D + c] = (real)(aj - A.p[o]);
         }
     }
     __device__ __forceinline__ static void load_tgt(const Args &A, int i, Tgt &t, double (&x)[D])
@@ -117,16 +144,16 @@ struct OpBwd {
 #pragma unroll
         for (int c = 0; c < D; ++c) {
             const size_t o = (size_t)c * A.n + (i >= 0 ? i : 0);
-            double ai = i >= 0 ? A.a[o] : 0.0, amp = ai - (i >= 0 ? A.p[o] : 0.0);
             x[c] = i >= 0 ? A.q[o] : 0.0;
-            t.a[c] = (real)ai;
-            t.l[c] = (real)(amp - 0.5 * ai);
+            t.a[c] = i >= 0 ? (real)A.a[o] : real(0);
+            t.np[c] = i >= 0 ? (real)(-A.p[o]) : real(0);
         }
     }
     __device__ __forceinline__ static void zero(Acc &a)
     {
 #pragma unroll
         for (int c = 0; c < D; ++c) a.g[c] = a.pd[c] = real(0);
+        a.sf = real(0);
     }
     __device__ __forceinline__ static void pair(const Args &A, const Tgt &t, const crd_t (&xi)[D],
                                                 const crd_t (&xj)[D], const lds_t (&fj)[F], Acc &acc)
@@ -141,9 +168,9 @@ struct OpBwd {
         KF::eval(r2, A.iw, A.iw2, k, cf);
         real lr = real(0);
 #pragma unroll
-        for (int c = 0; c < D; ++c) lr += t.l[c] * fj[c];
+        for (int c = 0; c < D; ++c) lr += t.np[c] * fj[c];
 #pragma unroll
-        for (int c = 0; c < D; ++c) lr += fj[2 * D + c] * t.a[c];
+        for (int c = 0; c < D; ++c) lr += fj[D + c] * t.a[c];
         const real f = lr * cf;
 #pragma unroll
         for (int c = 0; c < D; ++c) {
@@ -158,6 +185,38 @@ struct OpBwd {
             tot[c] += (double)a.g[c];
             if constexpr (P::fp32) tot[D + c] += (double)a.pd[c] * A.pds;
             else tot[D + c] += a.pd[c];
+        }
+    }
+    // expanded form: k = exp2(-t) (cf = k for the Gaussian); the dq sum
+    // sum_j f (x_i - x_j) is carried as sum_j f and sum_j f x_j
+    __device__ __forceinline__ static void pair_x(const Args &, const Tgt &t, const float (&m)[D],
+                                                  const float (&xj)[D + 1], const lds_t (&fj)[F], Acc &acc)
+    {
+        float s = xj[D];
+#pragma unroll
+        for (int c = 0; c < D; ++c) s = __builtin_fmaf(m[c], xj[c], s);
+        const float k = __builtin_amdgcn_exp2f(-s);
+        float lr = 0.0f;
+#pragma unroll
+        for (int c = 0; c < D; ++c) lr += t.np[c] * fj[c];
+#pragma unroll
+        for (int c = 0; c < D; ++c) lr += fj[D + c] * t.a[c];
+        const float f = lr * k;
+        acc.sf += f;
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            acc.g[c] += k * fj[D + c];
+            acc.pd[c] += f * xj[c];
+        }
+    }
+    __device__ __forceinline__ static void flush_x(const Args &A, const Acc &a, double (&tot)[OUT],
+                                                   const crd_t (&xi)[D], float sc)
+    {
+        const double s = (double)sc;
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            tot[c] += (double)a.g[c] * s;
+            tot[D + c] += ((double)xi[c] * (double)a.sf - (double)a.pd[c]) * (s * A.pds);
         }
     }
 };

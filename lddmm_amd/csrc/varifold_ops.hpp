@@ -74,6 +74,7 @@ struct OpVf {
     using lds_t = real;
     using crd_t = real;
     static constexpr bool kCull = P::fp32 && CEN == kCenG;
+    static constexpr bool kExpand = kCull;   // expanded form (pair_driver.hpp)
     struct Args {
         const double *ci, *ui;   // targets: deformed element centres / directions
         int ni;
@@ -81,6 +82,7 @@ struct OpVf {
         int nj;
         real ics2, ids2;
         float cut2;   // culling range (cull_r2); +inf: none
+        float xr2;    // expanded-form radius bound (expand_r2)
         double kap;   // FP32: cen_kappa of the staged offsets
     };
     __device__ __forceinline__ static double crd_scale(const Args &A) { return A.kap; }
@@ -171,6 +173,54 @@ struct OpVf {
                 tot[2 + c] += a.g[c];
                 tot[2 + D + c] += a.u[c];
             }
+        }
+    }
+    // expanded form (Gaussian position kernel): kc = exp2(-t); the g-sum
+    // sum_j wk (c_i - c_j) is c_i E - sum_j wk c_j with E the e-sum
+    __device__ __forceinline__ static void pair_x(const Args &A, const Tgt &t, const float (&m)[D],
+                                                  const float (&xj)[D + 1], const lds_t (&fj)[F], Acc &acc)
+    {
+        float s = xj[D], ang = 0.0f;
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            s = __builtin_fmaf(m[c], xj[c], s);
+            ang += t.u[c] * fj[c];
+        }
+        const float w = __builtin_amdgcn_exp2f(-s) * fj[D];
+        if constexpr (kBinet32) {
+            const float wa = w * ang;
+            const float wk = wa * ang;
+            acc.e += wk;
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                acc.g[c] += wk * xj[c];
+                acc.u[c] += wa * fj[c];
+            }
+        } else {
+            float dku;
+            const float ku = dir_value<DIR, float>(ang, A.ids2, dku);
+            const float wk = w * ku;
+            const float wd = w * dku;
+            acc.e += wk;
+            if constexpr (DIR != kDirB) acc.s += wk - wd * ang;
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                acc.g[c] += wk * xj[c];
+                acc.u[c] += wd * fj[c];
+            }
+        }
+    }
+    __device__ __forceinline__ static void flush_x(const Args &A, const Acc &a, double (&tot)[OUT],
+                                                   const crd_t (&xi)[D], float sc)
+    {
+        const double s = (double)sc, e = (double)a.e * s;
+        tot[0] += e;
+        tot[1] += DIR == kDirB ? -e : (double)a.s * s;
+        const double gs = s / A.kap, us = (kBinet32 ? 2.0 : 1.0) * s;
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            tot[2 + c] += ((double)xi[c] * (double)a.e - (double)a.g[c]) * gs;
+            tot[2 + D + c] += (double)a.u[c] * us;
         }
     }
 };
