@@ -48,16 +48,16 @@ TRANS_PER_PAIR = {("fwd_pairs", -1): 1, ("fwd_pairs", 1): 2, ("bwd_pairs", -1): 
 
 # Per-launch counter figures of the pair kernels on the sphere->cube workload, from
 # profiles/pair_kernels_sphere_cube.md (rocprofv3 PMC passes on this build):
-# traffic = HBM bytes per launch (FETCH_SIZE x 1.9925 + WRITE_SIZE, calibrated on a
+# traffic = HBM bytes per launch (FETCH_SIZE x 1.9921 + WRITE_SIZE, calibrated on a
 # copy kernel of known size), valu_busy = AMD's VALUBusy, valu_issue = share of SIMD
 # issue slots at 4 cycles per wave64 VALU instruction.
 ROCPROF_SPHERE_CUBE = {
-    ("fwd_pairs", "f64"): dict(traffic=5.635e6, valu_busy=0.76, valu_issue=0.72),
-    ("bwd_pairs", "f64"): dict(traffic=1.235e7, valu_busy=0.75, valu_issue=0.73),
-    ("vf_pairs", "f64"): dict(traffic=1.322e7, valu_busy=0.78, valu_issue=0.80),
-    ("fwd_pairs", "f32"): dict(traffic=8.066e6, valu_busy=0.83, valu_issue=0.75),
-    ("bwd_pairs", "f32"): dict(traffic=1.727e7, valu_busy=0.87, valu_issue=0.82),
-    ("vf_pairs", "f32"): dict(traffic=1.323e7, valu_busy=0.75, valu_issue=0.82),
+    ("fwd_pairs", "f64"): dict(traffic=8.064e6, valu_busy=0.85, valu_issue=0.81),
+    ("bwd_pairs", "f64"): dict(traffic=1.725e7, valu_busy=0.85, valu_issue=0.83),
+    ("vf_pairs", "f64"): dict(traffic=2.668e7, valu_busy=0.85, valu_issue=0.87),
+    ("fwd_pairs", "f32"): dict(traffic=1.239e7, valu_busy=0.90, valu_issue=0.81),
+    ("bwd_pairs", "f32"): dict(traffic=2.612e7, valu_busy=0.94, valu_issue=0.89),
+    ("vf_pairs", "f32"): dict(traffic=2.670e7, valu_busy=0.75, valu_issue=0.81),
 }
 
 
