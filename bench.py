@@ -133,7 +133,8 @@ def batch_bench(count, nmin, nmax, itr, seed=4, train_count=64):
     (zero initial momentum, stop at a relative energy decrease below 1e-6 or
     `itr` iterations) against the batched FP64 run, the batched FP32
     policy run to each plain run's final energy within 1e-6 relative, and
-    the batched FP32 policy started from the learned warm start.
+    the batched FP32 policy started from the learned warm start.  Batched
+    times include the batch's set-up, as plain times include each run's.
 
     Warm-start protocol: the network is trained on `train_count` problems
     of the same generator (other seeds) solved by the batched FP32 run; that
@@ -163,8 +164,12 @@ def batch_bench(count, nmin, nmax, itr, seed=4, train_count=64):
         return lddmm_amd.Batch(ps, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
                                k["knl_order"], k["time_stp"], T, precision=prec, chunk=chunk)
 
+    # the batch's set-up (spatial orders, uploads, target terms, chunk autotune)
+    # counts toward every batched time, as each plain run's set-up counts toward its
+    t0 = time.perf_counter()
     B = make(probs, "fp32")
     chunk, pairs = B.info()
+    t_setup = time.perf_counter() - t0
     a64, i64, r64 = B.run(itr, 0.0, 0.0, 0.5, 5, precision="fp64", rel_energy_tol=1e-6)
     a32, i32, r32 = B.run(3 * itr, 0.0, 0.0, 0.5, 5, precision="fp32", targets=targets)
     B64 = make(probs, "fp64", chunk)
@@ -174,7 +179,7 @@ def batch_bench(count, nmin, nmax, itr, seed=4, train_count=64):
         return (f - e_ref) / np.abs(e_ref)
 
     rel = rel_to_plain(a32)
-    t64, t32 = i64[0]["seconds"], i32[0]["seconds"]
+    t64, t32 = i64[0]["seconds"] + t_setup, i32[0]["seconds"] + t_setup
 
     # learned warm start: one-off training on other seeds of the generator
     t0 = time.perf_counter()
@@ -201,13 +206,13 @@ def batch_bench(count, nmin, nmax, itr, seed=4, train_count=64):
         for j, i in enumerate(missed):
             aw[i] = af[j]
         its_fb = [r["iterations"] for r in ifb]
-    t_warm = time.perf_counter() - t0
+    t_warm = time.perf_counter() - t0 + t_setup
     rel_w = rel_to_plain(aw)
     its_w = [r["iterations"] for r in iw]
     B.close()
     B64.close()
     return dict(problems=count, n_range=[int(min(B.n)), int(max(B.n))], chunk=chunk,
-                pairs_per_eval_total=float(pairs.sum()),
+                pairs_per_eval_total=float(pairs.sum()), batch_setup_seconds=t_setup,
                 plain_fp64_sequential=dict(seconds=t_seq, registrations_per_s=count / t_seq,
                                            median_iterations=float(np.median([i["iterations"] for i in seq]))),
                 batched_fp64=dict(seconds=t64, registrations_per_s=count / t64, rounds=r64,
@@ -351,11 +356,13 @@ def c4_report(count=256, nmin=500, nmax=20000, itr=300, seed=4):
                   file=sys.stderr, flush=True)
     t_seq = time.perf_counter() - t0
     e_ref = np.array([i["fcn"] for i in seq])
+    t0 = time.perf_counter()   # set-up counts toward the batched time (see batch_report)
     B = lddmm_amd.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
                         k["knl_order"], k["time_stp"], k["time_num"], precision="fp32", chunk=0)
     chunk, pairs = B.info()
+    t_setup = time.perf_counter() - t0
     a32, i32, r32 = B.run(3 * itr, 0.0, 0.0, 0.5, 5, precision="fp32", targets=e_ref * (1.0 + 1e-6), progress=25)
-    t32 = i32[0]["seconds"]
+    t32 = i32[0]["seconds"] + t_setup
     B.close()
     B64 = lddmm_amd.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
                           k["knl_order"], k["time_stp"], k["time_num"], precision="fp64", chunk=chunk)
@@ -370,7 +377,7 @@ def c4_report(count=256, nmin=500, nmax=20000, itr=300, seed=4):
                    fp32_status=int(i32[i]["status"]), fp32_phase_iterations=int(i32[i]["fp32_iterations"]))
               for i in range(count) if not i32[i]["reached_target"]]
     return dict(problems=count, n_range=[int(min(B.n)), int(max(B.n))], chunk=chunk,
-                pairs_per_eval_total=float(pairs.sum()),
+                pairs_per_eval_total=float(pairs.sum()), batch_setup_seconds=t_setup,
                 plain_fp64_sequential=dict(seconds=t_seq, registrations_per_s=count / t_seq,
                                            median_iterations=float(np.median([i["iterations"] for i in seq]))),
                 batched_fp32=dict(seconds=t32, registrations_per_s=count / t32, rounds=r32,

@@ -17,6 +17,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <atomic>
+#include <exception>
+#include <thread>
 #include <tuple>
 
 #include <unistd.h>
@@ -28,6 +31,33 @@
 namespace lddmm {
 
 namespace {
+
+// fn(p) for every problem p < P on up to 16 host threads (the set-up loops
+// write disjoint per-problem ranges); the first exception is rethrown
+template <class Fn>
+void for_each_problem(int P, Fn &&fn)
+{
+    const int hw = (int)std::thread::hardware_concurrency();
+    const int nt = std::max(1, std::min({P, 16, hw > 0 ? hw : 1}));
+    if (nt == 1) {
+        for (int p = 0; p < P; ++p) fn(p);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::exception_ptr> err(nt);
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t)
+        pool.emplace_back([&, t] {
+            try {
+                for (int p = next.fetch_add(1); p < P; p = next.fetch_add(1)) fn(p);
+            } catch (...) {
+                err[t] = std::current_exception();
+            }
+        });
+    for (std::thread &th : pool) th.join();
+    for (std::exception_ptr &e : err)
+        if (e) std::rethrow_exception(e);
+}
 
 using namespace ops;
 
@@ -626,7 +656,7 @@ Batch::Batch(const BatchSpec &spec, const double *q0, const int *elm, const doub
     // host layout [E_p][V]:
     vperm_.resize(ntot_);
     std::vector<int> selm((size_t)D * etot_), tperm(std::max<long long>(ttot_, 1));
-    for (int p = 0; p < P; ++p) {
+    for_each_problem(P, [&](int p) {
         const int n = spec_.n[p], ne = spec_.ne[p], nt = spec_.nt[p];
         const double *qp = q0 + (size_t)D * vo_[p];
         const int *ev = elm + (size_t)D * eo_[p];
@@ -648,7 +678,7 @@ Batch::Batch(const BatchSpec &spec, const double *q0, const int *elm, const doub
         const double *tp = nt > 0 ? tc + (size_t)D * to_[p] : nullptr;
         const std::vector<int> tq = spatial_order(nt, D, [&](int i, int c) { return tp[(size_t)i * D + c]; });
         for (int s = 0; s < nt; ++s) tperm[to_[p] + s] = tq[s];
-    }
+    });
     elm = selm.data();
 
     // adjacency tables per problem (getElmIfo, Cpp/src/getElmIfo.m), padded to one row count
@@ -663,7 +693,7 @@ Batch::Batch(const BatchSpec &spec, const double *q0, const int *elm, const doub
         rows_ = std::max(rows_, 1 + mx * D);
     }
     std::vector<int> hifo((size_t)rows_ * ntot_, 0);
-    for (int p = 0; p < P; ++p) {
+    for_each_problem(P, [&](int p) {
         const int n = spec_.n[p], ne = spec_.ne[p];
         const int *ev = elm + (size_t)D * eo_[p];
         int *blk = hifo.data() + (size_t)rows_ * vo_[p];   // [rows][n]
@@ -682,13 +712,13 @@ Batch::Batch(const BatchSpec &spec, const double *q0, const int *elm, const doub
                 }
                 ++count;
             }
-    }
+    });
     // host layout -> per-problem SoA blocks
     std::vector<double> hq((size_t)D * ntot_), htc((size_t)D * std::max<long long>(ttot_, 1)),
         htu((size_t)D * std::max<long long>(ttot_, 1));
     std::vector<int> helm((size_t)D * etot_);
     std::vector<double> hta(std::max<long long>(ttot_, 1));
-    for (int p = 0; p < P; ++p) {
+    for_each_problem(P, [&](int p) {
         const int n = spec_.n[p], ne = spec_.ne[p], nt = spec_.nt[p];
         for (int i = 0; i < n; ++i)
             for (int c = 0; c < D; ++c)
@@ -704,7 +734,7 @@ Batch::Batch(const BatchSpec &spec, const double *q0, const int *elm, const doub
             }
             hta[to_[p] + e] = ta[src];
         }
-    }
+    });
 
     chunk_ = spec_.chunk > 0 ? ((spec_.chunk + 63) / 64) * 64 : 1024;
     vchunk_ = 1024;

@@ -69,9 +69,9 @@
 // Differences x_i - x_j in the pair terms become x_i sum_j w - sum_j w x_j,
 // completed at the flush.  The FP32 rounding of t is relative to
 // |x_j|^2 + 2 |x_i||x_j| rather than to r2, so the expanded form is taken
-// only by blocks of small target radius, rt^2 <= args.xr2 = 4 (the kernel
-// falls to 1/16 over that radius): there its error stays within a small
-// factor of the direct form's, and exp2(-t) <= exp2(n_i) <= 16.  Wider
+// only by blocks of small target radius, rt^2 <= args.xr2 = 8 (the kernel
+// falls to 1/256 over that radius): there its error stays within a small
+// factor of the direct form's, and exp2(-t) <= exp2(n_i) <= 256.  Wider
 // tiles (small problems, narrow kernels) keep the direct form above.
 // Culling in the expanded form bounds t >= ms (ms - 2 rt) instead (then
 // exp2(-t) < 2^-64, and the factor exp2(-n_i) <= 1 keeps the bound).
@@ -225,7 +225,7 @@ __host__ __device__ inline float cull_r2(bool on) { return on ? (float)(64.0 * (
 
 // args.xr2 of an expanding operator: the largest squared target radius
 // (scaled units) of a block that takes the expanded form; -1: never
-__host__ __device__ inline float expand_r2(bool on) { return on ? 4.0f : -1.0f; }
+__host__ __device__ inline float expand_r2(bool on) { return on ? 8.0f : -1.0f; }
 
 template <class Op, class = void>
 struct ExpandOf : std::false_type {

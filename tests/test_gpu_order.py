@@ -246,3 +246,32 @@ def test_expanded_form_in_a_batch(gpu):
             assert abs(f - f64[i]) <= 1e-6 * abs(f64[i]) and rel_err(g, g64[i]) <= 1e-4
     B.close()
     B64.close()
+
+
+def test_expanded_form_2d(gpu):
+    """The expanded form on curves: a 1,600-point circle -> square, whose
+    128-point tiles are arcs small enough for it, against the direct form and
+    FP64 at two momentum scales."""
+    print("expanded form, 2D", flush=True)
+    w = G.circle_square(1600)
+    w["q0"] = w["q0"] * 0.95
+    w["time_num"], w["time_stp"] = 6, 0.2
+    n = len(w["q0"])
+    ifo = oracle.get_elm_ifo(w["elm"], n)
+    args = (w["q0"], w["elm"], ifo, w["tgt_cen"], w["tgt_vol"], w["tgt_dir"], w["cen_type"], w["cen_width"],
+            w["dir_type"], w["dir_width"], w["dis_wgt"], w["knl_order"], w["knl_width"], w["time_stp"], w["time_num"])
+    o32, o64 = gpu.Objective(*args, precision="fp32"), gpu.Objective(*args)
+    for seed, scale in ((0, 0.0), (3, 0.02)):
+        a = scale * G.normal(seed + 200, o32.varnum)
+        f64, g64 = o64(a)
+        try:
+            gpu.set_pair_expansion(False)
+            f0, g0 = o32(a)
+        finally:
+            gpu.set_pair_expansion(True)
+        f1, g1 = o32(a)
+        assert not np.array_equal(g0, g1)
+        for f, g in ((f0, g0), (f1, g1)):
+            assert abs(f - f64) <= 1e-6 * abs(f64) and rel_err(g, g64) <= 1e-4
+    o32.close()
+    o64.close()
