@@ -281,9 +281,10 @@ int lddmm_spatial_order(int32_t *perm, const double *ptsMat, int ptsNum, int dim
 /* Culling in the FP32 policy (no counterpart in the reference): a range of
  * sources whose Gaussian kernel values (deformation kernel, or the
  * varifold's 'G' position kernel) with a tile of targets are all below
- * 2^-64 is skipped; the terms left out are below 2^-64 times the terms'
- * other factors, far below the FP32 rounding of the sums.  On by default;
- * off computes every pair (results agree to FP64 rounding, for checking).
+ * 2^-48 is skipped; the terms left out are below 2^-48 times the terms'
+ * other factors, 24 binary orders below the FP32 rounding of the sums.  On
+ * by default; off computes every pair (results agree to a few FP64 ulps, for
+ * checking).
  * Process-wide, read at every launch. */
 int lddmm_set_pair_culling(int on);
 /* Expanded form of the FP32 Gaussian pair terms (no counterpart in the

@@ -158,8 +158,8 @@ def spatial_order(pts):
 
 def set_pair_culling(on):
     """Culling of FP32 Gaussian tile pairs whose kernel values are all below
-    2^-64 (default on); off computes every pair, with results that agree to
-    FP64 rounding."""
+    2^-48 (default on); off computes every pair, with results that agree to
+    a few FP64 ulps."""
     _check(lib().lddmm_set_pair_culling(int(bool(on))))
 
 

@@ -75,7 +75,7 @@ struct BatchView {
     int rows, T;
     double h, w;
     double *ecen, *edir, *evol, *dc, *dd, *efwd, *evf;
-    int cull;     // culling of FP32 Gaussian tile pairs (kernel values < 2^-64)
+    int cull;     // culling of FP32 Gaussian tile pairs (kernel values < 2^-48)
     int expand;   // expanded form of the FP32 Gaussian pair terms
 };
 

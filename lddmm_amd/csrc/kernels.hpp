@@ -9,9 +9,9 @@ namespace lddmm {
 
 // ---- pair sums over points (pair_kernels.hip) ---------------------------
 // Culling of FP32 Gaussian tile pairs whose kernel values are all below
-// 2^-64 (pair_driver.hpp); on by default, read at every launch (a batch keys
+// 2^-48 (pair_driver.hpp); on by default, read at every launch (a batch keys
 // its captured graphs on it).  Off computes every pair; the results agree to
-// FP64 rounding.
+// a few FP64 ulps.
 void set_pair_culling(bool on);
 bool pair_culling();
 // The expanded form of the FP32 Gaussian pair terms (pair_driver.hpp); on

@@ -44,17 +44,16 @@
 //
 // Culling (FP32 Gaussian kernels, kCull): a wave skips its range of staged
 // sources when every kernel value exp2(-r2) (scaled coordinates, KernelFnT)
-// between them and the block's targets is below 2^-64, i.e. r2 > 64.  Per
+// between them and the block's targets is below 2^-48, i.e. r2 > 48.  Per
 // block the driver takes the target tile's radius about the anchor (rt) and
 // per wave's range the smallest distance of a source from the anchor (ms);
-// when ms - rt exceeds sqrt(64) (with a 1e-4 relative margin for the FP32
-// rounding of both bounds and of r2: args.cut2 = 64 (1 + 1e-4)), no pair of
+// when ms - rt exceeds sqrt(48) (with a 1e-4 relative margin for the FP32
+// rounding of both bounds and of r2: args.cut2 = 48 (1 + 1e-4)), no pair of
 // the two can reach it.  Every term of a pair is a kernel value times factors
 // of the problem's own scale, so a skipped range changes a target's sum by
-// less than 2^-64 times the sum of its terms' factors: far below the FP32
-// rounding of the sums (2^-24 relative), and in practice below the FP64
-// rounding of the totals (culled and full sums agree to the last bits of
-// FP64).  With the spatially ordered problems this removes the far tiles of
+// less than 2^-48 times the sum of its terms' factors: 24 binary orders below
+// the FP32 unit roundoff of the sums, and in practice at the FP64 rounding of
+// the totals (culled and full sums agree to a few FP64 ulps).  With the spatially ordered problems this removes the far tiles of
 // narrow kernels (the varifold's Gaussian of width 0.1 on unit-sized shapes
 // skips most of its pairs).
 //
@@ -74,7 +73,7 @@
 // factor of the direct form's, and exp2(-t) <= exp2(n_i) <= 256.  Wider
 // tiles (small problems, narrow kernels) keep the direct form above.
 // Culling in the expanded form bounds t >= ms (ms - 2 rt) instead (then
-// exp2(-t) < 2^-64, and the factor exp2(-n_i) <= 1 keeps the bound).
+// exp2(-t) < 2^-48, and the factor exp2(-n_i) <= 1 keeps the bound).
 #pragma once
 
 #include "common.hpp"
@@ -219,9 +218,9 @@ struct KernelFnT<-1, float, 2> : KernelFnT<-1, float, 3> {
 constexpr int kWaves = kBlock / 64;
 
 // args.cut2 of a culling operator (FP32 Gaussian on coordinates scaled so
-// that k = exp2(-r2)): skipped pairs have k < 2^-64; a 1e-4 relative margin
+// that k = exp2(-r2)): skipped pairs have k < 2^-48; a 1e-4 relative margin
 // covers the FP32 rounding of the bounds; +inf turns culling off
-__host__ __device__ inline float cull_r2(bool on) { return on ? (float)(64.0 * (1.0 + 1e-4)) : __builtin_inff(); }
+__host__ __device__ inline float cull_r2(bool on) { return on ? (float)(48.0 * (1.0 + 1e-4)) : __builtin_inff(); }
 
 // args.xr2 of an expanding operator: the largest squared target radius
 // (scaled units) of a block that takes the expanded form; -1: never
