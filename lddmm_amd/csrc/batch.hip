@@ -46,14 +46,21 @@ void for_each_problem(int P, Fn &&fn)
     std::atomic<int> next{0};
     std::vector<std::exception_ptr> err(nt);
     std::vector<std::thread> pool;
-    for (int t = 0; t < nt; ++t)
-        pool.emplace_back([&, t] {
-            try {
-                for (int p = next.fetch_add(1); p < P; p = next.fetch_add(1)) fn(p);
-            } catch (...) {
-                err[t] = std::current_exception();
-            }
-        });
+    for (int t = 0; t < nt; ++t) {
+        try {
+            pool.emplace_back([&, t] {
+                try {
+                    for (int p = next.fetch_add(1); p < P; p = next.fetch_add(1)) fn(p);
+                } catch (...) {
+                    err[t] = std::current_exception();
+                }
+            });
+        } catch (...) {
+            break;   // no more threads: the started ones take every problem
+        }
+    }
+    if (pool.empty())
+        for (int p = next.fetch_add(1); p < P; p = next.fetch_add(1)) fn(p);
     for (std::thread &th : pool) th.join();
     for (std::exception_ptr &e : err)
         if (e) std::rethrow_exception(e);

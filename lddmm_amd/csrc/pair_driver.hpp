@@ -53,9 +53,10 @@
 // of the problem's own scale, so a skipped range changes a target's sum by
 // less than 2^-48 times the sum of its terms' factors: 24 binary orders below
 // the FP32 unit roundoff of the sums, and in practice at the FP64 rounding of
-// the totals (culled and full sums agree to a few FP64 ulps).  With the spatially ordered problems this removes the far tiles of
-// narrow kernels (the varifold's Gaussian of width 0.1 on unit-sized shapes
-// skips most of its pairs).
+// the totals (culled and full sums agree to a few FP64 ulps).  With the
+// spatially ordered problems this removes the far tiles of narrow kernels
+// (the varifold's Gaussian of width 0.1 on unit-sized shapes skips most of
+// its pairs).
 //
 // Expanded form (FP32 Gaussian operators with kExpand): with n_i = |x_i|^2
 // and t = |x_j|^2 - 2 x_i.x_j, r2 = n_i + t and exp2(-r2) =
