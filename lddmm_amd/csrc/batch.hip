@@ -70,7 +70,7 @@ using namespace ops;
 
 constexpr int kTile = 64;          // targets per work item (one per lane)
 constexpr int kVecBlk = 4096;      // elements per block of the batched vector kernels
-constexpr int kMaxBDots = 4;
+constexpr int kMaxBDots = 16;   // dot products per batched launch
 
 struct BatchView {
     const BatchDesc *desc;
@@ -477,23 +477,18 @@ __global__ __launch_bounds__(kBlock) void kb_dots2(const double *__restrict__ pa
 }
 
 // generic per-problem update over the block table
-enum VecOp : int { kCopy = 0, kCopyNeg, kLoop1, kScaleH0, kLoop2, kTry, kAccept };
+enum VecOp : int { kTry = 0, kAccept };
 
 struct VecArgs {
-    int op, m;
-    double *d;                 // direction (or xt for kTry)
-    const double *g, *x;       // gradient / iterate
-    const double *S, *Y;       // ring buffers [m][vtot]
-    double *Sw, *Yw;           // writable ring buffers (accept)
-    double *xw, *gw;           // writable iterate / gradient (accept)
+    int op;
+    double *d;                 // xt (kTry)
+    const double *g, *x;       // direction / iterate (kTry)
+    double *Sw, *Yw;           // ring buffers [m][vtot] (accept)
+    double *xw, *gw;           // iterate / gradient (accept)
     const double *xt, *gt;     // trial point (accept)
     long long vtot;
     const int *mask;           // per problem, 0 = skip
     const int *slot;           // per problem ring slot
-    const int *neg;            // per problem: negate after the update (loop2)
-    const double *sd, *yd;     // per problem scalars
-    const double *sy, *yy;     // per problem [m]
-    double *rec;               // per problem [m]
     const double *rel;         // per problem step (kTry)
 };
 
@@ -507,30 +502,10 @@ __global__ __launch_bounds__(kBlock) void kb_vec(VecArgs a, VecTab vt)
     const long long vo = vt.voff[p];
     const long long lo = (long long)b.y * kVecBlk, hi = min(vt.vn[p], lo + kVecBlk);
     const long long so = (long long)sl * a.vtot + vo;
-    double coef = 0.0;
-    switch (a.op) {
-    case kLoop1:
-        coef = a.sd[p] / a.sy[p * a.m + sl];
-        if (b.y == 0 && threadIdx.x == 0) a.rec[p * a.m + sl] = coef;
-        break;
-    case kScaleH0: coef = a.sy[p * a.m + sl] / a.yy[p * a.m + sl]; break;
-    case kLoop2: coef = a.rec[p * a.m + sl] - a.yd[p] / a.sy[p * a.m + sl]; break;
-    case kTry: coef = a.rel[p]; break;
-    default: break;
-    }
-    const bool neg = a.neg && a.neg[p];
+    const double coef = a.op == kTry ? a.rel[p] : 0.0;
     for (long long i = lo + threadIdx.x; i < hi; i += kBlock) {
         const long long k = vo + i;
         switch (a.op) {
-        case kCopy: a.d[k] = a.g[k]; break;
-        case kCopyNeg: a.d[k] = -a.g[k]; break;
-        case kLoop1: a.d[k] -= coef * a.Y[so + i]; break;
-        case kScaleH0: a.d[k] *= coef; break;
-        case kLoop2: {
-            double t = a.d[k] + coef * a.S[so + i];
-            a.d[k] = neg ? -t : t;
-            break;
-        }
         case kTry: a.d[k] = a.x[k] + coef * a.g[k]; break;   // d := xt, g := direction
         case kAccept: {
             double xn = a.xt[k], gn = a.gt[k];
@@ -545,15 +520,101 @@ __global__ __launch_bounds__(kBlock) void kb_vec(VecArgs a, VecTab vt)
     }
 }
 
-// curvature scalars of the newest pair into the per-slot tables
-__global__ void kb_store_pairs(const double *__restrict__ sdv, const double *__restrict__ ydv,
-                               const int *__restrict__ mask, const int *__restrict__ slot, double *__restrict__ sy,
-                               double *__restrict__ yy, int m, int P)
+// L-BFGS direction in the compact form of the two-loop recursion
+// (Cpp/src/getDirection.cpp).  With a_i = s_i.g, b_i = y_i.g and per problem
+// the tables SY[i][j] = s_i.y_j and YY[i][j] = y_i.y_j over the physical ring
+// slots, the recursion's coefficients follow from scalars alone:
+//   first loop (newest to oldest)
+//     alpha_i = (a_i - sum_{j newer} alpha_j SY[i][j]) / SY[i][i],
+//   H0 = gamma = SY[n][n] / YY[n][n] (n the newest pair),
+//   second loop (oldest to newest)
+//     beta_i = (gamma (b_i - sum_j alpha_j YY[i][j])
+//               + sum_{j older} (alpha_j - beta_j) SY[j][i]) / SY[i][i],
+// the same alpha and beta as the two loops (s_i.q and y_i.r re-associated),
+// and the direction is one pass,
+//   d = -gamma g + sum_j gamma alpha_j y_j - sum_j (alpha_j - beta_j) s_j.
+// One thread per problem; coef[p] = [g, y_0..y_{m-1}, s_0..s_{m-1}]
+// coefficients by slot.
+__global__ void kb_lbfgs_coef(const double *__restrict__ dots, const double *__restrict__ SY,
+                              const double *__restrict__ YY, const int *__restrict__ dir,
+                              const int *__restrict__ his, const int *__restrict__ newest, double *__restrict__ ab,
+                              double *__restrict__ coef, int m, int P)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P || !dir[p]) return;
+    double *c = coef + (size_t)p * (1 + 2 * m), *al = ab + (size_t)p * 2 * m, *be = al + m;
+    for (int j = 0; j < 1 + 2 * m; ++j) c[j] = 0.0;
+    const int h = his[p];
+    if (h == 0) {   // first iteration: d = -g
+        c[0] = -1.0;
+        return;
+    }
+    const double *sy = SY + (size_t)p * m * m, *yy = YY + (size_t)p * m * m;
+    const int n = newest[p];
+    auto slot = [&](int k) {   // k = 0: newest pair
+        const int i = n - k;
+        return i < 0 ? i + m : i;
+    };
+    for (int k = 0; k < h; ++k) {
+        const int i = slot(k);
+        double t = dots[(size_t)i * P + p];
+        for (int q = 0; q < k; ++q) t -= al[slot(q)] * sy[i * m + slot(q)];
+        al[i] = t / sy[i * m + i];
+    }
+    const double gam = sy[n * m + n] / yy[n * m + n];
+    for (int k = h - 1; k >= 0; --k) {
+        const int i = slot(k);
+        double t = dots[(size_t)(m + i) * P + p];
+        for (int q = 0; q < h; ++q) t -= al[slot(q)] * yy[i * m + slot(q)];
+        t *= gam;
+        for (int q = h - 1; q > k; --q) t += (al[slot(q)] - be[slot(q)]) * sy[slot(q) * m + i];
+        be[i] = t / sy[i * m + i];
+    }
+    c[0] = -gam;
+    for (int k = 0; k < h; ++k) {
+        const int j = slot(k);
+        c[1 + j] = gam * al[j];
+        c[1 + m + j] = -(al[j] - be[j]);
+    }
+}
+
+// d = coef_g g + sum over the filled slots (0 .. his-1) of coef_y Y + coef_s S
+__global__ __launch_bounds__(kBlock) void kb_dir(double *__restrict__ d, const double *__restrict__ g,
+                                                 const double *__restrict__ S, const double *__restrict__ Y,
+                                                 long long vtot, const double *__restrict__ coef,
+                                                 const int *__restrict__ dir, const int *__restrict__ his, int m,
+                                                 VecTab vt)
+{
+    const int2 b = vt.blk[blockIdx.x];
+    const int p = b.x;
+    if (!dir[p]) return;
+    const long long vo = vt.voff[p];
+    const long long lo = (long long)b.y * kVecBlk, hi = min(vt.vn[p], lo + kVecBlk);
+    const double *c = coef + (size_t)p * (1 + 2 * m);
+    const int h = his[p];
+    for (long long i = lo + threadIdx.x; i < hi; i += kBlock) {
+        const long long k = vo + i;
+        double v = c[0] * g[k];
+        for (int j = 0; j < h; ++j) v += c[1 + j] * Y[(size_t)j * vtot + k] + c[1 + m + j] * S[(size_t)j * vtot + k];
+        d[k] = v;
+    }
+}
+
+// the newest pair's row and column of SY and YY: dots[3j..3j+2][p] =
+// s_n.y_j, s_j.y_n, y_n.y_j for the filled slots j (n = newest)
+__global__ void kb_store_ring(const double *__restrict__ dots, const int *__restrict__ mask,
+                              const int *__restrict__ newest, const int *__restrict__ his, double *__restrict__ SY,
+                              double *__restrict__ YY, int m, int P)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P || !mask[p]) return;
-    sy[(size_t)p * m + slot[p]] = sdv[p];
-    yy[(size_t)p * m + slot[p]] = ydv[p];
+    const int n = newest[p];
+    double *sy = SY + (size_t)p * m * m, *yy = YY + (size_t)p * m * m;
+    for (int j = 0; j < his[p]; ++j) {
+        sy[n * m + j] = dots[(size_t)(3 * j) * P + p];
+        sy[j * m + n] = dots[(size_t)(3 * j + 1) * P + p];
+        yy[n * m + j] = yy[j * m + n] = dots[(size_t)(3 * j + 2) * P + p];
+    }
 }
 
 int cen_code(char t) { return t == 'G' ? 0 : 1; }
@@ -1193,12 +1254,12 @@ BatchRunResult Batch::run(double *x, const BatchLbfgsOptions &bo)
     }
     const int nent = (int)blk.size();
     Slab s;
-    double *g, *d, *xt, *gt, *S, *Y, *sc, *sy, *yy, *rec, *part, *rel_d;
+    double *g, *d, *xt, *gt, *S, *Y, *sc, *SYt, *YYt, *ab, *coef, *rdots, *part, *rel_d;
     int2 *blk_d;
     int *first_d, *count_d, *imask;
     long long *voff_d, *vn_d;
-    // scalar block: f, gg, gd, dd, sd, yd  (P each)
-    enum { F = 0, GG, GD, DDs, SD, YD, NS };
+    // scalar block: f, gg, gd, dd  (P each)
+    enum { F = 0, GG, GD, DDs, NS };
     s.reserve(&g, V);
     s.reserve(&d, V);
     s.reserve(&xt, V);
@@ -1206,9 +1267,14 @@ BatchRunResult Batch::run(double *x, const BatchLbfgsOptions &bo)
     s.reserve(&S, (size_t)V * m);
     s.reserve(&Y, (size_t)V * m);
     s.reserve(&sc, (size_t)NS * P);
-    s.reserve(&sy, (size_t)P * m);
-    s.reserve(&yy, (size_t)P * m);
-    s.reserve(&rec, (size_t)P * m);
+    // L-BFGS tables: SY[p][i][j] = s_i.y_j, YY[p][i][j] = y_i.y_j (ring slots),
+    // the recursion's alpha/beta, the direction's coefficients, and the
+    // batched dots of the direction (2m) and of an accepted pair (3m)
+    s.reserve(&SYt, (size_t)P * m * m);
+    s.reserve(&YYt, (size_t)P * m * m);
+    s.reserve(&ab, (size_t)P * 2 * m);
+    s.reserve(&coef, (size_t)P * (1 + 2 * m));
+    s.reserve(&rdots, (size_t)P * 3 * m);
     s.reserve(&part, (size_t)kMaxBDots * nent);
     s.reserve(&rel_d, P);
     s.reserve(&blk_d, nent);
@@ -1216,9 +1282,9 @@ BatchRunResult Batch::run(double *x, const BatchLbfgsOptions &bo)
     s.reserve(&count_d, P);
     s.reserve(&voff_d, P);
     s.reserve(&vn_d, P);
-    // int mask arrays: dir, trial, accept, newest slot, then per two-loop
-    // step the slot arrays of both loops and the negate flags
-    const int NM = 4 + 3 * m;
+    // int mask arrays: dir, trial, accept, newest slot, filled slots, then
+    // per ring slot j an array holding j where slot j is filled, else -1
+    const int NM = 5 + m;
     s.reserve(&imask, (size_t)NM * P);
     s.allocate();
     LDDMM_HIP(hipMemcpyAsync(blk_d, blk.data(), blk.size() * sizeof(int2), hipMemcpyHostToDevice, st_));
@@ -1241,11 +1307,14 @@ BatchRunResult Batch::run(double *x, const BatchLbfgsOptions &bo)
             if (c) (void)hipHostFree(c);
         }
     } hf{hs, hm, hrel};
-    int *m_dir = imask, *m_trial = imask + P, *m_acc = imask + 2 * P, *m_new = imask + 3 * P;
-    int *m_slot = imask + 4 * P;            // 2*m arrays of P
-    int *m_neg = imask + (4 + 2 * m) * P;   // m arrays of P
-    int *h_dir = hm, *h_trial = hm + P, *h_acc = hm + 2 * P, *h_new = hm + 3 * P, *h_slot = hm + 4 * P,
-        *h_neg = hm + (4 + 2 * m) * P;
+    int *m_dir = imask, *m_trial = imask + P, *m_acc = imask + 2 * P, *m_new = imask + 3 * P,
+        *m_his = imask + 4 * P, *m_ring = imask + 5 * P;   // m arrays of P
+    int *h_dir = hm, *h_trial = hm + P, *h_acc = hm + 2 * P, *h_new = hm + 3 * P, *h_his = hm + 4 * P,
+        *h_ring = hm + 5 * P;
+    auto set_ring = [&](int p, int filled) {
+        h_his[p] = filled;
+        for (int j = 0; j < m; ++j) h_ring[j * P + p] = j < filled ? j : -1;
+    };
     VecTab vt{blk_d, voff_d, vn_d};
 
     // synchronous: the pinned host masks are rewritten right after
@@ -1254,23 +1323,26 @@ BatchRunResult Batch::run(double *x, const BatchLbfgsOptions &bo)
         LDDMM_HIP(hipStreamSynchronize(st_));
     };
     auto vec = [&](VecArgs a) {
-        a.m = m;
         a.vtot = V;
         ProfScope ps(kProfLbfgsVector, st_);
         kb_vec<<<nent, kBlock, 0, st_>>>(a, vt);
         LDDMM_LAUNCH_CHECK();
     };
+    // nd dot products per problem into out[k*P + p], kMaxBDots per launch pair
     auto bdots = [&](int nd, const VRef *xs, const VRef *ys, const int *mask, double *out) {
-        BDotArgs a{};
-        for (int k = 0; k < nd; ++k) {
-            a.x[k] = xs[k];
-            a.y[k] = ys[k];
+        for (int k0 = 0; k0 < nd; k0 += kMaxBDots) {
+            const int nk = std::min(kMaxBDots, nd - k0);
+            BDotArgs a{};
+            for (int k = 0; k < nk; ++k) {
+                a.x[k] = xs[k0 + k];
+                a.y[k] = ys[k0 + k];
+            }
+            ProfScope ps(kProfLbfgsDots, st_);
+            kb_dots1<<<dim3(nent, nk), kBlock, 0, st_>>>(a, vt, mask, part, nent);
+            LDDMM_LAUNCH_CHECK();
+            kb_dots2<<<dim3(P, nk), kBlock, 0, st_>>>(part, nent, first_d, count_d, mask, out + (size_t)k0 * P, P);
+            LDDMM_LAUNCH_CHECK();
         }
-        ProfScope ps(kProfLbfgsDots, st_);
-        kb_dots1<<<dim3(nent, nd), kBlock, 0, st_>>>(a, vt, mask, part, nent);
-        LDDMM_LAUNCH_CHECK();
-        kb_dots2<<<dim3(P, nd), kBlock, 0, st_>>>(part, nent, first_d, count_d, mask, out, P);
-        LDDMM_LAUNCH_CHECK();
     };
     auto fetch = [&](int slot0, int nslots) {
         LDDMM_HIP(hipMemcpyAsync(hs + (size_t)slot0 * P, sc + (size_t)slot0 * P, sizeof(double) * nslots * P,
@@ -1356,80 +1428,28 @@ BatchRunResult Batch::run(double *x, const BatchLbfgsOptions &bo)
         for (int p = 0; p < P; ++p)
             if (st[p].active && st[p].need_dir) dirp.push_back(p);
         if (!dirp.empty()) {
-            int maxhis = 0;
             std::memset(hm, 0, sizeof(int) * NM * P);
-            for (int k = 0; k < 2 * m; ++k)
-                for (int p = 0; p < P; ++p) h_slot[k * P + p] = -1;
-            for (int p = 0; p < P; ++p) h_new[p] = -1;
+            for (int j = 0; j < m; ++j)
+                for (int p = 0; p < P; ++p) h_ring[j * P + p] = -1;
             for (int p : dirp) {
-                const int it = st[p].it + 1;
-                const int his = it <= m ? it - 1 : m;
-                maxhis = std::max(maxhis, his);
                 h_dir[p] = 1;
-                h_trial[p] = his == 0;   // d = -g
-                h_acc[p] = his > 0;      // d = g, then the two loops
-                h_new[p] = his > 0 ? st[p].new_idx : -1;
-                for (int h = 0; h < his; ++h) {
-                    int idx = st[p].new_idx - h;
-                    if (idx < 0) idx += m;
-                    h_slot[h * P + p] = idx;
-                }
-                int old = his < m ? 0 : st[p].new_idx + 1;
-                if (old == m) old = 0;
-                for (int h = 0; h < his; ++h) {
-                    int idx = old + h;
-                    if (idx >= m) idx -= m;
-                    h_slot[(m + h) * P + p] = idx;
-                    h_neg[h * P + p] = (h == his - 1);
-                }
+                h_new[p] = st[p].new_idx;
+                set_ring(p, std::min(st[p].it, m));   // pairs stored so far
             }
             upload_masks();
             {
-                VecArgs a{};
-                a.op = kCopyNeg;
-                a.d = d;
-                a.g = g;
-                a.mask = m_trial;
-                vec(a);
-                a.op = kCopy;
-                a.mask = m_acc;
-                vec(a);
-            }
-            for (int h = 0; h < maxhis; ++h) {
-                VRef xs[1] = {VRef{S, m_slot + h * P, V}}, ys[1] = {plain(d)};
-                bdots(1, xs, ys, nullptr, sc + (size_t)SD * P);
-                VecArgs a{};
-                a.op = kLoop1;
-                a.d = d;
-                a.Y = Y;
-                a.slot = m_slot + h * P;
-                a.sd = sc + (size_t)SD * P;
-                a.sy = sy;
-                a.rec = rec;
-                vec(a);
-            }
-            {
-                VecArgs a{};
-                a.op = kScaleH0;
-                a.d = d;
-                a.slot = m_new;
-                a.sy = sy;
-                a.yy = yy;
-                vec(a);
-            }
-            for (int h = 0; h < maxhis; ++h) {
-                VRef xs[1] = {VRef{Y, m_slot + (m + h) * P, V}}, ys[1] = {plain(d)};
-                bdots(1, xs, ys, nullptr, sc + (size_t)YD * P);
-                VecArgs a{};
-                a.op = kLoop2;
-                a.d = d;
-                a.S = S;
-                a.slot = m_slot + (m + h) * P;
-                a.neg = m_neg + h * P;   // negate on each problem's last second-loop step
-                a.yd = sc + (size_t)YD * P;
-                a.sy = sy;
-                a.rec = rec;
-                vec(a);
+                // a_j = s_j.g and b_j = y_j.g for the filled slots, one batched pass
+                std::vector<VRef> xs(2 * m), ys(2 * m, plain(g));
+                for (int j = 0; j < m; ++j) {
+                    xs[j] = VRef{S, m_ring + (size_t)j * P, V};
+                    xs[m + j] = VRef{Y, m_ring + (size_t)j * P, V};
+                }
+                bdots(2 * m, xs.data(), ys.data(), m_dir, rdots);
+                ProfScope ps(kProfLbfgsVector, st_);
+                kb_lbfgs_coef<<<ceil_div(P, 64), 64, 0, st_>>>(rdots, SYt, YYt, m_dir, m_his, m_new, ab, coef, m, P);
+                LDDMM_LAUNCH_CHECK();
+                kb_dir<<<nent, kBlock, 0, st_>>>(d, g, S, Y, V, coef, m_dir, m_his, m, vt);
+                LDDMM_LAUNCH_CHECK();
             }
             {
                 VRef xs[2] = {plain(g), plain(d)}, ys[2] = {plain(d), plain(d)};
@@ -1546,6 +1566,7 @@ BatchRunResult Batch::run(double *x, const BatchLbfgsOptions &bo)
                 q.new_idx = (q.new_idx == m - 1) ? 0 : q.new_idx + 1;
                 h_acc[p] = 1;
                 h_new[p] = q.new_idx;
+                set_ring(p, std::min(q.it, m));   // the accepted pair included
             }
             upload_masks();
             VecArgs a{};
@@ -1559,11 +1580,22 @@ BatchRunResult Batch::run(double *x, const BatchLbfgsOptions &bo)
             a.mask = m_acc;
             a.slot = m_new;
             vec(a);
-            VRef xs[2] = {VRef{S, m_new, V}, VRef{Y, m_new, V}}, ys[2] = {VRef{Y, m_new, V}, VRef{Y, m_new, V}};
-            bdots(2, xs, ys, m_acc, sc + (size_t)SD * P);
-            kb_store_pairs<<<ceil_div(P, 64), 64, 0, st_>>>(sc + (size_t)SD * P, sc + (size_t)YD * P, m_acc, m_new,
-                                                            sy, yy, m, P);
-            LDDMM_LAUNCH_CHECK();
+            {
+                // the new pair against every filled slot: s_n.y_j, s_j.y_n, y_n.y_j
+                std::vector<VRef> xs(3 * m), ys(3 * m);
+                for (int j = 0; j < m; ++j) {
+                    const VRef sj{S, m_ring + (size_t)j * P, V}, yj{Y, m_ring + (size_t)j * P, V};
+                    xs[3 * j] = VRef{S, m_new, V};
+                    ys[3 * j] = yj;
+                    xs[3 * j + 1] = sj;
+                    ys[3 * j + 1] = VRef{Y, m_new, V};
+                    xs[3 * j + 2] = VRef{Y, m_new, V};
+                    ys[3 * j + 2] = yj;
+                }
+                bdots(3 * m, xs.data(), ys.data(), m_acc, rdots);
+                kb_store_ring<<<ceil_div(P, 64), 64, 0, st_>>>(rdots, m_acc, m_new, m_his, SYt, YYt, m, P);
+                LDDMM_LAUNCH_CHECK();
+            }
             for (int p : acc) {
                 St &q = st[p];
                 const double fprev = q.fprev;
