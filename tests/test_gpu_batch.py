@@ -64,17 +64,23 @@ def test_batched_fp32_objgrd_error(gpu):
         obj.close()
 
 
-def test_batched_run_matches_single_runs(gpu):
+@pytest.mark.parametrize("mem,itr", [(5, 8), (1, 6), (12, 16)])
+def test_batched_run_matches_single_runs(gpu, mem, itr):
+    """The batched L-BFGS (compact form of the two-loop recursion, dot
+    products in groups of 16 per launch) against the single-problem two-loop
+    runs: memory 1, the default 5, and 12 with the ring wrapped (2m = 24 and
+    3m = 36 dot products per pass)."""
     probs, k = small_batch(), kernels()
     B = gpu.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"], k["knl_order"],
                   k["time_stp"], k["time_num"], chunk=256)
-    alps, infos, rounds = B.run(8, 0.0, 0.0, 0.5, 5)
-    print("batched run: %d rounds, iterations %s" % (rounds, [i["iterations"] for i in infos]), flush=True)
+    alps, infos, rounds = B.run(itr, 0.0, 0.0, 0.5, mem)
+    print("batched run, memory %d: %d rounds, iterations %s" % (mem, rounds, [i["iterations"] for i in infos]),
+          flush=True)
     for i, p in enumerate(probs):
         obj, ifo = single(gpu, p, k)
         a1, i1 = gpu.run(p["q0"], None, p["elm"], ifo, p["tgt_cen"], p["tgt_vol"], p["tgt_dir"], k["cen_type"],
                          k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"], k["knl_order"], p["knl_width"],
-                         k["time_stp"], k["time_num"], 8, 0.0, 0.0, 0.5, 5)
+                         k["time_stp"], k["time_num"], itr, 0.0, 0.0, 0.5, mem)
         assert infos[i]["status"] == i1["status"] and infos[i]["iterations"] == i1["iterations"]
         assert abs(infos[i]["fcn"] - i1["fcn"]) <= 1e-8 * abs(i1["fcn"]), (i, infos[i]["fcn"], i1["fcn"])
         fb, _ = obj(alps[i])
