@@ -904,8 +904,8 @@ void Batch::build_tables(const std::vector<int> &prec_of)
         const int g = pr == kFP32 ? 1 : 0;
         const BatchDesc &d = desc_h_[p];
         probs.push_back(p);
-        // FP32 pair kernels keep kKvR32 / kBwdR32 targets per lane
-        const int tkv = g ? 64 * kKvR32 : 64, tbwd = g ? 64 * kBwdR32 : 64;
+        // FP32 pair kernels keep kKvR32 / bwd_r32(order) targets per lane
+        const int tkv = g ? 64 * kKvR32 : 64, tbwd = g ? 64 * bwd_r32(spec_.order) : 64;
         for (int s = 0; s < d.splits; ++s) {
             for (int t = 0; t < ceil_div(d.n, tkv); ++t) kv[g].push_back(make_int4(p, t, s, 0));
             for (int t = 0; t < ceil_div(d.n, tbwd); ++t) bwd[g].push_back(make_int4(p, t, s, 0));
@@ -1100,7 +1100,7 @@ void Batch::launch_eval(const double *alp, double *grd, double *f_dev)
             each_prec([&](int g, auto Pc) {
                 if (!tb.nbwd[g]) return;
                 dispatch([&](auto Dc, auto Oc) {
-                    constexpr int R = decltype(Pc)::fp32 ? kBwdR32 : 1;
+                    constexpr int R = decltype(Pc)::fp32 ? bwd_r32(Oc()) : 1;
                     kb_bwd<Dc(), Oc(), decltype(Pc), R><<<tb.nbwd[g], kBlock, 0, st_>>>(v, t, tb.bwd[g].p);
                 });
                 LDDMM_LAUNCH_CHECK();

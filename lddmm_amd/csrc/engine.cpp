@@ -158,9 +158,9 @@ Objective::Objective(const DeviceProblem &p, hipStream_t st, int target_blocks) 
     target_blocks_ = tb;
     const int D = p_.dim;
     sp_kv_ = plan_kv(p_.n, p_.n, p_.prec, tb);
-    sp_bwd_ = plan_bwd(p_.n, p_.n, p_.prec, tb);
+    sp_bwd_ = plan_bwd(p_.n, p_.n, p_.prec, p_.order, tb);
     const int other = p_.prec == kFP32 ? kFP64 : kFP32;
-    const PairSplit kv_other = plan_kv(p_.n, p_.n, other, tb), bwd_other = plan_bwd(p_.n, p_.n, other, tb);
+    const PairSplit kv_other = plan_kv(p_.n, p_.n, other, tb), bwd_other = plan_bwd(p_.n, p_.n, other, p_.order, tb);
     // the varifold sums (one launch each for the deformed and the target
     // sources) take twice the point kernels' workgroup count: their finer
     // source chunks also cull more of the far pairs in FP32
@@ -360,7 +360,7 @@ void Objective::set_precision(int prec)
     p_.prec = prec;
     // the partial-sum buffer was sized for the plans of both policies
     sp_kv_ = plan_kv(p_.n, p_.n, prec, target_blocks_);
-    sp_bwd_ = plan_bwd(p_.n, p_.n, prec, target_blocks_);
+    sp_bwd_ = plan_bwd(p_.n, p_.n, prec, p_.order, target_blocks_);
 }
 
 void Objective::compute_target_target(int target_blocks)

@@ -23,7 +23,7 @@ bool pair_expansion();
 // fused backward pass (two targets per lane in FP32), plan_single for the dq
 // kernels (one target per lane).
 PairSplit plan_kv(int ni, int nj, int prec, int target_blocks);
-PairSplit plan_bwd(int ni, int nj, int prec, int target_blocks);
+PairSplit plan_bwd(int ni, int nj, int prec, int order, int target_blocks);
 PairSplit plan_single(int ni, int nj, int target_blocks);
 // part[(s*D + c)*ni + i] = sum_{j in chunk s} k(qi_i, qj_j) a_j[c]
 void launch_kv(int dim, int order, int prec, const double *qi, int ni, const double *qj, const double *a, int nj,

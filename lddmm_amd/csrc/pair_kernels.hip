@@ -149,16 +149,16 @@ void dispatch_all(int dim, int order, int prec, F &&f)
 }  // namespace
 
 int kv_targets_per_block(int prec) { return prec == kFP32 ? 64 * kKvR32 : 64; }
-int bwd_targets_per_block(int prec) { return prec == kFP32 ? 64 * kBwdR32 : 64; }
+int bwd_targets_per_block(int prec, int order) { return prec == kFP32 ? 64 * bwd_r32(order) : 64; }
 
 PairSplit plan_kv(int ni, int nj, int prec, int target_blocks)
 {
     return make_split_ti(ni, nj, kv_targets_per_block(prec), target_blocks);
 }
 
-PairSplit plan_bwd(int ni, int nj, int prec, int target_blocks)
+PairSplit plan_bwd(int ni, int nj, int prec, int order, int target_blocks)
 {
-    return make_split_ti(ni, nj, bwd_targets_per_block(prec), target_blocks);
+    return make_split_ti(ni, nj, bwd_targets_per_block(prec, order), target_blocks);
 }
 
 PairSplit plan_single(int ni, int nj, int target_blocks) { return make_split_ti(ni, nj, 64, target_blocks); }
@@ -219,7 +219,7 @@ void launch_bwd_pairs(int dim, int order, int prec, const double *q, const doubl
                       double width, const PairSplit &sp, double *part, hipStream_t st)
 {
     dispatch_all(dim, order, prec, [&](auto Dc, auto Oc, auto Pc) {
-        constexpr int R = decltype(Pc)::fp32 ? kBwdR32 : 1;
+        constexpr int R = decltype(Pc)::fp32 ? bwd_r32(Oc()) : 1;
         using Op = OpBwd<Dc(), Oc(), decltype(Pc), R>;
         using real = typename Op::real;
         const double iw = 1.0 / width, iw2 = 1.0 / (width * width), kap = Op::KF::kappa(iw, iw2);

@@ -12,8 +12,13 @@ namespace ops {
 
 // targets per lane of the FP32 point-kernel launches (FP64: one); the split
 // plans and the batched work items use 64 times these per block.  Four for
-// K.a measured slower than two (larger tiles, fewer resident waves).
-constexpr int kKvR32 = 2, kBwdR32 = 2;
+// K.a measured slower than two (larger tiles, fewer resident waves).  The
+// fused adjoint takes two with the Gaussian kernel and one with the Matern
+// kernels, whose longer pair term needs the registers (measured on the
+// sphere->cube Matern adjoint: one 5% faster than two; on the Gaussian
+// batches two 9-37% faster than one).
+constexpr int kKvR32 = 2;
+constexpr int bwd_r32(int order) { return order == -1 ? 2 : 1; }
 
 // ---------------------------------------------------------------------------
 // v_i = sum_j k(qi_i, qj_j) a_j   (forward flow, multiplyKernel)
