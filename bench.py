@@ -127,6 +127,62 @@ def cpu_baseline(w, ifo, pairs_per_eval):
                 seconds_per_eval=t_eval)
 
 
+def warm_start_leg(probs, targets, e_ref, B, B64, chunk, t_setup, itr, seed, nmin, nmax, train_count):
+    """The learned warm start on a batch whose plain energies are known: a
+    one-off training on `train_count` problems of the same generator (other
+    seeds) solved by the batched FP32 run, then the timed leg: feature
+    extraction, inference, the per-problem start scale selection, the
+    registration to the plain energies (at most 1.25 times the plain run's
+    iterations), and the fallback, a zero-momentum batched FP32 run for
+    every problem the warm-started run did not bring to its target.  B and
+    B64 are the batch in FP32 and FP64 (the latter re-evaluates results)."""
+    from lddmm_amd import warmstart as W
+    k = G.BATCH_KERNELS
+    T = k["time_num"]
+
+    def make(ps, chunk_):
+        return lddmm_amd.Batch(ps, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
+                               k["knl_order"], k["time_stp"], T, precision="fp32", chunk=chunk_)
+
+    t0 = time.perf_counter()
+    train = G.batch_problems(train_count, seed + 1000, nmin, nmax)
+    Bt = make(train, 0)
+    sol, _, _ = Bt.run(itr, 0.0, 0.0, 0.5, 5, precision="fp32", rel_energy_tol=1e-6)
+    Bt.close()
+    t_solve = time.perf_counter() - t0
+    ws = W.WarmStart(hidden=128, layers=4, seed=1)
+    t0 = time.perf_counter()
+    losses = ws.fit(train, sol, k["knl_order"], k["time_stp"], T, steps=3000, batch_rows=4096, lr=1e-3)
+    t_fit = time.perf_counter() - t0
+    # timed: inference, start selection, registration, fallback
+    t0 = time.perf_counter()
+    warm = [ws.predict(p, k["knl_order"], k["time_stp"], T) for p in probs]
+    starts, scale = W.choose_start(B, warm)
+    # the warm-started run gets 1.25 times the plain run's iteration budget;
+    # a problem that has not reached its target by then falls back
+    aw, iw, rw = B.run(int(1.25 * itr), 0.0, 0.0, 0.5, 5, alp0=starts, precision="fp32", targets=targets)
+    missed = [i for i, r in enumerate(iw) if not r["reached_target"]]
+    its_fb = []
+    if missed:
+        Bf = make([probs[i] for i in missed], chunk)
+        af, ifb, _ = Bf.run(3 * itr, 0.0, 0.0, 0.5, 5, precision="fp32", targets=targets[missed])
+        Bf.close()
+        for j, i in enumerate(missed):
+            aw[i] = af[j]
+        its_fb = [r["iterations"] for r in ifb]
+    t_warm = time.perf_counter() - t0 + t_setup
+    f, _ = B64.objgrd(aw)
+    rel_w = (f - e_ref) / np.abs(e_ref)
+    count = len(probs)
+    return dict(seconds=t_warm, registrations_per_s=count / t_warm, rounds=rw,
+                median_iterations=float(np.median([r["iterations"] for r in iw])),
+                warm_start_used=int(np.sum(scale > 0)), reached_target=count - len(missed),
+                fallback_problems=len(missed), fallback_iterations=its_fb,
+                within_1e6=int(np.sum(rel_w <= 1e-6 + 1e-8)), max_rel_energy_vs_plain=float(rel_w.max()),
+                training=dict(problems=train_count, solve_seconds=t_solve, fit_seconds=t_fit, adam_steps=3000,
+                              loss_first=float(losses[0]), loss_last=float(np.mean(losses[-20:]))))
+
+
 def batch_bench(count, nmin, nmax, itr, seed=4, train_count=64):
     """Uneven batch (the batched-registration workload, scaled by `count`
     and the size range): plain FP64 registrations one after another
@@ -135,16 +191,8 @@ def batch_bench(count, nmin, nmax, itr, seed=4, train_count=64):
     policy run to each plain run's final energy within 1e-6 relative, and
     the batched FP32 policy started from the learned warm start.  Batched
     times include the batch's set-up, as plain times include each run's.
-
-    Warm-start protocol: the network is trained on `train_count` problems
-    of the same generator (other seeds) solved by the batched FP32 run; that
-    training is reported as a one-off cost.  The timed warm-start leg
-    includes feature extraction, inference, the per-problem start scale
-    selection, the registration to the plain energies (at most 1.25 times
-    the plain run's iterations), and the fallback: a zero-momentum batched
-    FP32 run for every problem the warm-started run did not bring within
-    1e-6 of the plain energy."""
-    from lddmm_amd import warmstart as W
+    The warm-start protocol is warm_start_leg's (training reported as a
+    one-off cost)."""
     probs = G.batch_problems(count, seed, nmin, nmax)
     k = G.BATCH_KERNELS
     T = k["time_num"]
@@ -182,36 +230,9 @@ def batch_bench(count, nmin, nmax, itr, seed=4, train_count=64):
     rel = rel_to_plain(a32)
     t64, t32 = i64[0]["seconds"] + t_setup, i32[0]["seconds"] + t_setup
 
-    # learned warm start: one-off training on other seeds of the generator
-    t0 = time.perf_counter()
-    train = G.batch_problems(train_count, seed + 1000, nmin, nmax)
-    Bt = make(train, "fp32")
-    sol, _, _ = Bt.run(itr, 0.0, 0.0, 0.5, 5, precision="fp32", rel_energy_tol=1e-6)
-    Bt.close()
-    t_solve = time.perf_counter() - t0
-    ws = W.WarmStart(hidden=128, layers=4, seed=1)
-    t0 = time.perf_counter()
-    losses = ws.fit(train, sol, k["knl_order"], k["time_stp"], T, steps=3000, batch_rows=4096, lr=1e-3)
-    t_fit = time.perf_counter() - t0
-    # timed: inference, start selection, registration, fallback
-    t0 = time.perf_counter()
-    warm = [ws.predict(p, k["knl_order"], k["time_stp"], T) for p in probs]
-    starts, scale = W.choose_start(B, warm)
-    # the warm-started run gets 1.25 times the plain run's iteration budget;
-    # a problem that has not reached its target by then falls back
-    aw, iw, rw = B.run(int(1.25 * itr), 0.0, 0.0, 0.5, 5, alp0=starts, precision="fp32", targets=targets)
-    missed = [i for i, r in enumerate(iw) if not r["reached_target"]]
-    its_fb = []
-    if missed:
-        Bf = make([probs[i] for i in missed], "fp32", chunk)
-        af, ifb, _ = Bf.run(3 * itr, 0.0, 0.0, 0.5, 5, precision="fp32", targets=targets[missed])
-        Bf.close()
-        for j, i in enumerate(missed):
-            aw[i] = af[j]
-        its_fb = [r["iterations"] for r in ifb]
-    t_warm = time.perf_counter() - t0 + t_setup
-    rel_w = rel_to_plain(aw)
-    its_w = [r["iterations"] for r in iw]
+    wleg = warm_start_leg(probs, targets, e_ref, B, B64, chunk, t_setup, itr, seed, nmin, nmax, train_count)
+    wleg["median_iterations_plain"] = float(np.median([i["iterations"] for i in seq]))
+    t_warm = wleg["seconds"]
     B.close()
     B64.close()
     return dict(problems=count, n_range=[int(min(B.n)), int(max(B.n))], chunk=chunk,
@@ -225,15 +246,7 @@ def batch_bench(count, nmin, nmax, itr, seed=4, train_count=64):
                                   reached_target=int(sum(i["reached_target"] for i in i32)),
                                   within_1e6=int(np.sum(rel <= 1e-6 + 1e-8)),
                                   max_rel_energy_vs_plain=float(rel.max())),
-                batched_fp32_warm_start=dict(
-                    seconds=t_warm, registrations_per_s=count / t_warm, rounds=rw,
-                    median_iterations=float(np.median(its_w)),
-                    median_iterations_plain=float(np.median([i["iterations"] for i in seq])),
-                    warm_start_used=int(np.sum(scale > 0)), reached_target=count - len(missed),
-                    fallback_problems=len(missed), fallback_iterations=its_fb,
-                    within_1e6=int(np.sum(rel_w <= 1e-6 + 1e-8)), max_rel_energy_vs_plain=float(rel_w.max()),
-                    training=dict(problems=train_count, solve_seconds=t_solve, fit_seconds=t_fit, adam_steps=3000,
-                                  loss_first=float(losses[0]), loss_last=float(np.mean(losses[-20:])))),
+                batched_fp32_warm_start=wleg,
                 speedup_batched_fp32_vs_plain_fp64=t_seq / t32,
                 speedup_batched_fp32_warm_vs_plain_fp64=t_seq / t_warm)
 
@@ -334,14 +347,17 @@ def precision_report(freq=100, seed=31, itr=10):
                           final_shape_rms_dist=float(np.sqrt(np.mean(np.sum((qr32 - qr64) ** 2, axis=1))))))
 
 
-def c4_report(count=256, nmin=500, nmax=20000, itr=300, seed=4):
+def c4_report(count=256, nmin=500, nmax=20000, itr=300, seed=4, warm=False, train_count=64):
     """The batched-registration configuration at full size: `count` problems,
     vertex counts log-uniform in [nmin, nmax].  The plain FP64 method
     registers them one after another (zero start, stop at a relative energy
     decrease below 1e-6 or `itr` iterations) and sets each problem's energy;
     the batched FP32 policy then registers all of them in one run until each
     reaches its plain energy x (1 + 1e-6) (at most 3 x `itr` iterations,
-    finishing in FP64 on a stall), and its results are re-evaluated in FP64."""
+    finishing in FP64 on a stall), and its results are re-evaluated in FP64.
+    With `warm`, the same batch is also registered from the learned warm
+    start (warm_start_leg, trained on `train_count` problems of the
+    generator's other seeds)."""
     probs = G.batch_problems(count, seed, nmin, nmax)
     k = G.BATCH_KERNELS
     seq, a_plain = [], []
@@ -366,12 +382,17 @@ def c4_report(count=256, nmin=500, nmax=20000, itr=300, seed=4):
     t_setup = time.perf_counter() - t0
     a32, i32, r32 = B.run(3 * itr, 0.0, 0.0, 0.5, 5, precision="fp32", targets=e_ref * (1.0 + 1e-6), progress=25)
     t32 = i32[0]["seconds"] + t_setup
-    B.close()
     B64 = lddmm_amd.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
                           k["knl_order"], k["time_stp"], k["time_num"], precision="fp64", chunk=chunk)
     f64, _ = B64.objgrd(a32)
     # consistency: the plain runs' energies re-evaluated by the batched FP64 objective
     f_plain, _ = B64.objgrd(a_plain)
+    wleg = None
+    if warm:
+        wleg = warm_start_leg(probs, e_ref * (1.0 + 1e-6), e_ref, B, B64, chunk, t_setup, itr, seed, nmin, nmax,
+                              train_count)
+        wleg["speedup_vs_plain_fp64"] = t_seq / wleg["seconds"]
+    B.close()
     B64.close()
     rel = (f64 - e_ref) / np.abs(e_ref)
     missed = [dict(problem=i, n=int(B.n[i]), knl_width=float(probs[i]["knl_width"]), plain_energy=float(e_ref[i]),
@@ -390,7 +411,8 @@ def c4_report(count=256, nmin=500, nmax=20000, itr=300, seed=4):
                                   within_1e6=int(np.sum(rel <= 1e-6 + 1e-8)),
                                   max_rel_energy_vs_plain=float(rel.max()), missed=missed),
                 plain_energy_check=dict(max_rel_diff_batched_fp64=float(np.max(np.abs(f_plain - e_ref) / np.abs(e_ref)))),
-                speedup_batched_fp32_vs_plain_fp64=t_seq / t32)
+                speedup_batched_fp32_vs_plain_fp64=t_seq / t32,
+                **({} if wleg is None else dict(batched_fp32_warm_start=wleg)))
 
 
 def main():
@@ -407,6 +429,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision-report", action="store_true",
                     help="only the FP32-vs-FP64 report on the 100k-vertex superquadric pair")
+    ap.add_argument("--warm-start", action="store_true",
+                    help="with --c4-full: also register the batch from the learned warm start")
     ap.add_argument("--c4-full", action="store_true",
                     help="only the full-size batched configuration (256 problems, N up to 20,000)")
     ap.add_argument("--batch-profile", action="store_true",
@@ -423,7 +447,7 @@ def main():
         print(json.dumps(precision_report()), flush=True)
         return
     if args.c4_full:
-        print(json.dumps(c4_report()), flush=True)
+        print(json.dumps(c4_report(warm=args.warm_start)), flush=True)
         return
     if args.culling_report:
         print(json.dumps(culling_report()), flush=True)
