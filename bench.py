@@ -426,6 +426,7 @@ def main():
     ap.add_argument("--batch", type=int, default=32, help="problems of the uneven-batch leg (0 = skip)")
     ap.add_argument("--batch-nmin", type=int, default=500)
     ap.add_argument("--batch-nmax", type=int, default=5000)
+    ap.add_argument("--batch-chunk", type=int, default=512, help="source chunk of --batch-profile")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision-report", action="store_true",
                     help="only the FP32-vs-FP64 report on the 100k-vertex superquadric pair")
@@ -455,11 +456,11 @@ def main():
     if args.batch_profile:
         probs = G.batch_problems(args.batch, 4, args.batch_nmin, args.batch_nmax)
         k = G.BATCH_KERNELS
-        # a fixed source chunk (the autotuner's choice for this batch when
-        # nothing is profiling): under PMC counters its timing launches can
-        # pick differently from pass to pass
+        # a fixed source chunk (default 512, the autotuner's choice for the
+        # 32-problem batch when nothing is profiling): under PMC counters its
+        # timing launches can pick differently from pass to pass
         B = lddmm_amd.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
-                            k["knl_order"], k["time_stp"], k["time_num"], precision="fp32", chunk=512)
+                            k["knl_order"], k["time_stp"], k["time_num"], precision="fp32", chunk=args.batch_chunk)
         chunk, pairs = B.info()
         # direct launches (as whenever rocprofv3 drives the process): its kernel
         # tracing has crashed the host process during hipGraph capture here;
