@@ -276,3 +276,31 @@ def test_expanded_form_2d(gpu):
             assert abs(f - f64) <= 1e-6 * abs(f64) and rel_err(g, g64) <= 1e-4
     o32.close()
     o64.close()
+
+
+@pytest.mark.parametrize("dir_type", ["L", "O", "U"])
+def test_expanded_varifold_orientation_kernels(gpu, dir_type):
+    """The expanded form of the varifold's 'G' position kernel with the
+    non-Binet orientation kernels (its second pair-term branch), on a shape
+    whose element tiles take it, against the direct form and FP64."""
+    print("expanded varifold, G x %s" % dir_type, flush=True)
+    p = G.batch_problems(1, seed=43, nmin=6000, nmax=7000)[0]
+    k = dict(K, dir_type=dir_type, dir_width=0.0 if dir_type == "L" else 0.6, time_num=4, time_stp=1.0 / 3.0)
+    n, T = len(p["q0"]), k["time_num"]
+    ifo = oracle.get_elm_ifo(p["elm"], n)
+    args = (p["q0"], p["elm"], ifo, p["tgt_cen"], p["tgt_vol"], p["tgt_dir"], k["cen_type"], k["cen_width"],
+            k["dir_type"], k["dir_width"], k["dis_wgt"], k["knl_order"], p["knl_width"], k["time_stp"], T)
+    o32, o64 = gpu.Objective(*args, precision="fp32"), gpu.Objective(*args)
+    alp = 0.05 * G.normal(11, (T - 1) * n * 3).reshape(T - 1, n, 3)
+    f64, g64 = o64(alp)
+    try:
+        gpu.set_pair_expansion(False)
+        f0, g0 = o32(alp)
+    finally:
+        gpu.set_pair_expansion(True)
+    f1, g1 = o32(alp)
+    assert not np.array_equal(g0, g1)
+    for f, g in ((f0, g0), (f1, g1)):
+        assert abs(f - f64) <= 1e-6 * abs(f64) and rel_err(g, g64) <= 1e-4
+    o32.close()
+    o64.close()
