@@ -127,6 +127,9 @@ def test_argument_errors_are_reported(gpu):
         lambda: gpu.Objective(q, tri, ifo, tc, ta, tu, "G", 0.1, "O", 0.0, 1.0, -1, 0.3, 0.5, 3),
         lambda: gpu.Batch([dict(q0=q, elm=bad, tgt_cen=tc, tgt_dir=tu, tgt_vol=ta, knl_width=0.3)], "G", 0.1, "B",
                           0.0, 1.0, -1, 0.5, 3),
+        # several problems: the set-up runs on host threads, the error comes from one of them
+        lambda: gpu.Batch([dict(q0=q, elm=bad if i == 3 else tri, tgt_cen=tc, tgt_dir=tu, tgt_vol=ta, knl_width=0.3)
+                           for i in range(6)], "G", 0.1, "B", 0.0, 1.0, -1, 0.5, 3),
         lambda: gpu.l2Dist(q, q[:3], 1.0),
     ]
     for c in cases:
