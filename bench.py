@@ -79,30 +79,44 @@ def make_objective(w):
 
 
 def cpu_baseline(w, ifo, pairs_per_eval):
-    """FP64 C restatement of the reference CPU path (oracle/), one core,
-    on a bounded sample: one forward K.alpha product and one adjoint step
-    (K.amp and dq) at the full N, and the varifold with gradient of the
-    full deformed surface against a 1/16 slice of the target; the per-pair
-    cost of each part is scaled to one full evaluation (the reference's
-    own evaluation does 3 (T-1) N^2 + Ed^2 + Ed Et + Et^2 pair evaluations;
-    the rate is quoted in the new path's pair count so the two are
-    comparable per evaluation)."""
+    """The reference's CPU path, one core, on a bounded sample: one forward
+    K.alpha product and one adjoint step (K.amp and dq) at the full N, and
+    the varifold with gradient of the full deformed surface against a 1/16
+    slice of the target; the per-pair cost of each part is scaled to one
+    full evaluation (the reference's own evaluation does 3 (T-1) N^2 +
+    Ed^2 + Ed Et + Et^2 pair evaluations; the rate is quoted in the new
+    path's pair count so the two are comparable per evaluation).
+
+    kind "reference": the reference's own Cpp/src multiplyKernel, dqKernel
+    and varifold, compiled from its sources into oracle/_ref (its objgrd
+    driver needs MATLAB's BLAS header and is not built); kind "port" where
+    oracle/_ref was not built: the same operations in the FP64 C
+    restatement (oracle/)."""
     import oracle
     q = w["q0"]
     n, d = q.shape
     T1 = w["time_num"] - 1
     a = 0.01 * G.normal(7, n * d).reshape(n, d)
+    L = oracle.ref(d)
+    v, g = np.empty_like(q), np.empty_like(q)
     t0 = time.perf_counter()
-    v = oracle.multiply_kernel(q, a, w["knl_order"], w["knl_width"])
+    if L is not None:
+        L.ref_multiply_kernel(v, q, a, w["knl_order"], w["knl_width"], n, d)
+    else:
+        v = oracle.multiply_kernel(q, a, w["knl_order"], w["knl_width"])
     t_mult = time.perf_counter() - t0
     t0 = time.perf_counter()
-    oracle.dq_kernel(q, a, v, w["knl_order"], w["knl_width"])
+    if L is not None:
+        L.ref_dq_kernel(g, q, a, v, w["knl_order"], w["knl_width"], n, d)
+    else:
+        oracle.dq_kernel(q, a, v, w["knl_order"], w["knl_width"])
     t_dq = time.perf_counter() - t0
     et = len(w["tgt_vol"])
     sl = slice(0, max(1, et // 16))
     t0 = time.perf_counter()
     oracle.varifold(q, w["elm"], ifo, w["tgt_cen"][sl], w["tgt_dir"][sl], w["tgt_vol"][sl], w["cen_type"],
-                    w["cen_width"], w["dir_type"], w["dir_width"])
+                    w["cen_width"], w["dir_type"], w["dir_width"],
+                    **(dict(L=L, prefix="ref") if L is not None else {}))
     t_vf = time.perf_counter() - t0
     ed = w["elm"].shape[0]
     ets = sl.stop
@@ -119,11 +133,14 @@ def cpu_baseline(w, ifo, pairs_per_eval):
                     break
     except OSError:
         pass
-    return dict(value=pairs_per_eval / t_eval, unit="pair-interactions/s", cores=1, kind="port",
+    what = ("the reference's Cpp/src multiplyKernel, dqKernel and varifold (oracle/_ref)" if L is not None
+            else "oracle/lddmm_oracle.c")
+    return dict(value=pairs_per_eval / t_eval, unit="pair-interactions/s", cores=1,
+                kind="reference" if L is not None else "port",
                 host=dict(cpu=model, logical_cpus=os.cpu_count()),
-                sample=("oracle/lddmm_oracle.c, 1 core: K.alpha + dq at N=%d and the varifold with gradient vs "
-                        "%d of %d target triangles (%.1f s of CPU work), scaled to one full evaluation "
-                        "= %.1f s" % (n, ets, et, sample_s, t_eval)),
+                sample=("%s, 1 core: K.alpha + dq at N=%d and the varifold with gradient vs %d of %d target "
+                        "triangles (%.1f s of CPU work), scaled to one full evaluation = %.1f s"
+                        % (what, n, ets, et, sample_s, t_eval)),
                 seconds_per_eval=t_eval)
 
 
