@@ -52,12 +52,12 @@ TRANS_PER_PAIR = {("fwd_pairs", -1): 1, ("fwd_pairs", 1): 2, ("bwd_pairs", -1): 
 # copy kernel of known size), valu_busy = AMD's VALUBusy, valu_issue = share of SIMD
 # issue slots at 4 cycles per wave64 VALU instruction.
 ROCPROF_SPHERE_CUBE = {
-    ("fwd_pairs", "f64"): dict(traffic=8.064e6, valu_busy=0.85, valu_issue=0.82),
-    ("bwd_pairs", "f64"): dict(traffic=1.726e7, valu_busy=0.85, valu_issue=0.83),
-    ("vf_pairs", "f64"): dict(traffic=2.668e7, valu_busy=0.85, valu_issue=0.88),
-    ("fwd_pairs", "f32"): dict(traffic=1.239e7, valu_busy=0.87, valu_issue=0.81),
-    ("bwd_pairs", "f32"): dict(traffic=1.725e7, valu_busy=1.01, valu_issue=0.97),
-    ("vf_pairs", "f32"): dict(traffic=2.670e7, valu_busy=0.70, valu_issue=0.75),
+    ("fwd_pairs", "f64"): dict(traffic=8.064e6, valu_busy=0.83, valu_issue=0.80),
+    ("bwd_pairs", "f64"): dict(traffic=1.726e7, valu_busy=0.83, valu_issue=0.81),
+    ("vf_pairs", "f64"): dict(traffic=2.668e7, valu_busy=0.84, valu_issue=0.87),
+    ("fwd_pairs", "f32"): dict(traffic=1.239e7, valu_busy=0.89, valu_issue=0.81),
+    ("bwd_pairs", "f32"): dict(traffic=1.725e7, valu_busy=1.02, valu_issue=0.97),
+    ("vf_pairs", "f32"): dict(traffic=2.670e7, valu_busy=0.71, valu_issue=0.75),
 }
 
 

@@ -58,6 +58,31 @@ constexpr int kDotBlocks = 512;   // fixed grid of the deterministic dot reducti
 // grad_{q_i} k = c (q_i - q_j) (Cpp/src/computeKernel.cpp:36-37,170-234;
 // Cpp/src/dqKernel.cpp:57,215-291).  Order 0 has no gradient in the
 // reference (Cpp/src/dqKernel.cpp:39, no case 0), so c = 0 there.
+// sqrt(r2) for the Matern distance (r2 >= 0): Goldschmidt's square-root
+// iteration (R. E. Goldschmidt, MIT thesis 1964; P. Markstein, "IA-64 and
+// Elementary Functions", 2000, ch. 9) on the hardware reciprocal square
+// root v_rsq_f64: g = a y, h = y/2 converge to sqrt(a) and 1/(2 sqrt(a)),
+// and two residual corrections d = a - g^2, g += d h give the FP64 root.
+// The library sqrt adds the scaling of inputs below 2^-767 and the class
+// tests of 0 and inf (9 more instructions per pair); here r2 is clamped to
+// 1e-300 instead, so a point with itself (r2 = 0) gets a distance of
+// 1e-150 / sigma, which rounds away in every Matern factor (1 + d, e^-d):
+// the kernel values and gradient factors are those of d = 0.  Above 2^-767
+// the sequence is the library's refinement, so the distance is the same.
+__device__ __forceinline__ double sqrt_r2(double a)
+{
+    a = fmax(a, 1e-300);
+    const double y = __builtin_amdgcn_rsq(a);
+    double g = a * y, h = 0.5 * y;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, a);
+    g = fma(d, h, g);
+    d = fma(-g, g, a);
+    return fma(d, h, g);
+}
+
 template <int ORDER>
 struct KernelFn {
     // inv_w = 1/sigma, inv_w2 = 1/sigma^2
@@ -67,7 +92,7 @@ struct KernelFn {
             k = exp(-r2 * inv_w2);
             c = -2.0 * k * inv_w2;
         } else {
-            double d = sqrt(r2) * inv_w;
+            double d = sqrt_r2(r2) * inv_w;
             double e = exp(-d);
             if constexpr (ORDER == 0) {
                 k = e;
@@ -92,7 +117,7 @@ struct KernelFn {
         if constexpr (ORDER == -1) {
             return exp(-r2 * inv_w2);
         } else {
-            double d = sqrt(r2) * inv_w;
+            double d = sqrt_r2(r2) * inv_w;
             double e = exp(-d);
             if constexpr (ORDER == 0) return e;
             else if constexpr (ORDER == 1) return (1.0 + d) * e;
@@ -180,7 +205,7 @@ template <int ORDER>
 struct KernelFn2D {
     __device__ __forceinline__ static void eval(double r2, double inv_w, double inv_w2, double &k, double &c)
     {
-        const double d = sqrt(r2) * inv_w, d2 = r2 * inv_w2;
+        const double d = sqrt_r2(r2) * inv_w, d2 = r2 * inv_w2;
         double p0, p1;
         bessel_p01(d, p0, p1);
         if constexpr (ORDER == 0) {
