@@ -66,6 +66,32 @@ def test_dq_kernel_cross(gpu, dim, order):
     assert rel_err(gi, ei) <= TOL_OP and rel_err(gj, ej) <= TOL_OP
 
 
+def coincident_points(dim):
+    """Points with exact repeats (r2 = 0 between distinct indices) and with
+    partners 1e-120 and 1e-160 away (r2 below 2^-767 and subnormal): the
+    range where the FP64 Matern distance clamps r2 instead of scaling it."""
+    q = pts(21, 40, dim)
+    near = q[:10].copy()
+    near[:5, 0] += 1e-120
+    near[5:, 0] += 1e-160
+    return np.concatenate([q, q[:12], near])
+
+
+@pytest.mark.parametrize("dim,order", CASES)
+def test_coincident_points(gpu, dim, order):
+    q = coincident_points(dim)
+    n = len(q)
+    a, l, r = pts(22, n, dim, 1.0), pts(23, n, dim, 1.0), pts(24, n, dim, 1.0)
+    print("coincident points dim=%d order=%d" % (dim, order), flush=True)
+    assert rel_err(gpu.multiplyKernel(q, a, order, 0.3), oracle.multiply_kernel(q, a, order, 0.3)) <= TOL_OP
+    assert rel_err(gpu.computeKernel(q, order, 0.3), oracle.compute_kernel(q, order, 0.3)) <= TOL_OP
+    got = gpu.dqKernel(q, l, r, order, 0.3)
+    if order == 0:
+        assert not np.any(got)
+    else:
+        assert rel_err(got, oracle.dq_kernel(q, l, r, order, 0.3)) <= TOL_OP
+
+
 @pytest.mark.parametrize("dim,order", [(2, -1), (2, 0), (2, 3), (3, -1), (3, 2)])
 def test_compute_kernel(gpu, dim, order):
     q, qj = pts(13, 90, dim), pts(14, 41, dim)
