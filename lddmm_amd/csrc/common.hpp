@@ -64,14 +64,15 @@ constexpr int kDotBlocks = 512;   // fixed grid of the deterministic dot reducti
 // root v_rsq_f64: g = a y, h = y/2 converge to sqrt(a) and 1/(2 sqrt(a)),
 // and two residual corrections d = a - g^2, g += d h give the FP64 root.
 // The library sqrt adds the scaling of inputs below 2^-767 and the class
-// tests of 0 and inf (9 more instructions per pair); here r2 is clamped to
-// 1e-300 instead, so a point with itself (r2 = 0) gets a distance of
+// tests of 0 and inf (9 more instructions per pair); here 1e-300 is added
+// to r2 instead, so a point with itself (r2 = 0) gets a distance of
 // 1e-150 / sigma, which rounds away in every Matern factor (1 + d, e^-d):
-// the kernel values and gradient factors are those of d = 0.  Above 2^-767
-// the sequence is the library's refinement, so the distance is the same.
+// the kernel values and gradient factors are those of d = 0.  From 2^-767
+// up the addition leaves r2 unchanged and the sequence is the library's
+// refinement, so the distance is the same; a NaN r2 stays NaN.
 __device__ __forceinline__ double sqrt_r2(double a)
 {
-    a = fmax(a, 1e-300);
+    a += 1e-300;
     const double y = __builtin_amdgcn_rsq(a);
     double g = a * y, h = 0.5 * y;
     const double r = fma(-h, g, 0.5);
