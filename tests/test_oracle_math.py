@@ -267,3 +267,108 @@ def test_l2_objective_zero_momentum_closed_form():
     expect = h * oracle.multiply_kernel(q, 7.0 * (q - t), -1, 0.45)
     for s in g.reshape(P.T - 1, P.n, P.dim):
         assert rel_err(s, expect) <= 1e-13
+
+
+def _spd(seed, n, lo, hi):
+    """A symmetric matrix with eigenvalues spread over (lo, hi)."""
+    Q, _ = np.linalg.qr(G.normal(seed, n * n).reshape(n, n))
+    return Q @ np.diag(np.linspace(lo, hi, n)) @ Q.T
+
+
+def _dense_bfgs_iterates(A, x0, mem, iters):
+    """L-BFGS iterates on f = x^T A x / 2 with every unit step accepted,
+    built from the dense inverse-Hessian update: H0 = gamma I with gamma =
+    s^T y / y^T y of the newest pair (1 before any pair), then the BFGS
+    update H <- V^T H V + rho s s^T applied for the newest min(k-1, mem)
+    pairs, oldest first (Nocedal & Wright eq. 7.19)."""
+    xs, pairs, x = [x0.copy()], [], x0.copy()
+    for _ in range(iters):
+        g = A @ x
+        use = pairs[-mem:] if pairs else []
+        gamma = 1.0 if not use else float(use[-1][0] @ use[-1][1] / (use[-1][1] @ use[-1][1]))
+        H = gamma * np.eye(x.size)
+        for s, y in use:
+            rho = 1.0 / float(y @ s)
+            V = np.eye(x.size) - rho * np.outer(y, s)
+            H = V.T @ H @ V + rho * np.outer(s, s)
+        d = -H @ g
+        xn = x + d
+        pairs.append((xn - x, A @ xn - g))
+        xs.append(xn)
+        x = xn
+    return xs
+
+
+@pytest.mark.parametrize("mem", (1, 2, 3))
+def test_lbfgs_iterates_equal_dense_bfgs_on_a_quadratic(mem):
+    """Cpp/src/LBFGS.cpp:63-103: H0 from the newest pair (:69-81), the
+    history count min(k-1, m) (:83-86) and the ring index (:97).  On a
+    quadratic with eigenvalues in (0.5, 1.5) every unit step satisfies weak
+    Wolfe at (0, 0.5) (asserted below), so the line search accepts it with one
+    evaluation and the iterates are those of the dense BFGS recursion with
+    the same memory.  Five iterations at memory 1-3 wrap the ring."""
+    n, iters = 4, 5
+    A = _spd(5, n, 0.5, 1.5)
+    x0 = G.normal(6, n)
+    quad = lambda x: (0.5 * float(x @ A @ x), A @ x)
+    xs = _dense_bfgs_iterates(A, x0, mem, iters)
+    for k in range(iters):
+        x, xn = xs[k], xs[k + 1]
+        d = xn - x
+        f, g = quad(x)
+        fn, gn = quad(xn)
+        assert fn < f and gn @ d > 0.5 * (g @ d), k   # weak Wolfe at (0, 0.5) for the unit step
+    for k in range(1, iters + 1):
+        x, f, tr = oracle.lbfgs(quad, x0, itr_max=k, tol=0.0, wolfe1=0.0, wolfe2=0.5, mem=mem)
+        assert tr["status"] == 0 and tr["iters"] == k
+        assert np.all(tr["evals"][1:] == 1)
+        assert rel_err(x, xs[k]) <= 1e-12, (mem, k)
+    # a ring of the wrong size is not the same method: the memory-1 and
+    # memory-2 sequences part from the third iterate on
+    assert rel_err(_dense_bfgs_iterates(A, x0, 1, iters)[3], _dense_bfgs_iterates(A, x0, 2, iters)[3]) > 1e-6
+
+
+def test_line_search_limits_truncate_toward_zero():
+    """Cpp/src/lineSearch.cpp:52-53: bisMax = max(50, log2(1e5 |d|)) and
+    epdMax = max(10, log2(1e5 / |d|)) are doubles stored in an int, so the
+    logarithm is truncated.  The loop (:57) runs while bisCnt <= bisMax and
+    epdCnt <= epdMax, one evaluation per pass.
+    Expansion: on f = c.x along d = -c every step satisfies Armijo with
+    c1 = 0 and fails the curvature test, so the step doubles until
+    epdCnt > epdMax: |d| = 1 gives log2(1e5) = 16.61, epdMax = 16, 17
+    evaluations, code 2.
+    Bisection: on f = x^2/2 from x = R along d = -R with (c1, c2) = (0.9,
+    0.1) no step meets both conditions (Armijo needs t < 0.2, the curvature
+    test t > 0.9), so every pass bisects: R = 3e10 gives log2(3e15) = 51.41,
+    bisMax = 51, 52 evaluations, code 3; R = 1 gives bisMax = 50, 51."""
+    c = np.array([0.6, 0.8])
+    lin = lambda x: (float(c @ x), c.copy())
+    code, _, _, _, _, cnt = oracle.line_search(lin, np.zeros(2), c, -c, 0.0, 0.0, 0.5)
+    assert (code, cnt) == (2, 17)
+    c2 = 1e-3 * c   # |d| = 1e-3: log2(1e8) = 26.58 -> epdMax 26
+    lin2 = lambda x: (float(c2 @ x), c2.copy())
+    code, _, _, _, _, cnt = oracle.line_search(lin2, np.zeros(2), c2, -c2, 0.0, 0.0, 0.5)
+    assert (code, cnt) == (2, 27)
+    quad = lambda x: (0.5 * float(x @ x), x.copy())
+    for R, expect in ((3e10, 52), (1.0, 51)):
+        x = np.array([R])
+        code, _, _, _, _, cnt = oracle.line_search(quad, x, x, -x, 0.5 * R * R, 0.9, 0.1)
+        assert (code, cnt) == (3, expect), (R, code, cnt)
+
+
+@pytest.mark.parametrize("dim", (2, 3))
+@pytest.mark.parametrize("ct", "GC")
+@pytest.mark.parametrize("dt", "BLOU")
+def test_varifold_terms_combine_to_the_reference_value(golden_varifold, dim, ct, dt):
+    """DD - 2 DT + TT of orc_varifold_terms (the three pair sums the
+    objective caches separately) equals the compiled reference's varifold
+    value (Cpp/src/varifold3D.cpp:259-357, varifold2D.cpp:126-216)."""
+    g = golden_varifold
+    d = "d%d" % dim
+    t = "%s_%s%s" % (d, ct, dt)
+    cw, dw = g[t + "_w"]
+    dd, dtt, tt = oracle.varifold_terms(g[d + "_q"], g[d + "_elm"], g[d + "_tc"], g[d + "_tu"], g[d + "_ta"],
+                                        ct, cw, dt, dw)
+    ref = float(g[t + "_val"])
+    assert abs(dd - 2 * dtt + tt - ref) <= 1e-13 * max(abs(dd), abs(tt), abs(ref))
+    assert dd > 0 and tt > 0 or dt in "LO"
