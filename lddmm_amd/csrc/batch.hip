@@ -1376,7 +1376,8 @@ BatchRunResult Batch::run(double *x, const BatchLbfgsOptions &bo)
         st[p].glen = std::sqrt(hs[GG * P + p]);
         st[p].evals = 1;
         st[p].prec = run_prec;
-        if (!(st[p].glen >= o.tol) || o.itr_max == 0) st[p].active = false;
+        // Cpp/src/LBFGS.cpp:66 (a NaN gradient norm goes on to the direction, where it stops with code 1)
+        if (st[p].glen < o.tol || o.itr_max == 0) st[p].active = false;
     }
 
     auto has_target = [&](int p) { return p < (int)bo.targets.size() || o.has_target; };
@@ -1461,7 +1462,7 @@ BatchRunResult Batch::run(double *x, const BatchLbfgsOptions &bo)
                 St &q = st[p];
                 q.slope = hs[GD * P + p];
                 q.need_dir = false;
-                if (q.slope > -DBL_MIN) {
+                if (!(q.slope <= -DBL_MIN)) {   // a NaN slope is not a descent direction either
                     if (can_refine && q.prec == kFP32) {
                         stalled.push_back(p);
                         continue;
@@ -1526,7 +1527,8 @@ BatchRunResult Batch::run(double *x, const BatchLbfgsOptions &bo)
             ++q.evals;
             bool ok = false;
             if (gn < o.tol) ok = true;
-            else if (fn >= q.f + o.wolfe1 * q.rel * q.slope) q.hi = q.rel;
+            else if (fn >= q.f + o.wolfe1 * q.rel * q.slope || std::isnan(fn) || std::isnan(sl))
+                q.hi = q.rel;   // a NaN energy or slope shrinks the step like a failed Armijo test
             else if (sl <= o.wolfe2 * q.slope) q.lo = q.rel;
             else ok = true;
             if (ok) {
@@ -1623,11 +1625,15 @@ BatchRunResult Batch::run(double *x, const BatchLbfgsOptions &bo)
             }
         }
         refine(stalled);
-        for (int p : stalled)
+        for (int p : stalled) {
             if (has_target(p) && st[p].f <= target_of(p)) {
                 st[p].reached = 1;
                 st[p].active = false;
+            } else if (st[p].it >= o.itr_max) {
+                // the FP64 re-evaluation does not grant an iteration beyond the cap
+                st[p].active = false;
             }
+        }
     }
     LDDMM_HIP(hipStreamSynchronize(st_));
     BatchRunResult r;

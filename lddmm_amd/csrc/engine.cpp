@@ -467,13 +467,14 @@ void Objective::eval_internal(const double *alp, double *grd, double *f_dev)
 // ======================================================================
 namespace {
 
-// device scalar slots
-enum : int { kF = 0, kGG, kGD, kDD, kSD, kYD, kYY, kNumScal };
+// device scalar slots; kF..kDD are read back together after a trial
+enum : int { kF = 0, kGG, kGD, kSL, kDD, kNumScal };
 
 struct LbfgsState {
     Slab slab;
     double *g = nullptr, *d = nullptr, *xt = nullptr, *gt = nullptr, *S = nullptr, *Y = nullptr;
-    double *scal = nullptr, *sy = nullptr, *rec = nullptr, *yy = nullptr, *scratch = nullptr;
+    double *scal = nullptr, *SY = nullptr, *YY = nullptr, *rdots = nullptr, *coef = nullptr, *work = nullptr;
+    double *scratch = nullptr, *rscratch = nullptr;
     double *host = nullptr;   // pinned mirror of scal
     ~LbfgsState()
     {
@@ -496,13 +497,24 @@ void fetch(LbfgsState &s, int first, int count, hipStream_t st)
 
 }  // namespace
 
+// The L-BFGS of Cpp/src/LBFGS.cpp:25-112 with the line search of
+// Cpp/src/lineSearch.cpp:26-110.  The direction is the compact form of the
+// two-loop recursion (vector_kernels.hip): per accepted pair the tables
+// s_i.y_j and y_i.y_j gain one row and column (3h products in one launch),
+// and a direction is 2h products in one launch, the recursion coefficients
+// in one block and one fused pass that also forms g.d and d.d -- a constant
+// number of launches whatever the memory.  The first trial point (step 1)
+// is evaluated before the host reads the slope and |d|, so every trial costs
+// one host read; the host discards it when the direction is not a descent
+// direction.  The ring has min(memNum, itrMax) slots: a run of itrMax
+// iterations never stores more pairs, so the iterates are those of memNum.
 LbfgsResult run_lbfgs(Objective &obj, double *x_user, const LbfgsOptions &o, hipStream_t st)
 {
     auto t0 = std::chrono::steady_clock::now();
     if (o.mem < 1) throw Error(kErrArg, "memNum must be at least 1");
     if (o.itr_max < 0) throw Error(kErrArg, "itrMax must be non-negative");
     const long long nv = obj.problem().varnum();
-    const int m = o.mem;
+    const int m = std::max(1, std::min(o.mem, o.itr_max));
     LbfgsState s;
     // the iterate and every L-BFGS vector live in the objective's internal
     // vertex order; x_user is read at the start and written at the end
@@ -515,10 +527,13 @@ LbfgsResult run_lbfgs(Objective &obj, double *x_user, const LbfgsOptions &o, hip
     s.slab.reserve(&s.S, (size_t)nv * m);
     s.slab.reserve(&s.Y, (size_t)nv * m);
     s.slab.reserve(&s.scal, kNumScal);
-    s.slab.reserve(&s.sy, m);
-    s.slab.reserve(&s.rec, m);
-    s.slab.reserve(&s.yy, m);
+    s.slab.reserve(&s.SY, (size_t)m * m);
+    s.slab.reserve(&s.YY, (size_t)m * m);
+    s.slab.reserve(&s.rdots, (size_t)3 * m);
+    s.slab.reserve(&s.coef, (size_t)1 + 2 * m);
+    s.slab.reserve(&s.work, (size_t)3 * m);
     s.slab.reserve(&s.scratch, (size_t)kMaxDots * kDotBlocks);
+    s.slab.reserve(&s.rscratch, ring_dot_scratch(3 * m));
     s.slab.allocate();
     LDDMM_HIP(hipHostMalloc((void **)&s.host, sizeof(double) * kNumScal, hipHostMallocDefault));
     obj.to_internal(x_user, x, obj.problem().T - 1);
@@ -570,56 +585,41 @@ LbfgsResult run_lbfgs(Objective &obj, double *x_user, const LbfgsOptions &o, hip
         }
     };
     const bool can_refine = o.fp64_refine && obj.precision() == kFP32;
+    // one trial point x + rel d: f, |g|^2 and g.d into kF, kGG, kGD
+    auto trial = [&](double rel) {
+        {
+            ProfScope ps(kProfLbfgsVector, st);
+            vec_try(s.xt, x, rel, s.d, nv, st);
+        }
+        obj.eval_internal(s.xt, s.gt, s.scal + kF);
+        const double *xs[2] = {s.gt, s.gt}, *ys[2] = {s.gt, s.d};
+        dots(s, 2, xs, ys, nv, s.scal + kGG, st);
+    };
 
-    int new_idx = -1, status = 0;
+    int new_idx = -1, status = 0, stored = 0;
     for (int it = 1; it <= o.itr_max; ++it) {
         if (glen < o.tol) break;
         const bool in_fp32 = obj.precision() == kFP32;
 
-        // direction: two-loop recursion (Cpp/src/getDirection.cpp:9-66)
-        const int his = it <= m ? it - 1 : m;
+        // direction (Cpp/src/getDirection.cpp:9-66, compact form), g.d and d.d
+        const int his = std::min(stored, m);
         {
+            if (his > 0) {
+                const double *rings[2] = {s.S, s.Y}, *vecs[2] = {s.g, s.g};
+                ProfScope ps(kProfLbfgsDots, st);
+                launch_ring_dots(2, rings, vecs, his, nv, s.rscratch, s.rdots, st);
+            }
             ProfScope ps(kProfLbfgsVector, st);
-            if (his == 0) {
-                vec_copy_neg(s.d, s.g, nv, st);
-            } else {
-                LDDMM_HIP(hipMemcpyAsync(s.d, s.g, sizeof(double) * nv, hipMemcpyDeviceToDevice, st));
-            }
+            launch_lbfgs_coef(s.rdots, s.SY, s.YY, m, his, new_idx < 0 ? 0 : new_idx, s.work, s.coef, st);
+            launch_lbfgs_dir(s.d, s.g, s.S, s.Y, nv, s.coef, his, m, s.scratch, s.scal + kSL, st);
         }
-        if (his > 0) {
-            for (int h = 0; h < his; ++h) {
-                int idx = new_idx - h;
-                if (idx < 0) idx += m;
-                const double *xs[1] = {s.S + (size_t)idx * nv}, *ys[1] = {s.d};
-                dots(s, 1, xs, ys, nv, s.scal + kSD, st);
-                ProfScope ps(kProfLbfgsVector, st);
-                vec_loop1(s.d, s.Y + (size_t)idx * nv, s.scal + kSD, s.sy + idx, s.rec + idx, nv, st);
-            }
-            {
-                ProfScope ps(kProfLbfgsVector, st);
-                vec_scale_h0(s.d, s.sy + new_idx, s.yy + new_idx, nv, st);
-            }
-            int old = his < m ? 0 : new_idx + 1;
-            if (old == m) old = 0;
-            for (int h = 0; h < his; ++h) {
-                int idx = old + h;
-                if (idx >= m) idx -= m;
-                const double *xs[1] = {s.Y + (size_t)idx * nv}, *ys[1] = {s.d};
-                dots(s, 1, xs, ys, nv, s.scal + kYD, st);
-                ProfScope ps(kProfLbfgsVector, st);
-                vec_loop2(s.d, s.S + (size_t)idx * nv, s.scal + kYD, s.sy + idx, s.rec + idx, h == his - 1, nv,
-                          st);
-            }
-        }
-
-        // weak-Wolfe line search (Cpp/src/lineSearch.cpp:26-110)
-        {
-            const double *xs[2] = {s.g, s.d}, *ys[2] = {s.d, s.d};
-            dots(s, 2, xs, ys, nv, s.scal + kGD, st);
-        }
-        fetch(s, kGD, 2, st);
-        const double slope = s.host[kGD];
-        if (slope > -DBL_MIN && in_fp32 && can_refine) {
+        // the first trial (step 1) before the slope is read
+        trial(1.0);
+        fetch(s, kF, 5, st);
+        const double slope = s.host[kSL];
+        // a NaN slope is not a descent direction either
+        const bool descent = slope <= -DBL_MIN;
+        if (!descent && in_fp32 && can_refine) {
             refine("not a descent direction");
             if (o.has_target && fnow <= o.target) {
                 r.reached_target = 1;
@@ -628,29 +628,27 @@ LbfgsResult run_lbfgs(Objective &obj, double *x_user, const LbfgsOptions &o, hip
             --it;
             continue;
         }
-        if (slope > -DBL_MIN) {
+        if (!descent) {
             printf("\nNot a descent direction.\nQuit LBFGS.\n\n");
             fflush(stdout);
             status = 1;
             break;
         }
+
+        // weak-Wolfe line search (Cpp/src/lineSearch.cpp:26-110)
         const double dlen = std::sqrt(s.host[kDD]);
         const int bis_max = (int)std::fmax(50.0, std::log2(1e5 * dlen));
         const int epd_max = (int)std::fmax(10.0, std::log2(1e5 / dlen));
         double lo = 0.0, hi = DBL_MAX, rel = 1.0, fnew = 0.0, gnew = 0.0, step = 0.0;
         int bis = 0, epd = 0, cnt = 0, le = -1;
+        bool pending = true;   // the step-1 trial is evaluated and read
         while (bis <= bis_max && epd <= epd_max) {
-            {
-                ProfScope ps(kProfLbfgsVector, st);
-                vec_try(s.xt, x, rel, s.d, nv, st);
+            if (!pending) {
+                trial(rel);
+                fetch(s, kF, 3, st);
             }
-            obj.eval_internal(s.xt, s.gt, s.scal + kF);
+            pending = false;
             ++cnt;
-            {
-                const double *xs[2] = {s.gt, s.gt}, *ys[2] = {s.gt, s.d};
-                dots(s, 2, xs, ys, nv, s.scal + kGG, st);
-            }
-            fetch(s, kF, 3, st);
             fnew = s.host[kF];
             gnew = std::sqrt(s.host[kGG]);
             const double slope_t = s.host[kGD];
@@ -659,7 +657,8 @@ LbfgsResult run_lbfgs(Objective &obj, double *x_user, const LbfgsOptions &o, hip
                 le = 0;
                 break;
             }
-            if (fnew >= fnow + o.wolfe1 * rel * slope) hi = rel;
+            // a NaN energy or slope shrinks the step like a failed Armijo test
+            if (fnew >= fnow + o.wolfe1 * rel * slope || std::isnan(fnew) || std::isnan(slope_t)) hi = rel;
             else if (slope_t <= o.wolfe2 * slope) lo = rel;
             else {
                 step = rel * dlen;
@@ -700,17 +699,19 @@ LbfgsResult run_lbfgs(Objective &obj, double *x_user, const LbfgsOptions &o, hip
         }
 
         // accept: curvature pair into the ring buffer (Cpp/src/LBFGS.cpp:94-105)
+        // and its row and column of the tables
         new_idx = (new_idx == m - 1) ? 0 : new_idx + 1;
+        stored = std::min(stored + 1, m);
         double *sv = s.S + (size_t)new_idx * nv, *yv = s.Y + (size_t)new_idx * nv;
         {
             ProfScope ps(kProfLbfgsVector, st);
             vec_accept(x, s.g, s.xt, s.gt, sv, yv, nv, st);
         }
         {
-            const double *xs[2] = {sv, yv}, *ys[2] = {yv, yv};
-            dots(s, 2, xs, ys, nv, s.scal + kSD, st);
-            LDDMM_HIP(hipMemcpyAsync(s.sy + new_idx, s.scal + kSD, sizeof(double), hipMemcpyDeviceToDevice, st));
-            LDDMM_HIP(hipMemcpyAsync(s.yy + new_idx, s.scal + kYD, sizeof(double), hipMemcpyDeviceToDevice, st));
+            const double *rings[3] = {s.Y, s.S, s.Y}, *vecs[3] = {sv, yv, yv};
+            ProfScope ps(kProfLbfgsDots, st);
+            launch_ring_dots(3, rings, vecs, stored, nv, s.rscratch, s.rdots, st);
+            launch_lbfgs_store(s.rdots, stored, new_idx, m, s.SY, s.YY, st);
         }
         const double fprev = fnow;
         fnow = fnew;
