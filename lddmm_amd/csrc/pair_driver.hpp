@@ -105,18 +105,81 @@ struct PrecF32 {
 template <int ORDER, typename real, int D = 3>
 struct KernelFnT;
 
-// FP64: the reference's formulas (kappa = cmul = 1)
+// FP64, Gaussian (2D, 3D) and 3D Matern: the coordinates are scaled as in
+// FP32 below (anchored offsets times kappa, formed in FP64), so the kernel is
+// k = 2^-r2 (Gaussian) or a polynomial in s = sqrt(r2) times 2^-s (Matern),
+// with exp2_neg in place of the library exp and no per-pair multiply by the
+// width; the gradient factors' constants move to the flush (cmul / kappa).
+// Rounding: a scaled difference carries 2^-53 (|x_i - a| + |x_j - a|) kappa
+// per component instead of 2^-53 |x_i - x_j|; 2^-s is about 1 ulp.  FP64,
+// 2D Matern: the reference's formulas on unscaled coordinates (kappa = cmul
+// = 1; the Bessel factors need d itself).
 template <int ORDER, int D>
 struct KernelFnT<ORDER, double, D> {
-    __host__ __device__ static double kappa(double, double) { return 1.0; }
-    __host__ __device__ static double cmul(double, double) { return 1.0; }
+    static constexpr bool kScaled = !(D == 2 && ORDER >= 0);
+    static constexpr double kLog2e = 1.4426950408889634, kLn2 = 0.6931471805599453;
+    static constexpr double L1 = kLn2, L2 = kLn2 * kLn2, L3 = kLn2 * kLn2 * kLn2, L4 = kLn2 * kLn2 * kLn2 * kLn2;
+    __host__ __device__ static double kappa(double iw, double iw2)
+    {
+        if constexpr (!kScaled) return 1.0;
+        else return ORDER == -1 ? sqrt(iw2 * kLog2e) : iw * kLog2e;
+    }
+    __host__ __device__ static double cmul(double, double iw2)
+    {
+        if constexpr (!kScaled) return 1.0;
+        switch (ORDER) {
+        case -1: return -2.0 * iw2;
+        case 0: return 0.0;   // order 0 has no dq (Cpp/src/dqKernel.cpp:39)
+        case 1: return -iw2;
+        case 2: return -iw2 / 3.0;
+        case 3: return -iw2 * 3.0 / 15.0;
+        default: return -iw2 * 15.0 / 105.0;
+        }
+    }
+    // P_1 .. P_4 of d = s ln 2 (Cpp/src/computeKernel.cpp:170-234)
+    __device__ __forceinline__ static double poly(int p, double s)
+    {
+        switch (p) {
+        case 1: return fma(s, L1, 1.0);
+        case 2: return fma(s, fma(s, L2 * (1.0 / 3.0), L1), 1.0);
+        case 3: return fma(s, fma(s, fma(s, L3 * (1.0 / 15.0), L2 * (6.0 / 15.0)), L1), 1.0);
+        default:
+            return fma(s, fma(s, fma(s, fma(s, L4 * (1.0 / 105.0), L3 * (10.0 / 105.0)), L2 * (45.0 / 105.0)), L1),
+                       1.0);
+        }
+    }
     __device__ __forceinline__ static double value(double r2, double iw, double iw2)
     {
-        return KernelFnD<D, ORDER>::value(r2, iw, iw2);
+        if constexpr (!kScaled) {
+            return KernelFnD<D, ORDER>::value(r2, iw, iw2);
+        } else if constexpr (ORDER == -1) {
+            return exp2_neg(r2);
+        } else {
+            const double s = sqrt_r2(r2), e = exp2_neg(s);
+            if constexpr (ORDER == 0) return e;
+            else return poly(ORDER, s) * e;
+        }
     }
     __device__ __forceinline__ static void eval(double r2, double iw, double iw2, double &k, double &cf)
     {
-        KernelFnD<D, ORDER>::eval(r2, iw, iw2, k, cf);
+        if constexpr (!kScaled) {
+            KernelFnD<D, ORDER>::eval(r2, iw, iw2, k, cf);
+        } else if constexpr (ORDER == -1) {
+            k = exp2_neg(r2);
+            cf = k;
+        } else {
+            const double s = sqrt_r2(r2), e = exp2_neg(s);
+            if constexpr (ORDER == 0) {
+                k = e;
+                cf = 0.0;
+            } else if constexpr (ORDER == 1) {
+                k = poly(1, s) * e;
+                cf = e;
+            } else {
+                k = poly(ORDER, s) * e;
+                cf = poly(ORDER - 1, s) * e;
+            }
+        }
     }
 };
 
@@ -124,6 +187,7 @@ struct KernelFnT<ORDER, double, D> {
 // FP32 squared distance (unscaled), rounded to FP32.
 template <int ORDER>
 struct KernelFnT<ORDER, float, 2> {
+    static constexpr bool kScaled = true;   // anchored (kappa = 1)
     __host__ __device__ static double kappa(double, double) { return 1.0; }
     __host__ __device__ static double cmul(double, double) { return 1.0; }
     __device__ __forceinline__ static float value(float r2, float iw, float iw2)
@@ -148,6 +212,7 @@ struct KernelFnT<ORDER, float, 2> {
 // Cpp/src/dqKernel.cpp:215-291 written in s (coefficients times ln 2^m).
 template <int ORDER>
 struct KernelFnT<ORDER, float, 3> {
+    static constexpr bool kScaled = true;
     static constexpr double kLog2e = 1.4426950408889634, kLn2 = 0.6931471805599453;
     static constexpr float L1 = (float)kLn2, L2 = (float)(kLn2 * kLn2), L3 = (float)(kLn2 * kLn2 * kLn2),
                            L4 = (float)(kLn2 * kLn2 * kLn2 * kLn2);
@@ -230,6 +295,14 @@ __host__ __device__ inline float expand_r2(bool on) { return on ? 8.0f : -1.0f; 
 template <class Op, class = void>
 struct ExpandOf : std::false_type {
 };
+// operators whose coordinates are anchored and scaled offsets: every FP32
+// operator, and FP64 operators that declare kAnchor
+template <class Op, class = void>
+struct AnchorOf : std::integral_constant<bool, (sizeof(typename Op::crd_t) < sizeof(double))> {
+};
+template <class Op>
+struct AnchorOf<Op, std::void_t<decltype(Op::kAnchor)>> : std::integral_constant<bool, Op::kAnchor> {
+};
 template <class Op>
 struct ExpandOf<Op, std::void_t<decltype(Op::kExpand)>> : std::integral_constant<bool, Op::kExpand> {
 };
@@ -276,7 +349,7 @@ __device__ __forceinline__ void pairs_block(const typename Op::Args &args, int n
     constexpr int D = Op::D, F = Op::F, OUT = Op::OUT, R = Op::R, TI = 64 * R;
     using lds_t = typename Op::lds_t;
     using crd_t = typename Op::crd_t;
-    constexpr bool kAnchor = sizeof(crd_t) < sizeof(double);
+    constexpr bool kAnchor = AnchorOf<Op>::value;
     using SM = PairSmem<Op>;
     auto red = reinterpret_cast<double (*)[OUT][TI]>(smem);
     // the wave index is uniform: as a scalar it keeps the source loop's

@@ -32,6 +32,7 @@ struct OpKV {
     static constexpr bool kCull = P::fp32 && ORDER == -1;
     static constexpr bool kExpand = kCull;   // expanded form (pair_driver.hpp)
     using KF = KernelFnT<ORDER, real, D_>;
+    static constexpr bool kAnchor = KF::kScaled;   // scaled offsets (pair_driver.hpp)
     struct Args {
         const double *qi, *qj, *a;
         int ni, nj;
@@ -118,6 +119,7 @@ struct OpBwd {
     static constexpr bool kCull = P::fp32 && ORDER == -1;
     static constexpr bool kExpand = kCull;   // expanded form (pair_driver.hpp)
     using KF = KernelFnT<ORDER, real, D_>;
+    static constexpr bool kAnchor = KF::kScaled;   // scaled offsets (pair_driver.hpp)
     struct Args {
         const double *q, *a, *p;
         int n;
@@ -188,7 +190,7 @@ struct OpBwd {
 #pragma unroll
         for (int c = 0; c < D; ++c) {
             tot[c] += (double)a.g[c];
-            if constexpr (P::fp32) tot[D + c] += (double)a.pd[c] * A.pds;
+            if constexpr (kAnchor) tot[D + c] += (double)a.pd[c] * A.pds;
             else tot[D + c] += a.pd[c];
         }
     }
@@ -237,6 +239,7 @@ struct OpDqSelf {
     using crd_t = real;
     static constexpr bool kCull = false;
     using KF = KernelFnT<ORDER, real, D_>;
+    static constexpr bool kAnchor = KF::kScaled;   // scaled offsets (pair_driver.hpp)
     struct Args {
         const double *q, *l, *r;
         int n;
@@ -299,7 +302,7 @@ struct OpDqSelf {
     {
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-            if constexpr (P::fp32) tot[c] += (double)a.pd[c] * A.pds;
+            if constexpr (kAnchor) tot[c] += (double)a.pd[c] * A.pds;
             else tot[c] += a.pd[c];
         }
     }
@@ -317,6 +320,7 @@ struct OpDqCross {
     using crd_t = real;
     static constexpr bool kCull = false;
     using KF = KernelFnT<ORDER, real, D_>;
+    static constexpr bool kAnchor = KF::kScaled;   // scaled offsets (pair_driver.hpp)
     struct Args {
         const double *qi, *u, *qj, *v;
         int ni, nj;
@@ -373,7 +377,7 @@ struct OpDqCross {
     {
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-            if constexpr (P::fp32) tot[c] += (double)a.pd[c] * A.pds;
+            if constexpr (kAnchor) tot[c] += (double)a.pd[c] * A.pds;
             else tot[c] += a.pd[c];
         }
     }

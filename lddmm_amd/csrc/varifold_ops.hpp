@@ -34,6 +34,25 @@ __device__ __forceinline__ float cen_value_scaled(float r2)
     else return __builtin_amdgcn_rcpf(1.0f + r2);
 }
 
+// 1 / x for x >= 1 in FP64: the hardware reciprocal v_rcp_f64 and two
+// Newton steps y += y (1 - x y) (each doubles the correct bits; within an
+// ulp), in place of the library division's scaling and fix-up sequence
+__device__ __forceinline__ double rcp_ge1(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+    y = fma(y, fma(-x, y, 1.0), y);
+    return fma(y, fma(-x, y, 1.0), y);
+}
+
+// FP64 position kernels on coordinates scaled by cen_kappa: 'G' 2^-r2 by
+// exp2_neg, 'C' 1 / (1 + r2) by rcp_ge1
+template <int CEN>
+__device__ __forceinline__ double cen_value_scaled64(double r2)
+{
+    if constexpr (CEN == kCenG) return exp2_neg(r2);
+    else return rcp_ge1(1.0 + r2);
+}
+
 template <int CEN>
 __host__ __device__ inline double cen_kappa(double ics2)
 {
@@ -75,6 +94,8 @@ struct OpVf {
     using crd_t = real;
     static constexpr bool kCull = P::fp32 && CEN == kCenG;
     static constexpr bool kExpand = kCull;   // expanded form (pair_driver.hpp)
+    // anchored centres scaled by cen_kappa in both precisions
+    static constexpr bool kAnchor = true;
     struct Args {
         const double *ci, *ui;   // targets: deformed element centres / directions
         int ni;
@@ -83,7 +104,7 @@ struct OpVf {
         real ics2, ids2;
         float cut2;   // culling range (cull_r2); +inf: none
         float xr2;    // expanded-form radius bound (expand_r2)
-        double kap;   // FP32: cen_kappa of the staged offsets
+        double kap;   // cen_kappa of the staged offsets
     };
     __device__ __forceinline__ static double crd_scale(const Args &A) { return A.kap; }
     struct Tgt {
@@ -129,7 +150,7 @@ struct OpVf {
         }
         real kc;
         if constexpr (P::fp32) kc = cen_value_scaled<CEN>(r2);
-        else kc = cen_value<CEN, real>(r2, A.ics2);
+        else kc = cen_value_scaled64<CEN>(r2);
         const real w = kc * fj[D];
         if constexpr (kBinet32) {
             // ku = ang^2: wk = (w ang) ang, u-sum of (w ang) u_j (times 2 at the flush)
@@ -162,17 +183,13 @@ struct OpVf {
     {
         tot[0] += (double)a.e;
         tot[1] += DIR == kDirB ? -(double)a.e : (double)a.s;
-        // FP32: the g-sum is over scaled differences (1 / kappa)
-        const double gs = P::fp32 ? 1.0 / A.kap : 1.0, us = kBinet32 ? 2.0 : 1.0;
+        // the g-sum is over scaled differences (1 / kappa)
+        const double gs = 1.0 / A.kap, us = kBinet32 ? 2.0 : 1.0;
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-            if constexpr (P::fp32) {
-                tot[2 + c] += (double)a.g[c] * gs;
-                tot[2 + D + c] += (double)a.u[c] * us;
-            } else {
-                tot[2 + c] += a.g[c];
-                tot[2 + D + c] += a.u[c];
-            }
+            tot[2 + c] += (double)a.g[c] * gs;
+            if constexpr (P::fp32) tot[2 + D + c] += (double)a.u[c] * us;
+            else tot[2 + D + c] += a.u[c];
         }
     }
     // expanded form (Gaussian position kernel): kc = exp2(-t); the g-sum
