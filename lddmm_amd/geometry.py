@@ -237,16 +237,28 @@ def _geodesic_cached(freq):
     return _GEO_CACHE[freq]
 
 
-def batch_problems(count, seed, nmin=500, nmax=20000, jitter=0.01):
+def batch_sizes(count, seed, nmin=500, nmax=20000):
+    """Vertex counts of batch_problems(count, seed, nmin, nmax), without
+    building the problems."""
+    u = uniform(seed, 2 * count)
+    out = []
+    for p in range(count):
+        nn = np.exp(np.log(nmin) + (np.log(nmax) - np.log(nmin)) * u[2 * p])
+        f = max(1, int(round(np.sqrt((nn - 2) / 10.0))))
+        out.append(10 * f * f + 2)
+    return np.array(out)
+
+
+def batch_problems(count, seed, nmin=500, nmax=20000, jitter=0.01, indices=None):
     """Uneven batch of 3D registrations: vertex counts log-uniform in
     [nmin, nmax] (rounded to a geodesic sphere of frequency
     f = round(sqrt((N - 2) / 10)), N = 10 f^2 + 2); sources are jittered
     unit spheres (Gaussian vertex noise of std `jitter`), targets seeded
     superquadrics of the same frequency; deformation kernel widths uniform
-    in [0.15, 0.35]."""
+    in [0.15, 0.35].  `indices` builds only those problems of the batch."""
     u = uniform(seed, 2 * count)
     out = []
-    for p in range(count):
+    for p in (range(count) if indices is None else indices):
         nn = np.exp(np.log(nmin) + (np.log(nmax) - np.log(nmin)) * u[2 * p])
         f = max(1, int(round(np.sqrt((nn - 2) / 10.0))))
         s, tri = _geodesic_cached(f)

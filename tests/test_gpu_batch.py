@@ -163,3 +163,105 @@ def test_graph_replays_match_direct_launches(gpu):
         assert np.array_equal(x, y)
         assert p["fcn"] == q["fcn"] and p["iterations"] == q["iterations"]
     B.close()
+
+
+def oracle_problem(p, k):
+    ifo = oracle.get_elm_ifo(p["elm"], len(p["q0"]))
+    return oracle.OracleProblem(p["q0"], p["elm"], p["tgt_cen"], p["tgt_dir"], p["tgt_vol"], k["cen_type"],
+                                k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"], k["knl_order"],
+                                p["knl_width"], k["time_stp"], k["time_num"], ifo=ifo)
+
+
+def test_batched_objgrd_matches_oracle(gpu):
+    """Every problem of a batch (sizes 12 to 2,600 in one launch per kernel)
+    against the CPU oracle's objgrd (Cpp/src/objgrd.cpp:26-90) directly:
+    energy and gradient within 1e-10, at zero and at seeded momenta."""
+    probs, k = small_batch(), kernels()
+    B = gpu.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"], k["knl_order"],
+                  k["time_stp"], k["time_num"])
+    for scale in (0.0, 0.05):
+        alps = [scale * G.normal(140 + i, B.sizes[i]).reshape(k["time_num"] - 1, B.n[i], 3) for i in range(B.P)]
+        f, g = B.objgrd(alps)
+        for i, p in enumerate(probs):
+            ef, eg = oracle_problem(p, k).objgrd(alps[i].reshape(-1))
+            print("batch vs oracle: scale %.2f problem %d (N=%d): energy %.2e gradient %.2e" %
+                  (scale, i, B.n[i], abs(f[i] - ef) / abs(ef), rel_err(g[i].reshape(-1), eg)), flush=True)
+            assert abs(f[i] - ef) <= 1e-10 * abs(ef), (scale, i)
+            assert rel_err(g[i].reshape(-1), eg) <= 1e-10, (scale, i)
+    B.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_nan_in_one_problem_leaves_the_others_unchanged(gpu, precision):
+    """A NaN in one problem's initial momentum: its objective and gradient
+    are NaN, so its first direction has no negative slope and it stops with
+    status 1 (not a descent direction, Cpp/src/lineSearch.cpp:39-46) at its
+    start; every other problem runs exactly as in a batch without the NaN."""
+    probs, k = small_batch(), kernels()
+    B = gpu.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"], k["knl_order"],
+                  k["time_stp"], k["time_num"], chunk=256)
+    zero = [np.zeros((k["time_num"] - 1, B.n[i], 3)) for i in range(B.P)]
+    a_ref, i_ref, _ = B.run(8, 0.0, 0.0, 0.5, 5, alp0=zero, precision=precision)
+    bad = [z.copy() for z in zero]
+    bad[2][1, 3, 0] = np.nan
+    a_nan, i_nan, _ = B.run(8, 0.0, 0.0, 0.5, 5, alp0=bad, precision=precision)
+    print("NaN problem (%s): status %d, iterations %d" % (precision, i_nan[2]["status"], i_nan[2]["iterations"]),
+          flush=True)
+    assert i_nan[2]["status"] == 1 and i_nan[2]["iterations"] == 0
+    assert np.isnan(a_nan[2][1, 3, 0]) and np.count_nonzero(np.isnan(a_nan[2])) == 1   # the start is returned
+    for i in range(B.P):
+        if i == 2:
+            continue
+        assert i_nan[i]["status"] == i_ref[i]["status"] == 0
+        assert i_nan[i]["iterations"] == i_ref[i]["iterations"] and i_nan[i]["fcn"] == i_ref[i]["fcn"]
+        assert np.array_equal(a_nan[i], a_ref[i])
+    B.close()
+
+
+def test_batched_forced_line_search_failure_matches_oracle(gpu):
+    """Wolfe constants (0.9, 0.1) shared by the batch: on a locally quadratic
+    objective no step meets both, so a line search brackets and bisects to
+    bisMax and returns code 3 (Cpp/src/lineSearch.cpp:52-57,104-109), at the
+    first iteration or after a few accepted ones.  Every problem gives the
+    status, iterations and evaluations of its single-problem run, and of the
+    oracle's run for the problems of up to 700 points."""
+    probs, k = small_batch(), kernels()
+    B = gpu.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"], k["knl_order"],
+                  k["time_stp"], k["time_num"], chunk=256)
+    alps, infos, rounds = B.run(5, 0.0, 0.9, 0.1, 5)
+    print("forced failure: statuses %s, iterations %s, evaluations %s, %d rounds" %
+          ([r["status"] for r in infos], [r["iterations"] for r in infos], [r["evaluations"] for r in infos],
+           rounds), flush=True)
+    for i, p in enumerate(probs):
+        ifo = oracle.get_elm_ifo(p["elm"], len(p["q0"]))
+        _, i1 = gpu.run(p["q0"], None, p["elm"], ifo, p["tgt_cen"], p["tgt_vol"], p["tgt_dir"], k["cen_type"],
+                        k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"], k["knl_order"], p["knl_width"],
+                        k["time_stp"], k["time_num"], 5, 0.0, 0.9, 0.1, 5)
+        assert infos[i]["status"] == i1["status"] == 3
+        assert (infos[i]["iterations"], infos[i]["evaluations"]) == (i1["iterations"], i1["evaluations"])
+        if B.n[i] <= 700:
+            _, _, tr = oracle_problem(p, k).run(itr_max=5, tol=0.0, wolfe1=0.9, wolfe2=0.1, mem=5)
+            assert tr["status"] == 3 and (infos[i]["iterations"], infos[i]["evaluations"]) == \
+                (tr["iters"], tr["total_evals"]), (i, tr["iters"], tr["total_evals"])
+    B.close()
+
+
+def test_fp64_confirmation_does_not_exceed_the_iteration_cap(gpu):
+    """With a relative-decrease threshold of 1 every FP32 iteration hands
+    over to FP64 (the confirmation step of the FP32 policy); at the cap
+    itrMax = 1 the problems stop there instead of running a second
+    iteration, in the batch as in the single-problem driver."""
+    probs, k = small_batch(), kernels()
+    B = gpu.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"], k["knl_order"],
+                  k["time_stp"], k["time_num"], chunk=256)
+    for itr in (1, 2):
+        _, infos, _ = B.run(itr, 0.0, 0.0, 0.5, 5, precision="fp32", refine_rel_tol=1.0)
+        print("cap %d: iterations %s" % (itr, [r["iterations"] for r in infos]), flush=True)
+        assert all(r["iterations"] <= itr for r in infos)
+        p = probs[0]
+        ifo = oracle.get_elm_ifo(p["elm"], len(p["q0"]))
+        _, i1 = gpu.run(p["q0"], None, p["elm"], ifo, p["tgt_cen"], p["tgt_vol"], p["tgt_dir"], k["cen_type"],
+                        k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"], k["knl_order"], p["knl_width"],
+                        k["time_stp"], k["time_num"], itr, 0.0, 0.0, 0.5, 5, precision="fp32", refine_rel_tol=1.0)
+        assert i1["iterations"] <= itr
+    B.close()

@@ -67,14 +67,25 @@ def test_dq_kernel_cross(gpu, dim, order):
 
 
 def coincident_points(dim):
-    """Points with exact repeats (r2 = 0 between distinct indices) and with
-    partners 1e-120 and 1e-160 away (r2 below 2^-767 and subnormal): the
-    range where the FP64 Matern distance clamps r2 instead of scaling it."""
+    """Points with exact repeats (r2 = 0 between distinct indices) and a
+    cluster at the origin whose partners lie 1e-120, 1e-155 and 1e-160 away:
+    r2 = 1e-240 (below 2^-767), 1e-310 and 1e-320 (subnormal), the range
+    where the FP64 Matern distance adds 1e-300 to r2 instead of scaling it."""
     q = pts(21, 40, dim)
-    near = q[:10].copy()
-    near[:5, 0] += 1e-120
-    near[5:, 0] += 1e-160
-    return np.concatenate([q, q[:12], near])
+    tiny = np.zeros((4, dim))
+    tiny[1, 0], tiny[2, 0], tiny[3, dim - 1] = 1e-120, 1e-160, 1e-155
+    return np.concatenate([q, q[:12], tiny])
+
+
+def test_coincident_points_reach_tiny_r2():
+    """The fixture really holds squared distances in (0, 2^-767), subnormal
+    ones included (0.5 N(0,1) coordinates plus 1e-120 would round away)."""
+    for dim in (2, 3):
+        q = coincident_points(dim)
+        r2 = np.sum((q[:, None, :] - q[None, :, :]) ** 2, axis=-1)
+        pos = r2[r2 > 0]
+        assert np.any(pos < 2.0 ** -767) and np.any(pos < np.finfo(float).tiny)
+        assert np.any(r2[~np.eye(len(q), dtype=bool)] == 0)
 
 
 @pytest.mark.parametrize("dim,order", CASES)
@@ -145,6 +156,10 @@ def test_varifold_golden(gpu, golden_varifold, dim, ct, dt):
                            ct, cw, dt, dw)
     assert abs(val - float(g[t + "_val"])) <= TOL_OP * max(1.0, abs(float(g[t + "_val"])))
     assert rel_err(dq, g[t + "_dq"]) <= 1e-11
+    # value-only mode (the MATLAB varifold MEX with one output)
+    vv = gpu.varifold(g[d + "_q"], g[d + "_elm"], g[d + "_ifo"], g[d + "_tc"], g[d + "_tu"], g[d + "_ta"],
+                      ct, cw, dt, dw, grad=False)
+    assert abs(vv - float(g[t + "_valonly"])) <= TOL_OP * max(1.0, abs(float(g[t + "_valonly"])))
 
 
 def test_varifold_larger_and_without_target(gpu):
