@@ -295,6 +295,20 @@ int lddmm_set_pair_culling(int on);
  * direct form everywhere (results differ by FP32 rounding).  Process-wide,
  * read at every launch. */
 int lddmm_set_pair_expansion(int on);
+/* Executed-pair count (no counterpart in the reference; a diagnostic for pair
+ * rates): between begin and end, the operators that cull (FP32 Gaussian
+ * K.alpha, adjoint and 'G' varifold pair kernels) add the pairs they compute
+ * to device counters; end synchronises the device and returns the counts in
+ * executedPairs[3]: K.alpha, adjoint, varifold.
+ * Costs one atomic per computed source range while on; batches launch
+ * directly instead of replaying graphs meanwhile.  Process-wide. */
+int lddmm_pair_count_begin(void);
+/* Measured instruction rates of the device (no counterpart in the reference;
+ * the denominators of utilisation figures): rates[0] v_exp_f32, rates[1] FP32
+ * FMA, rates[2] FP64 FMA, per second over the whole chip, each from a grid of
+ * 4,096 blocks running independent dependency chains. */
+int lddmm_measure_rates(double *rates);
+int lddmm_pair_count_end(unsigned long long *executedPairs);
 /* Element varifold of a mesh (the target preprocessing of the examples,
  * Cpp/examples/circle2square/run_circle2square.m:21-24; the same formulas as
  * landmarksToVarifoldKernel, Cpp/src/varifold3D.cpp:8-39 and

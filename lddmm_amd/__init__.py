@@ -170,6 +170,26 @@ def set_pair_expansion(on):
     _check(lib().lddmm_set_pair_expansion(int(bool(on))))
 
 
+def measure_rates():
+    """Measured chip rates: dict(exp2_f32=v_exp_f32/s, fma_f32=FP32 FMA/s, fma_f64=FP64 FMA/s)."""
+    r = (C.c_double * 3)()
+    _check(lib().lddmm_measure_rates(r))
+    return dict(exp2_f32=r[0], fma_f32=r[1], fma_f64=r[2])
+
+
+def pair_count_begin():
+    """Start counting the pairs the culling pair kernels compute (diagnostic)."""
+    _check(lib().lddmm_pair_count_begin())
+
+
+def pair_count_end():
+    """Stop counting; returns the executed pairs since pair_count_begin() of
+    the culling kernels: dict(fwd_pairs=K.alpha, bwd_pairs=adjoint, vf_pairs=varifold)."""
+    v = (C.c_ulonglong * 3)()
+    _check(lib().lddmm_pair_count_end(v))
+    return dict(fwd_pairs=v[0], bwd_pairs=v[1], vf_pairs=v[2])
+
+
 # ---------------------------------------------------------------- operators
 def multiplyKernel(*args):
     """multiplyKernel(lmk, alp, order, width) or

@@ -84,6 +84,7 @@ struct BatchView {
     double *ecen, *edir, *evol, *dc, *dd, *efwd, *evf;
     int cull;     // culling of FP32 Gaussian tile pairs (kernel values < 2^-48)
     int expand;   // expanded form of the FP32 Gaussian pair terms
+    unsigned long long *exec;   // executed-pair counter (pair_counter), or null
 };
 
 template <int NC>
@@ -129,7 +130,8 @@ __global__ __launch_bounds__(kBlock) void kb_kv(BatchView v, int t, const int4 *
                         (real)d.iw2,
                         cull_r2(v.cull != 0),
                         expand_r2(v.expand != 0),
-                        Op::KF::kappa(d.iw, d.iw2)};
+                        Op::KF::kappa(d.iw, d.iw2),
+                        v.exec};
     const int j0 = it.z * d.chunk, j1 = min(d.n, j0 + d.chunk);
     pairs_block<Op>(A, d.n, it.y, j0, j1, v.part + d.po + (size_t)it.z * D * d.n, smem);
 }
@@ -154,7 +156,8 @@ __global__ __launch_bounds__(kBlock) void kb_bwd(BatchView v, int t, const int4 
                         cull_r2(v.cull != 0),
                         expand_r2(v.expand != 0),
                         kap,
-                        Op::KF::cmul(d.iw, d.iw2) / kap};
+                        Op::KF::cmul(d.iw, d.iw2) / kap,
+                        v.exec};
     const int j0 = it.z * d.chunk, j1 = min(d.n, j0 + d.chunk);
     pairs_block<Op>(A, d.n, it.y, j0, j1, v.part + d.po + (size_t)it.z * 2 * D * d.n, smem);
 }
@@ -266,6 +269,7 @@ __global__ __launch_bounds__(kBlock) void kb_vf(BatchView v, double ics2, double
     A.cut2 = cull_r2(v.cull != 0);
     A.xr2 = expand_r2(v.expand != 0);
     A.kap = cen_kappa<CEN>(ics2);
+    A.exec = v.exec;
     const int j0 = (tgt ? it.z - d.vsd : it.z) * d.vchunk, j1 = min(A.nj, j0 + d.vchunk);
     pairs_block<Op>(A, d.ne, it.y, j0, j1, v.vpart + d.vpo + (size_t)it.z * (2 * D + 2) * d.ne, smem);
 }
@@ -977,7 +981,7 @@ void Batch::eval(const double *alp, double *grd, double *f_dev, const std::vecto
 {
     build_tables(prec_of);
     if (tab_->nprobs == 0) return;
-    if (!use_graphs_ || Profiler::get().enabled()) {
+    if (!use_graphs_ || Profiler::get().enabled() || pair_counter()) {
         launch_eval(alp, grd, f_dev);
         return;
     }
@@ -1024,7 +1028,7 @@ void Batch::launch_eval(const double *alp, double *grd, double *f_dev)
     const Tables &tb = *tab_;
     BatchView v{desc_, nde_, padj_, part_, vpart_, alp, grd, q0_, tc_, tu_, ta_, elm_, ifo_, rows_, T,
                 spec_.h, spec_.w, ecen_, edir_, evol_, dc_, dd_, efwd_, evf_, pair_culling() ? 1 : 0,
-                pair_expansion() ? 1 : 0};
+                pair_expansion() ? 1 : 0, pair_counter()};
 
     // q_0 of the listed problems
     if (D == 2) kb_init_flow<2><<<tb.nvtile, kTile, 0, st_>>>(v, tb.vtile.p);
@@ -1160,7 +1164,7 @@ void Batch::autotune()
         const int g = spec_.prec == kFP32 ? 1 : 0;
         BatchView v{desc_, nde_, padj_, part_, vpart_, alp.p, nullptr, q0_, tc_, tu_, ta_, elm_, ifo_, rows_,
                     spec_.T, spec_.h, spec_.w, ecen_, edir_, evol_, dc_, dd_, efwd_, evf_, pair_culling() ? 1 : 0,
-                    pair_expansion() ? 1 : 0};
+                    pair_expansion() ? 1 : 0, nullptr};
         auto launch = [&] {
             dispatch_order(spec_.order, [&](auto O) {
                 if (g) {

@@ -1030,6 +1030,49 @@ int lddmm_set_pair_expansion(int on)
     return 0;
 }
 
+int lddmm_measure_rates(double *rates)
+{
+    return guarded([&] {
+        require(rates != nullptr, "null argument");
+        Stream st;
+        measure_rates(rates, st.s);
+        return 0;
+    });
+}
+
+int lddmm_pair_count_begin(void)
+{
+    return guarded([&] {
+        require(pair_counter() == nullptr, "a pair count is already running");
+        unsigned long long *c = nullptr;
+        LDDMM_HIP(hipMalloc((void **)&c, 3 * sizeof(unsigned long long)));
+        const hipError_t e = hipMemset(c, 0, 3 * sizeof(unsigned long long));
+        if (e != hipSuccess) {
+            (void)hipFree(c);
+            LDDMM_HIP(e);
+        }
+        set_pair_counter(c);
+        return 0;
+    });
+}
+
+int lddmm_pair_count_end(unsigned long long *executedPairs)
+{
+    return guarded([&] {
+        unsigned long long *c = pair_counter();
+        require(c != nullptr, "no pair count is running");
+        set_pair_counter(nullptr);
+        unsigned long long v[3] = {0, 0, 0};
+        hipError_t e = hipDeviceSynchronize();
+        if (e == hipSuccess) e = hipMemcpy(v, c, sizeof(v), hipMemcpyDeviceToHost);
+        (void)hipFree(c);
+        LDDMM_HIP(e);
+        if (executedPairs)
+            for (int k = 0; k < 3; ++k) executedPairs[k] = v[k];
+        return 0;
+    });
+}
+
 int lddmm_get_elm_ifo(int32_t *elmIfoMat, const int32_t *elmVtxMat, int vtxPerElm, int elmNum, int ndeNum)
 {
     return guarded([&] {

@@ -19,6 +19,12 @@ bool pair_culling();
 // direct form (results differ by FP32 rounding).
 void set_pair_expansion(bool on);
 bool pair_expansion();
+// Executed-pair counting (diagnostic): with three device counters set, the
+// culling operators (FP32 Gaussian K.a, adjoint and 'G' varifold, slots 0, 1,
+// 2) add the pairs they compute to theirs; null (the default) counts nothing.  A batch
+// launches directly (no graph replay) while a counter is set.
+void set_pair_counter(unsigned long long *dev_counter);
+unsigned long long *pair_counter();
 // Split plans for the point kernels: plan_kv for K.a and plan_bwd for the
 // fused backward pass (two targets per lane in FP32), plan_single for the dq
 // kernels (one target per lane).
@@ -102,5 +108,9 @@ void launch_energy(const double *fwd_parts, int nf, const double *vf_parts, int 
 // Vertex permutation of `rows` stacked length-n rows (perm[slot] = vertex):
 // gather dst[r*n + s] = src[r*n + perm[s]]; scatter dst[r*n + perm[s]] = src[r*n + s]
 void launch_permute(const double *src, double *dst, const int *perm, int n, int rows, bool gather, hipStream_t st);
+
+// ---- measured instruction rates (rates.hip): out[0] v_exp_f32 per second,
+// out[1] FP32 FMAs per second, out[2] FP64 FMAs per second, whole chip
+void measure_rates(double *out, hipStream_t st);
 
 }  // namespace lddmm

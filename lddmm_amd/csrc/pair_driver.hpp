@@ -42,6 +42,11 @@
 //   crd_scale(args)                    FP32 only: kappa of the staged offsets
 //   kCull, args.cut2                   culling (below)
 //
+// Executed pairs: with args.exec set (device counters, pair_counter() in
+// kernels.hpp), every culling operator adds the pairs of each source range it
+// does not skip to its slot Op::kCountSlot, so pair rates can count computed
+// pairs only.
+//
 // Culling (FP32 Gaussian kernels, kCull): a wave skips its range of staged
 // sources when every kernel value exp2(-r2) (scaled coordinates, KernelFnT)
 // between them and the block's targets is below 2^-48, i.e. r2 > 48.  Per
@@ -452,6 +457,10 @@ __device__ __forceinline__ void pairs_block(const typename Op::Args &args, int n
                     skip = gap > 0.0f && gap * gap > (float)args.cut2;
                 }
                 if (__builtin_amdgcn_readfirstlane((int)skip)) continue;
+                // diagnostic count of the pairs a culling operator computes
+                if (args.exec != nullptr && lane == 0)
+                    atomicAdd(args.exec + Op::kCountSlot,
+                              (unsigned long long)(t1 - t0) * (unsigned long long)min(TI, ni - tile * TI));
             }
             typename Op::Acc acc[R];
 #pragma unroll

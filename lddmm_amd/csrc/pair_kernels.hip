@@ -165,8 +165,11 @@ PairSplit plan_single(int ni, int nj, int target_blocks) { return make_split_ti(
 
 namespace {
 std::atomic<bool> g_pair_culling{true}, g_pair_expansion{true};
+std::atomic<unsigned long long *> g_pair_counter{nullptr};
 }
 
+void set_pair_counter(unsigned long long *dev_counter) { g_pair_counter.store(dev_counter); }
+unsigned long long *pair_counter() { return g_pair_counter.load(); }
 void set_pair_culling(bool on) { g_pair_culling.store(on); }
 bool pair_culling() { return g_pair_culling.load(); }
 void set_pair_expansion(bool on) { g_pair_expansion.store(on); }
@@ -190,7 +193,8 @@ void launch_kv(int dim, int order, int prec, const double *qi, int ni, const dou
                             (real)iw2,
                             cull_r2(pair_culling()),
                             expand_r2(pair_expansion()),
-                            Op::KF::kappa(iw, iw2)};
+                            Op::KF::kappa(iw, iw2),
+                            pair_counter()};
         launch_pairs<Op>(A, ni, nj, sp, 0, part, st);
     });
 }
@@ -224,7 +228,7 @@ void launch_bwd_pairs(int dim, int order, int prec, const double *q, const doubl
         using real = typename Op::real;
         const double iw = 1.0 / width, iw2 = 1.0 / (width * width), kap = Op::KF::kappa(iw, iw2);
         typename Op::Args A{q,   a,   p, n, (real)iw, (real)iw2, cull_r2(pair_culling()), expand_r2(pair_expansion()),
-                            kap, Op::KF::cmul(iw, iw2) / kap};
+                            kap, Op::KF::cmul(iw, iw2) / kap, pair_counter()};
         launch_pairs<Op>(A, n, n, sp, 0, part, st);
     });
 }

@@ -31,6 +31,7 @@ struct OpKV {
     using crd_t = real;
     static constexpr bool kCull = P::fp32 && ORDER == -1;
     static constexpr bool kExpand = kCull;   // expanded form (pair_driver.hpp)
+    static constexpr int kCountSlot = 0;     // executed-pair counter slot
     using KF = KernelFnT<ORDER, real, D_>;
     static constexpr bool kAnchor = KF::kScaled;   // scaled offsets (pair_driver.hpp)
     struct Args {
@@ -40,6 +41,7 @@ struct OpKV {
         float cut2;   // culling range (cull_r2); +inf: none
         float xr2;    // expanded-form radius bound (expand_r2)
         double kap;   // KF::kappa
+        unsigned long long *exec = nullptr;   // executed-pair count (pair_counter), or none
     };
     __device__ __forceinline__ static double crd_scale(const Args &A) { return A.kap; }
     struct Tgt {
@@ -118,6 +120,7 @@ struct OpBwd {
     using crd_t = real;
     static constexpr bool kCull = P::fp32 && ORDER == -1;
     static constexpr bool kExpand = kCull;   // expanded form (pair_driver.hpp)
+    static constexpr int kCountSlot = 1;     // executed-pair counter slot
     using KF = KernelFnT<ORDER, real, D_>;
     static constexpr bool kAnchor = KF::kScaled;   // scaled offsets (pair_driver.hpp)
     struct Args {
@@ -127,6 +130,7 @@ struct OpBwd {
         float cut2;        // culling range (cull_r2); +inf: none
         float xr2;         // expanded-form radius bound (expand_r2)
         double kap, pds;   // KF::kappa; dq flush factor KF::cmul / kappa
+        unsigned long long *exec = nullptr;   // executed-pair count (pair_counter), or none
     };
     __device__ __forceinline__ static double crd_scale(const Args &A) { return A.kap; }
     struct Tgt {

@@ -237,10 +237,10 @@ void launch_vf_pairs(int dim, int prec, const VfParams &vp, const double *dcen, 
             const real ics2 = (real)inv_sq(vp.cen_width), ids2 = (real)inv_sq(vp.dir_width);
             const float cut2 = cull_r2(pair_culling()), xr2 = expand_r2(pair_expansion());
             const double kap = cen_kappa<Cc()>(inv_sq(vp.cen_width));
-            typename Op::Args dd{dcen, ddir, ne, dcen, ddir, dvol, ne, ics2, ids2, cut2, xr2, kap};
+            typename Op::Args dd{dcen, ddir, ne, dcen, ddir, dvol, ne, ics2, ids2, cut2, xr2, kap, pair_counter()};
             launch_pairs<Op>(dd, ne, ne, spd, 0, part, st);
             if (nt > 0) {
-                typename Op::Args dt{dcen, ddir, ne, tcen, tdir, tvol, nt, ics2, ids2, cut2, xr2, kap};
+                typename Op::Args dt{dcen, ddir, ne, tcen, tdir, tvol, nt, ics2, ids2, cut2, xr2, kap, pair_counter()};
                 launch_pairs<Op>(dt, ne, nt, spt, spd.splits, part, st);
             }
         };

@@ -96,6 +96,7 @@ struct OpVf {
     static constexpr bool kExpand = kCull;   // expanded form (pair_driver.hpp)
     // anchored centres scaled by cen_kappa in both precisions
     static constexpr bool kAnchor = true;
+    static constexpr int kCountSlot = 2;   // executed-pair counter slot
     struct Args {
         const double *ci, *ui;   // targets: deformed element centres / directions
         int ni;
@@ -105,6 +106,7 @@ struct OpVf {
         float cut2;   // culling range (cull_r2); +inf: none
         float xr2;    // expanded-form radius bound (expand_r2)
         double kap;   // cen_kappa of the staged offsets
+        unsigned long long *exec = nullptr;   // executed-pair count (pair_counter), or none
     };
     __device__ __forceinline__ static double crd_scale(const Args &A) { return A.kap; }
     struct Tgt {
