@@ -1,23 +1,34 @@
 """Benchmark of the LDDMM registration hot path on one MI355X.
 
-Workload (default, `--config sphere_cube`): 3D surface matching, source an
+`value` (the default `--config sphere_cube`): 3D surface matching, source an
 icosphere of radius 1 subdivided 5 times (10,242 vertices, 20,480
 triangles), target the surface of the cube [-0.8, 0.8]^3 with 58 x 58 cells
 per face (40,368 triangles), Matern nu = 3/2 deformation kernel of width
 0.3, varifold Gaussian width 0.15 x Binet, 20 Euler steps, matching weight
-100, all FP64.
+100, all FP64.  A step is one evaluation of the objective and its gradient
+(forward flow, varifold, adjoint sweep) with every input already resident in
+HBM; the metric is kernel pair-interactions per second: (T-1) N^2 forward +
+(T-1) N^2 fused backward + Ed^2 + Ed Et varifold pairs per evaluation (the
+constant target-target term is computed once per problem and excluded).
+The same evaluation under the FP32 policy is reported beside it, with its
+errors against FP64, and the roofline of the dominant kernel from HIP
+events on the stream it runs on.
 
-A step is one evaluation of the objective and its gradient (forward flow,
-varifold, adjoint sweep) with every input already resident in HBM.  The
-metric is kernel pair-interactions per second: (T-1) N^2 forward + (T-1) N^2
-fused backward + Ed^2 + Ed Et varifold pairs per evaluation (the constant
-target-target term is computed once per problem and excluded).
+`batch` (the registrations/s half of the metric): the full batched
+configuration, 256 problems of the uneven generator (vertex counts
+log-uniform in [500, 20000], seed 4, T = 21, L-BFGS memory 5).  The plain
+FP64 method registers them one after another (zero start, stop at a
+relative decrease below 1e-6 or 300 iterations) and sets each problem's
+energy; the batched FP32 policy from zero momentum and the batched FP32
+policy from the learned warm start (network trained offline by
+lddmm_amd.train_warmstart) register all of them until each reaches its
+plain energy within 1e-6, confirmed in FP64.  `--batch 32 --batch-nmax 5000`
+gives a short run.
 
-Besides the timed steps the run reports one full L-BFGS registration
-(300 iterations, memory 5, weak Wolfe 0 / 0.5) as registrations/s, the
-roofline of the dominant kernel from HIP events on the stream it runs on,
-and a CPU baseline: the FP64 C restatement of the reference (oracle/) timed
-on a bounded sample of the same workload on one host core.
+`cpu_baseline`: the reference's own Cpp/src kernels (compiled into
+oracle/_ref; the FP64 C restatement in oracle/ where that is absent) timed on
+a bounded sample on one host core and scaled to one full evaluation; the
+batch section carries the same for its 256 registrations.
 """
 import argparse
 import json
@@ -33,6 +44,8 @@ sys.path.insert(0, ROOT)
 import lddmm_amd  # noqa: E402
 from lddmm_amd import geometry as G  # noqa: E402
 
+T_START = time.perf_counter()
+
 FP64_PEAK = 78.6e12   # MI355X FP64 vector FLOP/s
 
 # Algorithmic FLOPs per pair interaction (FMA = 2, add/sub/mul = 1;
@@ -43,6 +56,8 @@ FLOPS_PER_PAIR = {
     ("bwd_pairs", 3, -1): 33, ("bwd_pairs", 3, 1): 35, ("bwd_pairs", 2, -1): 22,
     ("vf_pairs", 3, "GB"): 32, ("vf_pairs", 2, "GB"): 23,
 }
+# hardware transcendental instructions per pair of the FP32 kernels (v_exp_f32,
+# plus v_sqrt_f32 for the Matern distance)
 TRANS_PER_PAIR = {("fwd_pairs", -1): 1, ("fwd_pairs", 1): 2, ("bwd_pairs", -1): 1, ("bwd_pairs", 1): 2,
                   ("vf_pairs", "GB"): 1}
 
@@ -144,84 +159,142 @@ def cpu_baseline(w, ifo, pairs_per_eval):
                 seconds_per_eval=t_eval)
 
 
-def warm_start_leg(probs, targets, e_ref, B, B64, chunk, t_setup, itr, seed, nmin, nmax, train_count):
-    """The learned warm start on a batch whose plain energies are known: a
-    one-off training on `train_count` problems of the same generator (other
-    seeds) solved by the batched FP32 run, then the timed leg: feature
-    extraction, inference, the per-problem start scale selection, the
-    registration to the plain energies (at most 1.25 times the plain run's
-    iterations), and the fallback, a zero-momentum batched FP32 run for
-    every problem the warm-started run did not bring to its target.  B and
-    B64 are the batch in FP32 and FP64 (the latter re-evaluates results)."""
+def c4_cpu_baseline(probs, evals, pairs, sample=2):
+    """The reference's CPU path on the batched configuration, one core,
+    extrapolated from a bounded sample: the reference's own multiplyKernel,
+    dqKernel and varifold (oracle/_ref; the oracle where it was not built)
+    timed on the `sample` problems nearest the median size (one forward
+    product, one adjoint step and the varifold with gradient against a slice
+    of the target), giving a per-pair cost that is applied to every
+    problem's pair count times the evaluations its plain run took."""
+    import oracle
+    k = G.BATCH_KERNELS
+    T1 = k["time_num"] - 1
+    order = np.argsort([len(p["q0"]) for p in probs])
+    pick = order[len(order) // 2: len(order) // 2 + sample]
+    L = oracle.ref(3)
+    t_pts = t_vf = 0.0
+    n_pts = n_vf = 0
+    for i in pick:
+        p = probs[i]
+        q = p["q0"]
+        n = len(q)
+        a = 0.01 * G.normal(7, q.size).reshape(q.shape)
+        v, g = np.empty_like(q), np.empty_like(q)
+        t0 = time.perf_counter()
+        if L is not None:
+            L.ref_multiply_kernel(v, q, a, k["knl_order"], p["knl_width"], n, 3)
+            L.ref_dq_kernel(g, q, a, v, k["knl_order"], p["knl_width"], n, 3)
+            L.ref_multiply_kernel(v, q, a, k["knl_order"], p["knl_width"], n, 3)
+        else:
+            v = oracle.multiply_kernel(q, a, k["knl_order"], p["knl_width"])
+            oracle.dq_kernel(q, a, v, k["knl_order"], p["knl_width"])
+            oracle.multiply_kernel(q, a, k["knl_order"], p["knl_width"])
+        t_pts += time.perf_counter() - t0
+        n_pts += 3 * n * n
+        ifo = lddmm_amd.getElmIfo(p["elm"], n, one_based=False)
+        et = len(p["tgt_vol"])
+        sl = slice(0, max(1, et // 8))
+        t0 = time.perf_counter()
+        oracle.varifold(q, p["elm"], ifo, p["tgt_cen"][sl], p["tgt_dir"][sl], p["tgt_vol"][sl], k["cen_type"],
+                        k["cen_width"], k["dir_type"], k["dir_width"], **(dict(L=L, prefix="ref") if L else {}))
+        t_vf += time.perf_counter() - t0
+        ed = len(p["elm"])
+        n_vf += ed * ed + ed * sl.stop + sl.stop * sl.stop
+    per_pt, per_vf = t_pts / n_pts, t_vf / n_vf
+    total = 0.0
+    for p, e in zip(probs, evals):
+        n, ed, et = len(p["q0"]), len(p["elm"]), len(p["tgt_vol"])
+        total += e * (3 * T1 * n * n * per_pt + (ed * ed + ed * et + et * et) * per_vf)
+    return dict(value=len(probs) / total, unit="registrations/s", cores=1,
+                kind="reference" if L is not None else "port",
+                sample="the reference's Cpp/src multiplyKernel, dqKernel and varifold (oracle/_ref), 1 core, on "
+                       "%d median-size problems (%.1f s of CPU work): %.3g s per point pair, %.3g s per varifold "
+                       "pair, applied to every problem's pairs x the evaluations of its plain run = %.0f s for "
+                       "the %d registrations" % (len(pick), t_pts + t_vf, per_pt, per_vf, total, len(probs)),
+                seconds=total)
+
+
+def warm_start_leg(probs, targets, e_ref, B, B64, t_setup, itr, weights):
+    """The learned warm start on a batch whose plain energies are known,
+    with the network trained offline by lddmm_amd.train_warmstart (its
+    parameters and training record in `weights`).  Timed: the features
+    (including the start's varifold gradient), inference, the per-problem
+    start scale (choose_start: the multiple of the prediction, zero included,
+    with the lowest objective), the batched FP32 registration to the plain
+    energies (at most 1.25 times the plain iteration limit) and the
+    fallback, a zero-momentum batched FP32 run for every problem the
+    warm-started run did not bring to its target.  B and B64 are the batch in
+    FP32 and FP64 (the latter re-evaluates results)."""
     from lddmm_amd import warmstart as W
     k = G.BATCH_KERNELS
     T = k["time_num"]
-
-    def make(ps, chunk_):
-        return lddmm_amd.Batch(ps, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
-                               k["knl_order"], k["time_stp"], T, precision="fp32", chunk=chunk_)
-
+    ws = W.WarmStart.load(weights)
     t0 = time.perf_counter()
-    train = G.batch_problems(train_count, seed + 1000, nmin, nmax)
-    Bt = make(train, 0)
-    sol, _, _ = Bt.run(itr, 0.0, 0.0, 0.5, 5, precision="fp32", rel_energy_tol=1e-6)
-    Bt.close()
-    t_solve = time.perf_counter() - t0
-    ws = W.WarmStart(hidden=128, layers=4, seed=1)
-    t0 = time.perf_counter()
-    losses = ws.fit(train, sol, k["knl_order"], k["time_stp"], T, steps=3000, batch_rows=4096, lr=1e-3)
-    t_fit = time.perf_counter() - t0
-    # timed: inference, start selection, registration, fallback
-    t0 = time.perf_counter()
-    warm = [ws.predict(p, k["knl_order"], k["time_stp"], T) for p in probs]
+    warm = [ws.predict(p, T) for p in probs]
+    t_inf = time.perf_counter() - t0
     starts, scale = W.choose_start(B, warm)
-    # the warm-started run gets 1.25 times the plain run's iteration budget;
-    # a problem that has not reached its target by then falls back
     aw, iw, rw = B.run(int(1.25 * itr), 0.0, 0.0, 0.5, 5, alp0=starts, precision="fp32", targets=targets)
     missed = [i for i, r in enumerate(iw) if not r["reached_target"]]
     its_fb = []
     if missed:
-        Bf = make([probs[i] for i in missed], chunk)
+        Bf = lddmm_amd.Batch([probs[i] for i in missed], k["cen_type"], k["cen_width"], k["dir_type"],
+                             k["dir_width"], k["dis_wgt"], k["knl_order"], k["time_stp"], T, precision="fp32")
         af, ifb, _ = Bf.run(3 * itr, 0.0, 0.0, 0.5, 5, precision="fp32", targets=targets[missed])
         Bf.close()
         for j, i in enumerate(missed):
             aw[i] = af[j]
         its_fb = [r["iterations"] for r in ifb]
     t_warm = time.perf_counter() - t0 + t_setup
+    ws.close()
     f, _ = B64.objgrd(aw)
     rel_w = (f - e_ref) / np.abs(e_ref)
     count = len(probs)
-    return dict(seconds=t_warm, registrations_per_s=count / t_warm, rounds=rw,
+    m = ws.meta
+    return dict(seconds=t_warm, registrations_per_s=count / t_warm, rounds=rw, inference_seconds=t_inf,
                 median_iterations=float(np.median([r["iterations"] for r in iw])),
                 warm_start_used=int(np.sum(scale > 0)), reached_target=count - len(missed),
                 fallback_problems=len(missed), fallback_iterations=its_fb,
                 within_1e6=int(np.sum(rel_w <= 1e-6 + 1e-8)), max_rel_energy_vs_plain=float(rel_w.max()),
-                training=dict(problems=train_count, solve_seconds=t_solve, fit_seconds=t_fit, adam_steps=3000,
-                              loss_first=float(losses[0]), loss_last=float(np.mean(losses[-20:]))))
+                training=dict(problems=int(m.get("problems", 0)), rows=int(m.get("rows", 0)),
+                              solver="plain FP64 L-BFGS (batched evaluation), other seeds than the benchmark's",
+                              solve_seconds=m.get("solve_seconds"), fit_seconds=m.get("fit_seconds"),
+                              adam_steps=int(m.get("steps", 0)), loss_first=m.get("loss_first"),
+                              loss_last=m.get("loss_last"), weights=os.path.relpath(weights, ROOT)))
 
 
-def batch_bench(count, nmin, nmax, itr, seed=4, train_count=64):
-    """Uneven batch (the batched-registration workload, scaled by `count`
-    and the size range): plain FP64 registrations one after another
-    (zero initial momentum, stop at a relative energy decrease below 1e-6 or
-    `itr` iterations) against the batched FP64 run, the batched FP32
-    policy run to each plain run's final energy within 1e-6 relative, and
-    the batched FP32 policy started from the learned warm start.  Batched
-    times include the batch's set-up, as plain times include each run's.
-    The warm-start protocol is warm_start_leg's (training reported as a
-    one-off cost)."""
+WARM_WEIGHTS = os.path.join(ROOT, "lddmm_amd", "data", "warmstart_c4.npz")
+
+
+def batch_bench(count, nmin, nmax, itr, seed=4, warm=True, batched_fp64=False, budget_s=None, t_start=None):
+    """The batched-registration configuration: `count` problems of the
+    uneven generator (vertex counts log-uniform in [nmin, nmax]; 256 in
+    [500, 20000] is the full configuration).  The plain FP64 method
+    registers them one after another (zero start, stop at a relative energy
+    decrease below 1e-6 or `itr` iterations) and sets each problem's energy;
+    the batched FP32 policy then registers all of them until each reaches
+    its plain energy x (1 + 1e-6) (at most 3 `itr` iterations, finishing in
+    FP64 on a stall), and so does the batched FP32 policy from the learned
+    warm start.  Every result is re-evaluated in FP64 against the plain
+    energy.  Batched times include the batch's set-up, as plain times
+    include each run's.  The warm-start leg is skipped (and says so) when
+    `budget_s` seconds since `t_start` have passed before it starts."""
     probs = G.batch_problems(count, seed, nmin, nmax)
     k = G.BATCH_KERNELS
     T = k["time_num"]
-    seq = []
+    seq, a_plain = [], []
     t0 = time.perf_counter()
-    for p in probs:
+    for i, p in enumerate(probs):
         ifo = lddmm_amd.getElmIfo(p["elm"], len(p["q0"]), one_based=False)
-        _, info = lddmm_amd.run(p["q0"], None, p["elm"], ifo, p["tgt_cen"], p["tgt_vol"], p["tgt_dir"],
+        a, info = lddmm_amd.run(p["q0"], None, p["elm"], ifo, p["tgt_cen"], p["tgt_vol"], p["tgt_dir"],
                                 k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
                                 k["knl_order"], p["knl_width"], k["time_stp"], k["time_num"], itr, 0.0, 0.0, 0.5, 5,
                                 rel_energy_tol=1e-6)
         seq.append(info)
+        a_plain.append(a)
+        if i % 16 == 15:
+            print("plain FP64: %d of %d problems, %.1f s" % (i + 1, count, time.perf_counter() - t0),
+                  file=sys.stderr, flush=True)
     t_seq = time.perf_counter() - t0
     e_ref = np.array([i["fcn"] for i in seq])
     targets = e_ref * (1.0 + 1e-6)
@@ -236,36 +309,54 @@ def batch_bench(count, nmin, nmax, itr, seed=4, train_count=64):
     B = make(probs, "fp32")
     chunk, pairs = B.info()
     t_setup = time.perf_counter() - t0
-    a64, i64, r64 = B.run(itr, 0.0, 0.0, 0.5, 5, precision="fp64", rel_energy_tol=1e-6)
-    a32, i32, r32 = B.run(3 * itr, 0.0, 0.0, 0.5, 5, precision="fp32", targets=targets)
+    a32, i32, r32 = B.run(3 * itr, 0.0, 0.0, 0.5, 5, precision="fp32", targets=targets, progress=100)
+    t32 = i32[0]["seconds"] + t_setup
+    print("batched FP32: %.1f s" % t32, file=sys.stderr, flush=True)
     B64 = make(probs, "fp64", chunk)
-
-    def rel_to_plain(alps):
-        f, _ = B64.objgrd(alps)
-        return (f - e_ref) / np.abs(e_ref)
-
-    rel = rel_to_plain(a32)
-    t64, t32 = i64[0]["seconds"] + t_setup, i32[0]["seconds"] + t_setup
-
-    wleg = warm_start_leg(probs, targets, e_ref, B, B64, chunk, t_setup, itr, seed, nmin, nmax, train_count)
-    wleg["median_iterations_plain"] = float(np.median([i["iterations"] for i in seq]))
-    t_warm = wleg["seconds"]
+    f64, _ = B64.objgrd(a32)
+    rel = (f64 - e_ref) / np.abs(e_ref)
+    f_plain, _ = B64.objgrd(a_plain)   # consistency of the two FP64 evaluations
+    out = dict(problems=count, n_range=[int(min(B.n)), int(max(B.n))], chunk=chunk,
+               pairs_per_eval_total=float(pairs.sum()), batch_setup_seconds=t_setup,
+               plain_fp64_sequential=dict(seconds=t_seq, registrations_per_s=count / t_seq,
+                                          median_iterations=float(np.median([i["iterations"] for i in seq])),
+                                          evaluations=int(sum(i["evaluations"] for i in seq))),
+               batched_fp32=dict(seconds=t32, registrations_per_s=count / t32, rounds=r32,
+                                 median_iterations=float(np.median([r["iterations"] for r in i32])),
+                                 max_iterations=int(max(r["iterations"] for r in i32)),
+                                 reached_target=int(sum(r["reached_target"] for r in i32)),
+                                 within_1e6=int(np.sum(rel <= 1e-6 + 1e-8)),
+                                 max_rel_energy_vs_plain=float(rel.max()),
+                                 missed=[dict(problem=i, n=int(B.n[i]), plain_energy=float(e_ref[i]),
+                                              fp32_energy_fp64=float(f64[i]), iterations=int(i32[i]["iterations"]),
+                                              status=int(i32[i]["status"]))
+                                         for i in range(count) if not i32[i]["reached_target"]]),
+               plain_energy_check=dict(max_rel_diff_batched_fp64=float(np.max(np.abs(f_plain - e_ref) /
+                                                                             np.abs(e_ref)))),
+               speedup_batched_fp32_vs_plain_fp64=t_seq / t32)
+    if batched_fp64:
+        a64, i64, r64 = B.run(itr, 0.0, 0.0, 0.5, 5, precision="fp64", rel_energy_tol=1e-6)
+        t64 = i64[0]["seconds"] + t_setup
+        out["batched_fp64"] = dict(seconds=t64, registrations_per_s=count / t64, rounds=r64,
+                                   median_iterations=float(np.median([i["iterations"] for i in i64])))
+    if warm:
+        spent = time.perf_counter() - t_start if t_start is not None else 0.0
+        if not os.path.exists(WARM_WEIGHTS):
+            out["batched_fp32_warm_start"] = dict(skipped="no trained weights at " +
+                                                  os.path.relpath(WARM_WEIGHTS, ROOT))
+        elif budget_s is not None and spent > budget_s:
+            out["batched_fp32_warm_start"] = dict(skipped="time budget: %.0f s spent before the leg" % spent)
+        else:
+            wleg = warm_start_leg(probs, targets, e_ref, B, B64, t_setup, itr, WARM_WEIGHTS)
+            wleg["median_iterations_plain"] = float(np.median([i["iterations"] for i in seq]))
+            out["batched_fp32_warm_start"] = wleg
+            out["speedup_batched_fp32_warm_vs_plain_fp64"] = t_seq / wleg["seconds"]
+            out["warm_vs_zero_start"] = t32 / wleg["seconds"]
+    out["cpu_baseline"] = c4_cpu_baseline(probs, [i["evaluations"] for i in seq], pairs)
+    out["speedup_plain_fp64_gpu_vs_cpu"] = out["cpu_baseline"]["seconds"] / t_seq
     B.close()
     B64.close()
-    return dict(problems=count, n_range=[int(min(B.n)), int(max(B.n))], chunk=chunk,
-                pairs_per_eval_total=float(pairs.sum()), batch_setup_seconds=t_setup,
-                plain_fp64_sequential=dict(seconds=t_seq, registrations_per_s=count / t_seq,
-                                           median_iterations=float(np.median([i["iterations"] for i in seq]))),
-                batched_fp64=dict(seconds=t64, registrations_per_s=count / t64, rounds=r64,
-                                  median_iterations=float(np.median([i["iterations"] for i in i64]))),
-                batched_fp32=dict(seconds=t32, registrations_per_s=count / t32, rounds=r32,
-                                  median_iterations=float(np.median([i["iterations"] for i in i32])),
-                                  reached_target=int(sum(i["reached_target"] for i in i32)),
-                                  within_1e6=int(np.sum(rel <= 1e-6 + 1e-8)),
-                                  max_rel_energy_vs_plain=float(rel.max())),
-                batched_fp32_warm_start=wleg,
-                speedup_batched_fp32_vs_plain_fp64=t_seq / t32,
-                speedup_batched_fp32_warm_vs_plain_fp64=t_seq / t_warm)
+    return out
 
 
 def culling_report(count=32, seed=4, nmin=500, nmax=5000, reps=5):
@@ -364,93 +455,28 @@ def precision_report(freq=100, seed=31, itr=10):
                           final_shape_rms_dist=float(np.sqrt(np.mean(np.sum((qr32 - qr64) ** 2, axis=1))))))
 
 
-def c4_report(count=256, nmin=500, nmax=20000, itr=300, seed=4, warm=False, train_count=64):
-    """The batched-registration configuration at full size: `count` problems,
-    vertex counts log-uniform in [nmin, nmax].  The plain FP64 method
-    registers them one after another (zero start, stop at a relative energy
-    decrease below 1e-6 or `itr` iterations) and sets each problem's energy;
-    the batched FP32 policy then registers all of them in one run until each
-    reaches its plain energy x (1 + 1e-6) (at most 3 x `itr` iterations,
-    finishing in FP64 on a stall), and its results are re-evaluated in FP64.
-    With `warm`, the same batch is also registered from the learned warm
-    start (warm_start_leg, trained on `train_count` problems of the
-    generator's other seeds)."""
-    probs = G.batch_problems(count, seed, nmin, nmax)
-    k = G.BATCH_KERNELS
-    seq, a_plain = [], []
-    t0 = time.perf_counter()
-    for i, p in enumerate(probs):
-        ifo = lddmm_amd.getElmIfo(p["elm"], len(p["q0"]), one_based=False)
-        a, info = lddmm_amd.run(p["q0"], None, p["elm"], ifo, p["tgt_cen"], p["tgt_vol"], p["tgt_dir"],
-                                k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
-                                k["knl_order"], p["knl_width"], k["time_stp"], k["time_num"], itr, 0.0, 0.0, 0.5, 5,
-                                rel_energy_tol=1e-6)
-        seq.append(info)
-        a_plain.append(a)
-        if i % 16 == 15:
-            print("plain FP64: %d of %d problems, %.1f s" % (i + 1, count, time.perf_counter() - t0),
-                  file=sys.stderr, flush=True)
-    t_seq = time.perf_counter() - t0
-    e_ref = np.array([i["fcn"] for i in seq])
-    t0 = time.perf_counter()   # set-up counts toward the batched time (see batch_report)
-    B = lddmm_amd.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
-                        k["knl_order"], k["time_stp"], k["time_num"], precision="fp32", chunk=0)
-    chunk, pairs = B.info()
-    t_setup = time.perf_counter() - t0
-    a32, i32, r32 = B.run(3 * itr, 0.0, 0.0, 0.5, 5, precision="fp32", targets=e_ref * (1.0 + 1e-6), progress=25)
-    t32 = i32[0]["seconds"] + t_setup
-    B64 = lddmm_amd.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
-                          k["knl_order"], k["time_stp"], k["time_num"], precision="fp64", chunk=chunk)
-    f64, _ = B64.objgrd(a32)
-    # consistency: the plain runs' energies re-evaluated by the batched FP64 objective
-    f_plain, _ = B64.objgrd(a_plain)
-    wleg = None
-    if warm:
-        wleg = warm_start_leg(probs, e_ref * (1.0 + 1e-6), e_ref, B, B64, chunk, t_setup, itr, seed, nmin, nmax,
-                              train_count)
-        wleg["speedup_vs_plain_fp64"] = t_seq / wleg["seconds"]
-    B.close()
-    B64.close()
-    rel = (f64 - e_ref) / np.abs(e_ref)
-    missed = [dict(problem=i, n=int(B.n[i]), knl_width=float(probs[i]["knl_width"]), plain_energy=float(e_ref[i]),
-                   plain_iterations=int(seq[i]["iterations"]), plain_status=int(seq[i]["status"]),
-                   fp32_energy_fp64=float(f64[i]), fp32_iterations=int(i32[i]["iterations"]),
-                   fp32_status=int(i32[i]["status"]), fp32_phase_iterations=int(i32[i]["fp32_iterations"]))
-              for i in range(count) if not i32[i]["reached_target"]]
-    return dict(problems=count, n_range=[int(min(B.n)), int(max(B.n))], chunk=chunk,
-                pairs_per_eval_total=float(pairs.sum()), batch_setup_seconds=t_setup,
-                plain_fp64_sequential=dict(seconds=t_seq, registrations_per_s=count / t_seq,
-                                           median_iterations=float(np.median([i["iterations"] for i in seq]))),
-                batched_fp32=dict(seconds=t32, registrations_per_s=count / t32, rounds=r32,
-                                  median_iterations=float(np.median([r["iterations"] for r in i32])),
-                                  max_iterations=int(max(r["iterations"] for r in i32)),
-                                  reached_target=int(sum(r["reached_target"] for r in i32)),
-                                  within_1e6=int(np.sum(rel <= 1e-6 + 1e-8)),
-                                  max_rel_energy_vs_plain=float(rel.max()), missed=missed),
-                plain_energy_check=dict(max_rel_diff_batched_fp64=float(np.max(np.abs(f_plain - e_ref) / np.abs(e_ref)))),
-                speedup_batched_fp32_vs_plain_fp64=t_seq / t32,
-                **({} if wleg is None else dict(batched_fp32_warm_start=wleg)))
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="sphere_cube")
-    ap.add_argument("--no-registration", action="store_true")
+    ap.add_argument("--registration", action="store_true",
+                    help="also one full L-BFGS registration of the workload (FP64 and the FP32 policy)")
+    ap.add_argument("--no-registration", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--reg-iters", type=int, default=0, help="L-BFGS iterations of the registration (0 = config)")
-    ap.add_argument("--batch", type=int, default=32, help="problems of the uneven-batch leg (0 = skip)")
+    ap.add_argument("--batch", type=int, default=256,
+                    help="problems of the batched-registration leg (256: the full configuration; 0 = skip)")
     ap.add_argument("--batch-nmin", type=int, default=500)
-    ap.add_argument("--batch-nmax", type=int, default=5000)
+    ap.add_argument("--batch-nmax", type=int, default=20000)
     ap.add_argument("--batch-chunk", type=int, default=512, help="source chunk of --batch-profile")
+    ap.add_argument("--batched-fp64", action="store_true", help="also the batched FP64 run of the batch leg")
+    ap.add_argument("--no-warm-start", action="store_true", help="skip the learned warm start of the batch leg")
+    ap.add_argument("--budget", type=float, default=470.0,
+                    help="seconds after which the warm-start leg is skipped rather than started")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision-report", action="store_true",
                     help="only the FP32-vs-FP64 report on the 100k-vertex superquadric pair")
-    ap.add_argument("--warm-start", action="store_true",
-                    help="with --c4-full: also register the batch from the learned warm start")
-    ap.add_argument("--c4-full", action="store_true",
-                    help="only the full-size batched configuration (256 problems, N up to 20,000)")
     ap.add_argument("--batch-profile", action="store_true",
                     help="only the batched FP32 registration of the uneven batch (for kernel traces)")
     ap.add_argument("--culling-report", action="store_true",
@@ -463,9 +489,6 @@ def main():
 
     if args.precision_report:
         print(json.dumps(precision_report()), flush=True)
-        return
-    if args.c4_full:
-        print(json.dumps(c4_report(warm=args.warm_start)), flush=True)
         return
     if args.culling_report:
         print(json.dumps(culling_report()), flush=True)
@@ -591,30 +614,54 @@ def main():
     obj.synchronize()
     lddmm_amd.profile_enable(False)
     p32 = lddmm_amd.profile_read()
+    # pairs the culling kernels compute (the rest are skipped below 2^-48):
+    # one counted evaluation; rates and roofline use computed pairs only
+    lddmm_amd.pair_count_begin()
+    obj.eval_device(d_f.ptr, d_grd.ptr, d_alp.ptr)
+    counted = lddmm_amd.pair_count_end()
+    launches = {"fwd_pairs": T - 1, "bwd_pairs": T - 1, "vf_pairs": 1}
     k32 = {}
     for k, (np_, _) in per_launch.items():
         ms, cnt = p32.get(k, (0.0, 0))
         if cnt:
-            k32[k] = dict(avg_ms=ms / cnt, pairs_per_s=np_ / (ms / cnt * 1e-3))
+            ex = counted[k] / launches[k] if counted[k] else np_   # not culled: every pair computed
+            k32[k] = dict(avg_ms=ms / cnt, pairs_per_s=np_ / (ms / cnt * 1e-3), executed_pairs_per_launch=ex,
+                          executed_fraction=ex / np_, executed_pairs_per_s=ex / (ms / cnt * 1e-3))
     dom32 = max(k32, key=lambda k: k32[k]["avg_ms"] * p32[k][1]) if k32 else None
     rp32 = ROCPROF_SPHERE_CUBE.get((dom32, "f32")) if args.config == "sphere_cube" else None
     roof32 = None
     if dom32 and per_launch[dom32][1]:
-        t32 = per_launch[dom32][0] * per_launch[dom32][1] / (k32[dom32]["avg_ms"] * 1e-3) / 1e12
+        t32 = k32[dom32]["executed_pairs_per_launch"] * per_launch[dom32][1] / (k32[dom32]["avg_ms"] * 1e-3) / 1e12
         roof32 = dict(bound="valu", kernel=dom32, achieved=t32, peak=FP64_PEAK / 1e12, unit="TFLOP/s",
                       frac=t32 / (FP64_PEAK / 1e12), traffic=rp32["traffic"] if rp32 else None,
                       valu_busy_rocprof=rp32["valu_busy"] if rp32 else None,
                       valu_issue_rocprof=rp32["valu_issue"] if rp32 else None,
                       frac_vs_packed_fp32_peak=t32 / 157.3,
-                      note="FP32 VALU peak without packed FP32 = 78.6 TFLOP/s (this build compiles all HIP "
-                           "code without packed FP32 instructions; 157.3 TFLOP/s needs them)")
+                      note="computed pairs only (executed_pairs_per_launch); FP32 VALU peak without packed "
+                           "FP32 = 78.6 TFLOP/s (this build compiles all HIP code without packed FP32 "
+                           "instructions; 157.3 TFLOP/s needs them)")
+    # transcendental utilisation of the FP32 pair kernels: computed pairs x
+    # exp/sqrt per pair against the measured v_exp_f32 rate of this chip
+    rates = lddmm_amd.measure_rates()
+    out["measured_rates"] = dict(rates, note="lddmm_measure_rates: v_exp_f32 and FMA throughput of 4,096 blocks x "
+                                            "256 threads x 8 independent chains (rates.hip)")
+    for k, kd32 in k32.items():
+        tp = TRANS_PER_PAIR.get((k, order if k != "vf_pairs" else vfkey))
+        if tp:
+            kd32["trans_per_pair"] = tp
+            kd32["trans_utilisation"] = kd32["executed_pairs_per_s"] * tp / rates["exp2_f32"]
+    if roof32:
+        roof32["valu_peak_measured_tflops"] = 2 * rates["fma_f32"] / 1e12
+        roof32["frac_vs_measured_peak"] = roof32["achieved"] / (2 * rates["fma_f32"] / 1e12)
+    roofline["valu_peak_measured_tflops"] = 2 * rates["fma_f64"] / 1e12
+    roofline["frac_vs_measured_peak"] = (roofline["achieved"] or 0.0) / (2 * rates["fma_f64"] / 1e12)
     out["fp32_policy"] = dict(value=pairs * args.steps / el32, ms_per_step=1e3 * el32 / args.steps,
                               speedup_vs_fp64=elapsed / el32, energy_rel_err=abs(f32 - f64) / abs(f64),
                               grad_rel_err=float(np.max(np.abs(g32 - g64)) / np.max(np.abs(g64))), kernels=k32,
                               roofline=roof32)
     obj.set_precision("fp64")
 
-    if not args.no_registration:
+    if args.registration:
         # Protocol of the registrations/s metric: the plain FP64 run (the
         # config's iteration count) sets E_ref; the FP32 policy runs until
         # its energy is within 1e-6 of E_ref (finishing in FP64 if its FP32
@@ -639,7 +686,8 @@ def main():
         out["registration"] = regs
 
     if args.batch > 0:
-        out["batch"] = batch_bench(args.batch, args.batch_nmin, args.batch_nmax, 300)
+        out["batch"] = batch_bench(args.batch, args.batch_nmin, args.batch_nmax, 300, warm=not args.no_warm_start,
+                                   batched_fp64=args.batched_fp64, budget_s=args.budget, t_start=T_START)
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, ifo, pairs)
