@@ -156,6 +156,20 @@ int lddmm_multiply_kernel(double *vlcMat, const double *lmkMat, const double *al
 int lddmm_multiply_kernel_cross(double *vlcMat, const double *lmkiMat, const double *lmkjMat,
                                 const double *alpMat, int knlOrder, double knlWidth,
                                 int lmkiNum, int lmkjNum, int dimNum);
+/* No counterpart in the reference.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): lddmm_multiply_kernel_cross on device arrays in the CUDA layout
+ * (N x D column-major: x[0..N) y[0..N) [z[0..N)]), on `stream` (a hipStream_t; NULL
+ * is the null stream), asynchronous.  The per-split partial sums live in a fixed
+ * 16 MiB device array per GPU (2^21 doubles) that the library never frees; sizes
+ * whose partials exceed it (splits x dimNum x lmkiNum doubles: at most 7.9e5 below
+ * 131,072 targets, dimNum x lmkiNum above, so up to 699,050 targets in 3D) return -1
+ * with the need in lddmm_last_error().  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (other streams or threads) race.  Results equal
+ * lddmm_multiply_kernel_cross's bit for bit. */
+int lddmm_multiply_kernel_cross_noalloc(double *d_vlcMat, const double *d_lmkiMat, const double *d_lmkjMat,
+                                        const double *d_alpMat, int knlOrder, double knlWidth, int lmkiNum,
+                                        int lmkjNum, int dimNum, void *stream);
 /* dqKernel (Cpp/src/dqKernel.cpp:31, 307; MATLAB/src/dqKernelITF.cpp:16-72):
  * gradient of lft^T K(q) rgt with respect to q; cross form gives both sides. */
 int lddmm_dq_kernel(double *dqKMat, const double *lmkMat, const double *lftMat, const double *rgtMat,

@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "liblddmm_amd.so")
 ARCH = os.environ.get("LDDMM_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["pair_kernels.hip", "varifold_kernels.hip", "vector_kernels.hip", "batch.hip", "mlp.hip",
-           "rates.hip", "engine.cpp", "capi.cpp"]
+           "noalloc.hip", "rates.hip", "engine.cpp", "capi.cpp"]
 
 # -packed-fp32-ops: keep the compiler from forming v_pk_{add,mul,fma}_f32
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",

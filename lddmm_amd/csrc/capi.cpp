@@ -843,6 +843,35 @@ int lddmm_multiply_kernel_cross(double *vlcMat, const double *lmkiMat, const dou
     });
 }
 
+int lddmm_multiply_kernel_cross_noalloc(double *d_vlcMat, const double *d_lmkiMat, const double *d_lmkjMat,
+                                        const double *d_alpMat, int knlOrder, double knlWidth, int lmkiNum,
+                                        int lmkjNum, int dimNum, void *stream)
+{
+    return guarded([&] {
+        require(d_vlcMat && d_lmkiMat && d_lmkjMat && d_alpMat, "null argument");
+        require(dimNum == 2 || dimNum == 3, "dimNum must be 2 or 3");
+        require(lmkiNum >= 0 && lmkjNum >= 0, "negative size");
+        require(knlWidth > 0.0, "knlWidth must be positive");
+        if (lmkiNum == 0) return 0;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        double *scr = noalloc_scratch();
+        if (lmkjNum == 0) {   // no sources: zero splits sum to zeros
+            launch_sum_splits(dimNum, 0, lmkiNum, scr, 1.0, d_vlcMat, st);
+            return 0;
+        }
+        // the plan of lddmm_multiply_kernel_cross: the same kernels, the same sums
+        const PairSplit sp = plan_kv(lmkiNum, lmkjNum, kFP64, 2048);
+        const size_t need = (size_t)sp.splits * dimNum * lmkiNum;
+        if (need > kNoAllocScratchDoubles)
+            throw Error(kErrArg, "lddmm_multiply_kernel_cross_noalloc: these sizes need " + std::to_string(need) +
+                                     " doubles of scratch, more than its " +
+                                     std::to_string(kNoAllocScratchDoubles));
+        launch_kv(dimNum, knlOrder, kFP64, d_lmkiMat, lmkiNum, d_lmkjMat, d_alpMat, lmkjNum, knlWidth, sp, scr, st);
+        launch_sum_splits(dimNum, sp.splits, lmkiNum, scr, 1.0, d_vlcMat, st);
+        return 0;
+    });
+}
+
 int lddmm_multiply_kernel(double *vlcMat, const double *lmkMat, const double *alpMat, int knlOrder, double knlWidth,
                           int lmkNum, int dimNum)
 {

@@ -113,4 +113,10 @@ void launch_permute(const double *src, double *dst, const int *perm, int n, int 
 // out[1] FP32 FMAs per second, out[2] FP64 FMAs per second, whole chip
 void measure_rates(double *out, hipStream_t st);
 
+// ---- fixed scratch of the entry points that allocate nothing (noalloc.hip) ----
+// 2 Mi doubles (16 MiB) per GPU, held from module load to process end
+constexpr size_t kNoAllocScratchDoubles = size_t(1) << 21;
+// device address of the current device's scratch (looked up once per device)
+double *noalloc_scratch();
+
 }  // namespace lddmm
