@@ -63,16 +63,16 @@ TRANS_PER_PAIR = {("fwd_pairs", -1): 1, ("fwd_pairs", 1): 2, ("bwd_pairs", -1): 
 
 # Per-launch counter figures of the pair kernels on the sphere->cube workload, from
 # profiles/pair_kernels_sphere_cube.md (rocprofv3 PMC passes on this build):
-# traffic = HBM bytes per launch (FETCH_SIZE x 1.9921 + WRITE_SIZE, calibrated on a
-# copy kernel of known size), valu_busy = AMD's VALUBusy, valu_issue = share of SIMD
+# traffic = HBM bytes per launch (FETCH_SIZE x 1.9960 + WRITE_SIZE, calibrated on a
+# vector kernel of known traffic, k_try), valu_busy = AMD's VALUBusy, valu_issue = share of SIMD
 # issue slots at 4 cycles per wave64 VALU instruction.
 ROCPROF_SPHERE_CUBE = {
-    ("fwd_pairs", "f64"): dict(traffic=8.064e6, valu_busy=0.83, valu_issue=0.80),
-    ("bwd_pairs", "f64"): dict(traffic=1.726e7, valu_busy=0.83, valu_issue=0.81),
-    ("vf_pairs", "f64"): dict(traffic=2.668e7, valu_busy=0.84, valu_issue=0.87),
-    ("fwd_pairs", "f32"): dict(traffic=1.239e7, valu_busy=0.89, valu_issue=0.81),
-    ("bwd_pairs", "f32"): dict(traffic=1.725e7, valu_busy=1.02, valu_issue=0.97),
-    ("vf_pairs", "f32"): dict(traffic=2.670e7, valu_busy=0.71, valu_issue=0.75),
+    ("fwd_pairs", "f64"): dict(traffic=8.07e6, valu_busy=0.81, valu_issue=0.77),
+    ("bwd_pairs", "f64"): dict(traffic=1.728e7, valu_busy=0.80, valu_issue=0.78),
+    ("vf_pairs", "f64"): dict(traffic=2.671e7, valu_busy=0.83, valu_issue=0.86),
+    ("fwd_pairs", "f32"): dict(traffic=1.237e7, valu_busy=0.89, valu_issue=0.81),
+    ("bwd_pairs", "f32"): dict(traffic=1.727e7, valu_busy=1.02, valu_issue=0.97),
+    ("vf_pairs", "f32"): dict(traffic=2.678e7, valu_busy=0.69, valu_issue=0.67),
 }
 
 
