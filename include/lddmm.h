@@ -301,6 +301,16 @@ int lddmm_spatial_order(int32_t *perm, const double *ptsMat, int ptsNum, int dim
  * checking).
  * Process-wide, read at every launch. */
 int lddmm_set_pair_culling(int on);
+/* The culling level (no counterpart in the reference): tile pairs whose
+ * Gaussian values are all below 2^-bits are skipped, bits in 24..126, 48 by
+ * default.  Process-wide, read at every launch. */
+int lddmm_set_pair_cull_level(int bits);
+/* Culling by group (no counterpart in the reference): the culling kernels
+ * test precomputed bounds of aligned 64-source groups and stage only the
+ * groups in range, four to a tile (on by default); off tests each staged
+ * quarter tile instead.  The FP32 sums are the same per group, so results
+ * agree to FP64 rounding.  Process-wide, read at every launch. */
+int lddmm_set_pair_groups(int on);
 /* Expanded form of the FP32 Gaussian pair terms (no counterpart in the
  * reference): a pair forms r^2 - |x_i|^2 = |x_j|^2 - 2 x_i.x_j with D
  * multiply-adds and the factor exp2(-|x_i|^2) of the Gaussian is applied

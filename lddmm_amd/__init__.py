@@ -163,6 +163,18 @@ def set_pair_culling(on):
     _check(lib().lddmm_set_pair_culling(int(bool(on))))
 
 
+def set_pair_cull_level(bits):
+    """Skip FP32 Gaussian tile pairs whose kernel values are all below 2^-bits
+    (default 48; 24..126)."""
+    _check(lib().lddmm_set_pair_cull_level(int(bits)))
+
+
+def set_pair_groups(on):
+    """Culling by precomputed 64-source group bounds, surviving groups packed
+    four to a tile (default on); off tests each staged quarter tile."""
+    _check(lib().lddmm_set_pair_groups(int(bool(on))))
+
+
 def set_pair_expansion(on):
     """Expanded form of the FP32 Gaussian pair terms (default on): r^2 as
     |x_i|^2 + (|x_j|^2 - 2 x_i.x_j) with the factor exp2(-|x_i|^2) applied

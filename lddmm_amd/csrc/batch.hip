@@ -82,10 +82,32 @@ struct BatchView {
     int rows, T;
     double h, w;
     double *ecen, *edir, *evol, *dc, *dd, *efwd, *evf;
-    int cull;     // culling of FP32 Gaussian tile pairs (kernel values < 2^-48)
+    int cull;     // culling level of FP32 Gaussian tile pairs (values < 2^-cull skipped; 0: off)
     int expand;   // expanded form of the FP32 Gaussian pair terms
     unsigned long long *exec;   // executed-pair counter (pair_counter), or null
+    // group bounds of q_t, of the deformed element centres and of the targets
+    // (BatchDesc gvo, geo, gto) for the culling operators; used when gpts /
+    // gvf (point kernels / varifold) are set
+    double4 *grpv, *grpe, *grpt;
+    int gpts, gvf;
 };
+
+// bounds of the aligned 64-point groups: q_t (which = 0), the deformed
+// element centres (1) or the target centres (2), one group per item
+template <int D>
+__global__ __launch_bounds__(kTile) void kb_group_bounds(BatchView v, int t, int which, const int4 *__restrict__ items)
+{
+    const int4 it = items[blockIdx.x];
+    const BatchDesc d = v.desc[it.x];
+    if (which == 0) {
+        const double *q = v.nde + (size_t)v.T * D * d.vo + (size_t)t * D * d.n;
+        group_bounds_wave<D>(q, d.n, it.y, threadIdx.x, v.grpv + d.gvo + (size_t)t * ((d.n + 63) / 64));
+    } else if (which == 1) {
+        group_bounds_wave<D>(v.ecen + (size_t)D * d.eo, d.ne, it.y, threadIdx.x, v.grpe + d.geo);
+    } else {
+        group_bounds_wave<D>(v.tc + (size_t)D * d.to, d.nt, it.y, threadIdx.x, v.grpt + d.gto);
+    }
+}
 
 template <int NC>
 __device__ __forceinline__ void sum_splits(const double *__restrict__ part, int splits, int n, int i,
@@ -128,10 +150,11 @@ __global__ __launch_bounds__(kBlock) void kb_kv(BatchView v, int t, const int4 *
                         d.n,
                         (real)d.iw,
                         (real)d.iw2,
-                        cull_r2(v.cull != 0),
+                        cull_r2(v.cull),
                         expand_r2(v.expand != 0),
                         Op::KF::kappa(d.iw, d.iw2),
-                        v.exec};
+                        v.exec,
+                        v.gpts ? v.grpv + d.gvo + (size_t)t * ((d.n + 63) / 64) : nullptr};
     const int j0 = it.z * d.chunk, j1 = min(d.n, j0 + d.chunk);
     pairs_block<Op>(A, d.n, it.y, j0, j1, v.part + d.po + (size_t)it.z * D * d.n, smem);
 }
@@ -153,11 +176,12 @@ __global__ __launch_bounds__(kBlock) void kb_bwd(BatchView v, int t, const int4 
                         d.n,
                         (real)d.iw,
                         (real)d.iw2,
-                        cull_r2(v.cull != 0),
+                        cull_r2(v.cull),
                         expand_r2(v.expand != 0),
                         kap,
                         Op::KF::cmul(d.iw, d.iw2) / kap,
-                        v.exec};
+                        v.exec,
+                        v.gpts ? v.grpv + d.gvo + (size_t)t * ((d.n + 63) / 64) : nullptr};
     const int j0 = it.z * d.chunk, j1 = min(d.n, j0 + d.chunk);
     pairs_block<Op>(A, d.n, it.y, j0, j1, v.part + d.po + (size_t)it.z * 2 * D * d.n, smem);
 }
@@ -266,10 +290,11 @@ __global__ __launch_bounds__(kBlock) void kb_vf(BatchView v, double ics2, double
     A.nj = tgt ? d.nt : d.ne;
     A.ics2 = (real)ics2;
     A.ids2 = (real)ids2;
-    A.cut2 = cull_r2(v.cull != 0);
+    A.cut2 = cull_r2(v.cull);
     A.xr2 = expand_r2(v.expand != 0);
     A.kap = cen_kappa<CEN>(ics2);
     A.exec = v.exec;
+    A.grp = v.gvf ? (tgt ? v.grpt + d.gto : v.grpe + d.geo) : nullptr;
     const int j0 = (tgt ? it.z - d.vsd : it.z) * d.vchunk, j1 = min(A.nj, j0 + d.vchunk);
     pairs_block<Op>(A, d.ne, it.y, j0, j1, v.vpart + d.vpo + (size_t)it.z * (2 * D + 2) * d.ne, smem);
 }
@@ -842,6 +867,17 @@ Batch::Batch(const BatchSpec &spec, const double *q0, const int *elm, const doub
     slab_.reserve(&efwd_, ntot_);
     slab_.reserve(&evf_, etot_);
     slab_.reserve(&tt_, P);
+    {
+        long long gv = 0, ge = 0, gt = 0;
+        for (int p = 0; p < P; ++p) {
+            gv += (long long)spec_.T * ((spec_.n[p] + 63) / 64);
+            ge += (spec_.ne[p] + 63) / 64;
+            gt += (spec_.nt[p] + 63) / 64;
+        }
+        slab_.reserve(&grpv_, (size_t)std::max(gv, 1LL));
+        slab_.reserve(&grpe_, (size_t)std::max(ge, 1LL));
+        slab_.reserve(&grpt_, (size_t)std::max(gt, 1LL));
+    }
     slab_.allocate();
     LDDMM_HIP(hipMemcpyAsync(q0_, hq.data(), hq.size() * sizeof(double), hipMemcpyHostToDevice, st_));
     LDDMM_HIP(hipMemcpyAsync(elm_, helm.data(), helm.size() * sizeof(int), hipMemcpyHostToDevice, st_));
@@ -863,7 +899,7 @@ void Batch::build_descs()
 {
     const int P = spec_.P(), D = spec_.dim;
     desc_h_.assign(P, BatchDesc{});
-    long long po = 0, vpo = 0;
+    long long po = 0, vpo = 0, gvo = 0, geo = 0, gto = 0;
     for (int p = 0; p < P; ++p) {
         BatchDesc &d = desc_h_[p];
         d.vo = vo_[p];
@@ -884,6 +920,12 @@ void Batch::build_descs()
         po += (long long)d.splits * 2 * D * d.n;
         d.vpo = vpo;
         d.tpo = vpo;   // the target-target pass runs once, before any evaluation
+        d.gvo = gvo;
+        d.geo = geo;
+        d.gto = gto;
+        gvo += (long long)spec_.T * ((d.n + 63) / 64);
+        geo += (d.ne + 63) / 64;
+        gto += (d.nt + 63) / 64;
         vpo += std::max((long long)(d.vsd + d.vst) * (2 * D + 2) * d.ne, (long long)d.tsplits * std::max(d.nt, 1));
     }
 }
@@ -967,6 +1009,19 @@ void Batch::compute_target_terms()
     }
     kb_tt_sum<<<P, kBlock, 0, st_>>>(v, tt_);
     LDDMM_LAUNCH_CHECK();
+    // group bounds of the fixed target centres (FP32 culling of the varifold)
+    std::vector<int4> gitems;
+    for (int p = 0; p < P; ++p)
+        for (int g = 0; g < ceil_div(desc_h_[p].nt, kTile); ++g) gitems.push_back(make_int4(p, g, 0, 0));
+    DArr<int4> git;
+    if (!gitems.empty()) {
+        git.ensure(gitems.size());
+        LDDMM_HIP(hipMemcpyAsync(git.p, gitems.data(), gitems.size() * sizeof(int4), hipMemcpyHostToDevice, st_));
+        v.grpt = grpt_;
+        if (D == 2) kb_group_bounds<2><<<(int)gitems.size(), kTile, 0, st_>>>(v, 0, 2, git.p);
+        else kb_group_bounds<3><<<(int)gitems.size(), kTile, 0, st_>>>(v, 0, 2, git.p);
+        LDDMM_LAUNCH_CHECK();
+    }
     LDDMM_HIP(hipStreamSynchronize(st_));
 }
 
@@ -985,7 +1040,7 @@ void Batch::eval(const double *alp, double *grd, double *f_dev, const std::vecto
         launch_eval(alp, grd, f_dev);
         return;
     }
-    const int cull = pair_culling() ? 1 : 0, expand = pair_expansion() ? 1 : 0;
+    const int cull = 2 * pair_cull_bits() + (pair_groups() ? 1 : 0), expand = pair_expansion() ? 1 : 0;
     for (const GraphEntry &e : graphs_)
         if (e.alp == alp && e.grd == grd && e.f == f_dev && e.cull == cull && e.expand == expand) {
             LDDMM_HIP(hipGraphLaunch(e.exec, st_));
@@ -1027,8 +1082,9 @@ void Batch::launch_eval(const double *alp, double *grd, double *f_dev)
     const int D = spec_.dim, T = spec_.T;
     const Tables &tb = *tab_;
     BatchView v{desc_, nde_, padj_, part_, vpart_, alp, grd, q0_, tc_, tu_, ta_, elm_, ifo_, rows_, T,
-                spec_.h, spec_.w, ecen_, edir_, evol_, dc_, dd_, efwd_, evf_, pair_culling() ? 1 : 0,
-                pair_expansion() ? 1 : 0, pair_counter()};
+                spec_.h, spec_.w, ecen_, edir_, evol_, dc_, dd_, efwd_, evf_, pair_cull_bits(),
+                pair_expansion() ? 1 : 0, pair_counter(), grpv_, grpe_, grpt_,
+                pair_groups() && spec_.order == -1 ? 1 : 0, pair_groups() && spec_.vp.cen_type == 'G' ? 1 : 0};
 
     // q_0 of the listed problems
     if (D == 2) kb_init_flow<2><<<tb.nvtile, kTile, 0, st_>>>(v, tb.vtile.p);
@@ -1049,6 +1105,11 @@ void Batch::launch_eval(const double *alp, double *grd, double *f_dev)
     for (int t = 0; t < T - 1; ++t) {
         {
             ProfScope ps(kProfFwdPairs, st_);
+            if (v.gpts && tb.nkv[1]) {   // group bounds of q_t for the FP32 culling
+                if (D == 2) kb_group_bounds<2><<<tb.nvtile, kTile, 0, st_>>>(v, t, 0, tb.vtile.p);
+                else kb_group_bounds<3><<<tb.nvtile, kTile, 0, st_>>>(v, t, 0, tb.vtile.p);
+                LDDMM_LAUNCH_CHECK();
+            }
             each_prec([&](int g, auto Pc) {
                 if (!tb.nkv[g]) return;
                 dispatch([&](auto Dc, auto Oc) {
@@ -1071,6 +1132,11 @@ void Batch::launch_eval(const double *alp, double *grd, double *f_dev)
     }
     {
         ProfScope ps(kProfVfPairs, st_);
+        if (v.gvf && tb.nvf[1]) {
+            if (D == 2) kb_group_bounds<2><<<tb.netile, kTile, 0, st_>>>(v, 0, 1, tb.etile.p);
+            else kb_group_bounds<3><<<tb.netile, kTile, 0, st_>>>(v, 0, 1, tb.etile.p);
+            LDDMM_LAUNCH_CHECK();
+        }
         each_prec([&](int g, auto Pc) {
             if (!tb.nvf[g]) return;
             dispatch_vf3(D, spec_.vp, [&](auto Dc, auto Cc, auto Uc) {
@@ -1163,7 +1229,7 @@ void Batch::autotune()
         const Tables &tb = *tab_;
         const int g = spec_.prec == kFP32 ? 1 : 0;
         BatchView v{desc_, nde_, padj_, part_, vpart_, alp.p, nullptr, q0_, tc_, tu_, ta_, elm_, ifo_, rows_,
-                    spec_.T, spec_.h, spec_.w, ecen_, edir_, evol_, dc_, dd_, efwd_, evf_, pair_culling() ? 1 : 0,
+                    spec_.T, spec_.h, spec_.w, ecen_, edir_, evol_, dc_, dd_, efwd_, evf_, pair_cull_bits(),
                     pair_expansion() ? 1 : 0, nullptr};
         auto launch = [&] {
             dispatch_order(spec_.order, [&](auto O) {

@@ -22,6 +22,9 @@ struct BatchDesc {
     long long po;     // point-kernel partial sums (doubles)
     long long vpo;    // varifold partial sums (doubles)
     long long tpo;    // target-target partial sums (doubles)
+    long long gvo;    // group bounds of q_t: T slices of ceil(n / 64) (double4)
+    long long geo;    // group bounds of the deformed element centres
+    long long gto;    // group bounds of the target centres
     int n, ne, nt;
     int chunk;        // point-kernel source chunk
     int splits;       // ceil(n / chunk)
@@ -110,6 +113,7 @@ private:
     double *nde_ = nullptr, *padj_ = nullptr, *part_ = nullptr, *vpart_ = nullptr;
     double *ecen_ = nullptr, *edir_ = nullptr, *evol_ = nullptr, *dc_ = nullptr, *dd_ = nullptr;
     double *efwd_ = nullptr, *evf_ = nullptr, *tt_ = nullptr;
+    double4 *grpv_ = nullptr, *grpe_ = nullptr, *grpt_ = nullptr;   // group bounds (FP32 culling)
     // work tables for the current active set
     struct Tables;
     std::unique_ptr<Tables> tab_;
@@ -117,7 +121,7 @@ private:
     struct GraphEntry {
         const double *alp;
         double *grd, *f;
-        int cull, expand;   // pair_culling(), pair_expansion() at capture
+        int cull, expand;   // pair_cull_bits(), pair_expansion() at capture
         hipGraphExec_t exec;
     };
     std::vector<GraphEntry> graphs_;

@@ -1053,6 +1053,20 @@ int lddmm_set_pair_culling(int on)
     return 0;
 }
 
+int lddmm_set_pair_cull_level(int bits)
+{
+    return guarded([&] {
+        set_pair_cull_bits(bits);
+        return 0;
+    });
+}
+
+int lddmm_set_pair_groups(int on)
+{
+    set_pair_groups(on != 0);
+    return 0;
+}
+
 int lddmm_set_pair_expansion(int on)
 {
     set_pair_expansion(on != 0);

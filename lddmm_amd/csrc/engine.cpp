@@ -196,6 +196,9 @@ Objective::Objective(const DeviceProblem &p, hipStream_t st, int target_blocks) 
     slab_.reserve(&vperm_, (size_t)p_.n);
     slab_.reserve(&alp_i_, (size_t)p_.varnum());
     slab_.reserve(&grd_i_, (size_t)p_.varnum());
+    slab_.reserve(&grp_q_, (size_t)p_.T * group_count(p_.n));
+    slab_.reserve(&grp_e_, (size_t)std::max(1, group_count(p_.ne)));
+    slab_.reserve(&grp_t_, (size_t)std::max(1, group_count(p_.nt)));
     if (p_.tl) {
         slab_.reserve(&tl_, (size_t)D * p_.n);
     } else {
@@ -371,6 +374,7 @@ void Objective::compute_target_target(int target_blocks)
         tt_host_ = 0.0;
         return;
     }
+    launch_group_bounds(p_.dim, p_.tc, p_.nt, grp_t_, st_);   // fixed targets: once
     PairSplit sp = plan_vf(p_.nt, p_.nt, target_blocks);
     {
         ProfScope ps(kProfVfTargetTarget, st_);
@@ -396,12 +400,18 @@ void Objective::eval_internal(const double *alp, double *grd, double *f_dev)
     // forward flow q_{t+1} = q_t + h K(q_t) alpha_t, energy partials <alpha_t, v_t>
     // (Cpp/src/objgrd.cpp:41-59)
     LDDMM_HIP(hipMemcpyAsync(nde_, p_.q0, dn * sizeof(double), hipMemcpyDeviceToDevice, st_));
+    // FP32 Gaussian: group bounds of each q_t for the culling of the pair
+    // kernels (forward here, the adjoint below)
+    const bool groups = p_.prec == kFP32 && p_.order == -1 && pair_groups();
+    const int ng = group_count(n);
     for (int t = 0; t < T - 1; ++t) {
         const double *qt = nde_ + (size_t)t * dn;
         const double *at = alp + (size_t)t * dn;
         {
             ProfScope ps(kProfFwdPairs, st_);
-            launch_kv(D, p_.order, p_.prec, qt, n, qt, at, n, p_.width, sp_kv_, part_, st_);
+            if (groups) launch_group_bounds(D, qt, n, grp_q_ + (size_t)t * ng, st_);
+            launch_kv(D, p_.order, p_.prec, qt, n, qt, at, n, p_.width, sp_kv_, part_, st_,
+                      groups ? grp_q_ + (size_t)t * ng : nullptr);
         }
         {
             ProfScope ps(kProfFwdReduce, st_);
@@ -429,8 +439,11 @@ void Objective::eval_internal(const double *alp, double *grd, double *f_dev)
         }
         {
             ProfScope ps(kProfVfPairs, st_);
+            const bool vgroups = p_.prec == kFP32 && p_.vp.cen_type == 'G' && pair_groups();
+            if (vgroups) launch_group_bounds(D, ecen_, p_.ne, grp_e_, st_);
             launch_vf_pairs(D, p_.prec, p_.vp, ecen_, edir_, evol_, p_.ne, p_.tc, p_.tu, p_.ta, p_.nt, sp_vfd_,
-                            sp_vft_, part_, st_);
+                            sp_vft_, part_, st_, vgroups ? grp_e_ : nullptr,
+                            vgroups && p_.nt > 0 ? grp_t_ : nullptr);
         }
         {
             ProfScope ps(kProfVfFinalize, st_);
@@ -453,7 +466,8 @@ void Objective::eval_internal(const double *alp, double *grd, double *f_dev)
         const double *at = alp + (size_t)t * dn;
         {
             ProfScope ps(kProfBwdPairs, st_);
-            launch_bwd_pairs(D, p_.order, p_.prec, qt, at, p_adj_, n, p_.width, sp_bwd_, part_, st_);
+            launch_bwd_pairs(D, p_.order, p_.prec, qt, at, p_adj_, n, p_.width, sp_bwd_, part_, st_,
+                             groups ? grp_q_ + (size_t)t * ng : nullptr);
         }
         {
             ProfScope ps(kProfBwdReduce, st_);

@@ -207,6 +207,9 @@ private:
     double *q0_ = nullptr, *tc_ = nullptr, *tu_ = nullptr, *ta_ = nullptr, *tl_ = nullptr;
     int *elm_ = nullptr, *ifo_ = nullptr, *vperm_ = nullptr;
     double *alp_i_ = nullptr, *grd_i_ = nullptr;
+    // FP32 culling: group bounds of q_t (T slices), of the deformed element
+    // centres and of the target centres
+    double4 *grp_q_ = nullptr, *grp_e_ = nullptr, *grp_t_ = nullptr;
     int nb_fwd_ = 0, nb_vf_ = 0, nb_l2_ = 0, target_blocks_ = 0;
     double tt_host_ = 0.0;
     void reorder();

@@ -14,6 +14,16 @@ namespace lddmm {
 // a few FP64 ulps.
 void set_pair_culling(bool on);
 bool pair_culling();
+// The culling level: kernel values below 2^-bits are skipped (24..126; 48 by
+// default); pair_cull_bits() is the level in force, 0 when culling is off.
+constexpr int kDefaultCullBits = 48;
+void set_pair_cull_bits(int bits);
+int pair_cull_bits();
+// Culling by precomputed 64-source group bounds with the surviving groups
+// packed densely (on by default); off keeps the tile-by-tile range test.
+// pair_groups() is true when groups are on and culling is on.
+void set_pair_groups(bool on);
+bool pair_groups();
 // The expanded form of the FP32 Gaussian pair terms (pair_driver.hpp); on
 // by default, read at every launch like the culling switch.  Off gives the
 // direct form (results differ by FP32 rounding).
@@ -32,8 +42,14 @@ PairSplit plan_kv(int ni, int nj, int prec, int target_blocks);
 PairSplit plan_bwd(int ni, int nj, int prec, int order, int target_blocks);
 PairSplit plan_single(int ni, int nj, int target_blocks);
 // part[(s*D + c)*ni + i] = sum_{j in chunk s} k(qi_i, qj_j) a_j[c]
+// Bounds (centroid, radius) of the aligned 64-point groups of an SoA point set,
+// one double4 per group: the culling operators (FP32) skip whole groups out of
+// range of a block's targets and pack the others densely.  group_count(n) groups.
+int group_count(int n);
+void launch_group_bounds(int dim, const double *x, int n, double4 *out, hipStream_t st);
+// grp (optional, FP32 Gaussian only): group bounds of the sources qj
 void launch_kv(int dim, int order, int prec, const double *qi, int ni, const double *qj, const double *a, int nj,
-               double width, const PairSplit &sp, double *part, hipStream_t st);
+               double width, const PairSplit &sp, double *part, hipStream_t st, const double4 *grp = nullptr);
 // out[c*n + i] = scale * sum_s part[(s*comps + c)*n + i]
 void launch_sum_splits(int comps, int splits, int n, const double *part, double scale, double *out,
                        hipStream_t st);
@@ -46,7 +62,8 @@ int fwd_reduce_blocks(int n);
 // sum_j k_ij amp_j[c] (amp = a - p), for c >= D the dq sum of
 // l^T K r with l = amp - a/2, r = a.  Plan: plan_bwd.
 void launch_bwd_pairs(int dim, int order, int prec, const double *q, const double *a, const double *p, int n,
-                      double width, const PairSplit &sp, double *part, hipStream_t st);
+                      double width, const PairSplit &sp, double *part, hipStream_t st,
+                      const double4 *grp = nullptr);
 // g = h * sum_s gpart;  p -= h * sum_s dpart
 void launch_bwd_reduce(int dim, int splits, int n, const double *part, double h, double *g, double *p,
                        hipStream_t st);
@@ -74,7 +91,8 @@ PairSplit plan_vf(int ni, int nj, int target_blocks);
 // partial sums [(s*(2D+2) + c)*ne + i].
 void launch_vf_pairs(int dim, int prec, const VfParams &vp, const double *dcen, const double *ddir, const double *dvol,
                      int ne, const double *tcen, const double *tdir, const double *tvol, int nt, const PairSplit &spd,
-                     const PairSplit &spt, double *part, hipStream_t st);
+                     const PairSplit &spt, double *part, hipStream_t st, const double4 *grp_d = nullptr,
+                     const double4 *grp_t = nullptr);
 // Per element: energy partials per block into eparts, dc and dd; the
 // first `sd` splits are deformed-deformed, the rest deformed-target.
 void launch_vf_finalize(int dim, const VfParams &vp, int sd, int splits, int ne, const double *part,

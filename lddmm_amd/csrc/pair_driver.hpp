@@ -289,9 +289,13 @@ struct KernelFnT<-1, float, 2> : KernelFnT<-1, float, 3> {
 constexpr int kWaves = kBlock / 64;
 
 // args.cut2 of a culling operator (FP32 Gaussian on coordinates scaled so
-// that k = exp2(-r2)): skipped pairs have k < 2^-48; a 1e-4 relative margin
-// covers the FP32 rounding of the bounds; +inf turns culling off
-__host__ __device__ inline float cull_r2(bool on) { return on ? (float)(48.0 * (1.0 + 1e-4)) : __builtin_inff(); }
+// that k = exp2(-r2)): skipped pairs have k < 2^-bits (48 by default,
+// pair_cull_bits); a 1e-4 relative margin covers the FP32 rounding of the
+// bounds; bits <= 0 gives +inf, culling off
+__host__ __device__ inline float cull_r2(int bits)
+{
+    return bits > 0 ? (float)(bits * (1.0 + 1e-4)) : __builtin_inff();
+}
 
 // args.xr2 of an expanding operator: the largest squared target radius
 // (scaled units) of a block that takes the expanded form; -1: never
@@ -326,6 +330,38 @@ __device__ __forceinline__ float wave_min(float v)
     return v;
 }
 
+__device__ __forceinline__ double wave_max_d(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Bounds of the aligned 64-point groups of an SoA point set (x[c*n + i]):
+// out[g] = (centroid, radius) of points 64g .. 64g+63, in FP64 and unscaled
+// coordinates.  One wave per group (g from the caller); every lane returns.
+// The culling operators skip whole groups whose sphere is out of range of a
+// block's target tile (pairs_block), so the four waves stay busy.
+template <int D>
+__device__ __forceinline__ void group_bounds_wave(const double *__restrict__ x, int n, int g, int lane,
+                                                  double4 *__restrict__ out)
+{
+    const int i = g * 64 + lane;
+    const bool ok = i < n;
+    const double cnt = (double)min(64, n - g * 64);
+    double c[3] = {0.0, 0.0, 0.0}, v[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        v[k] = ok ? x[(size_t)k * n + i] : 0.0;
+        c[k] = wave_sum(v[k]) / cnt;
+    }
+    double r2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) r2 += (v[k] - c[k]) * (v[k] - c[k]);
+    const double r = sqrt(wave_max_d(ok ? r2 : 0.0));
+    if (lane == 0) out[g] = make_double4(c[0], c[1], c[2], r);
+}
+
 // LDS bytes of one block: the staging tiles and the final wave reduction
 // share one buffer.
 template <class Op>
@@ -339,8 +375,10 @@ struct PairSmem {
     static constexpr size_t rec = (xoff + csize * DX + 15) & ~size_t(15);
     static constexpr size_t stage = rec * kTileJ;
     static constexpr size_t cull = Op::kCull ? sizeof(float) * kTileJ : 0;   // source distances
+    // surviving-group list of a window of kBlock groups and the per-wave counts
+    static constexpr size_t grp = Op::kCull ? sizeof(int) * (kBlock + kWaves) : 0;
     static constexpr size_t red = sizeof(double) * kWaves * Op::OUT * 64 * Op::R;
-    static constexpr size_t bytes = stage + cull > red ? stage + cull : red;
+    static constexpr size_t bytes = stage + cull + grp > red ? stage + cull + grp : red;
 };
 
 // One block's work: targets [tile*TI, tile*TI + TI) against sources
@@ -416,83 +454,167 @@ __device__ __forceinline__ void pairs_block(const typename Op::Args &args, int n
     }
     float *s2buf = reinterpret_cast<float *>(smem + SM::stage);   // squared source distances to the anchor
 
-    // one sweep over the chunk's sources in the direct (E false) or the
-    // expanded form
-    auto sweep = [&](auto form) {
+    // staging of source j into record slot `slot` (and its squared distance
+    // to the anchor for the range test)
+    auto stage = [&](auto form, int j, int slot) {
+        constexpr bool E = decltype(form)::value;
+        double x[D];
+        lds_t f[F > 0 ? F : 1];
+        Op::load_src(args, j, x, f);
+        char *rec = smem + SM::rec * slot;
+#pragma unroll
+        for (int c = 0; c < F; ++c) reinterpret_cast<lds_t *>(rec)[c] = f[c];
+        float s2 = 0.0f;
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            const crd_t v = to_crd(x[c], c);
+            reinterpret_cast<crd_t *>(rec + SM::xoff)[c] = v;
+            s2 += (float)v * (float)v;
+        }
+        if constexpr (E) reinterpret_cast<crd_t *>(rec + SM::xoff)[D] = s2;
+        if constexpr (Op::kCull) s2buf[slot] = s2;
+    };
+    // the wave's pairs with the staged records [t0, t1): range test (culling
+    // operators), the pair terms in the policy's type, the flush into FP64
+    auto range = [&](auto form, int t0, int t1) {
         constexpr bool E = decltype(form)::value;
         using Ox = std::conditional_t<E, Op, Op>;   // dependent: pair_x/flush_x only when E
+        if (t0 >= t1) return;
+        if constexpr (Op::kCull) {
+            const float ms2 = wave_min(t0 + lane < t1 ? s2buf[t0 + lane] : __builtin_inff());
+            const float ms = __builtin_sqrtf(ms2);
+            bool skip;
+            if constexpr (E) {
+                const float gap = ms - 2.0f * rt;
+                skip = gap > 0.0f && ms * gap > (float)args.cut2;
+            } else {
+                const float gap = ms - rt;
+                skip = gap > 0.0f && gap * gap > (float)args.cut2;
+            }
+            if (__builtin_amdgcn_readfirstlane((int)skip)) return;
+            // diagnostic count of the pairs a culling operator computes
+            if (args.exec != nullptr && lane == 0)
+                atomicAdd(args.exec + Op::kCountSlot,
+                          (unsigned long long)(t1 - t0) * (unsigned long long)min(TI, ni - tile * TI));
+        }
+        typename Op::Acc acc[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) Op::zero(acc[r]);
+#pragma unroll 2
+        for (int t = t0; t < t1; ++t) {
+            crd_t xj[E ? D + 1 : D];
+            lds_t fj[F > 0 ? F : 1];
+            const char *rec = smem + SM::rec * t;
+#pragma unroll
+            for (int c = 0; c < F; ++c) fj[c] = reinterpret_cast<const lds_t *>(rec)[c];
+#pragma unroll
+            for (int c = 0; c < (E ? D + 1 : D); ++c) xj[c] = reinterpret_cast<const crd_t *>(rec + SM::xoff)[c];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if constexpr (E) Ox::pair_x(args, tg[r], xm[r], xj, fj, acc[r]);
+                else Op::pair(args, tg[r], xi[r], xj, fj, acc[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if constexpr (E) Ox::flush_x(args, acc[r], tot[r], xi[r], sc[r]);
+            else Op::flush(args, acc[r], tot[r]);
+        }
+    };
+    // one sweep over the chunk's sources in the direct (E false) or the
+    // expanded form: consecutive tiles of kTileJ sources, wave w taking the
+    // w-th quarter of each.  The culling operators keep the quarters at 64
+    // sources also in a partial last tile, so that every FP32 range sum is
+    // over one aligned 64-source group, as in the grouped sweep below: with
+    // and without culling the same groups give the same FP32 sums.
+    auto sweep = [&](auto form) {
         for (int jt = j0; jt < j1; jt += kTileJ) {
             const int cnt = min(kTileJ, j1 - jt);
             __syncthreads();
-            if ((int)threadIdx.x < cnt) {
-                double x[D];
-                lds_t f[F > 0 ? F : 1];
-                Op::load_src(args, jt + threadIdx.x, x, f);
-                char *rec = smem + SM::rec * threadIdx.x;
-#pragma unroll
-                for (int c = 0; c < F; ++c) reinterpret_cast<lds_t *>(rec)[c] = f[c];
-                float s2 = 0.0f;
-#pragma unroll
-                for (int c = 0; c < D; ++c) {
-                    const crd_t v = to_crd(x[c], c);
-                    reinterpret_cast<crd_t *>(rec + SM::xoff)[c] = v;
-                    s2 += (float)v * (float)v;
-                }
-                if constexpr (E) reinterpret_cast<crd_t *>(rec + SM::xoff)[D] = s2;
-                if constexpr (Op::kCull) s2buf[threadIdx.x] = s2;
-            }
+            if ((int)threadIdx.x < cnt) stage(form, jt + threadIdx.x, threadIdx.x);
             __syncthreads();
-            const int share = (cnt + kWaves - 1) / kWaves;   // <= 64
-            const int t0 = wave * share, t1 = min(t0 + share, cnt);
-            if (t0 >= t1) continue;
-            if constexpr (Op::kCull) {
-                const float ms2 = wave_min(t0 + lane < t1 ? s2buf[t0 + lane] : __builtin_inff());
-                const float ms = __builtin_sqrtf(ms2);
-                bool skip;
+            const int share = Op::kCull ? 64 : (cnt + kWaves - 1) / kWaves;   // <= 64
+            const int t0 = wave * share;
+            range(form, t0, min(t0 + share, cnt));
+        }
+    };
+    // The grouped sweep of the culling operators, given the bounds of the
+    // aligned 64-source groups (args.grp): a window of up to kBlock groups is
+    // tested against the block's targets (one group per thread: the distance
+    // from the anchor to the group's sphere, less the tile radius, against
+    // the culling range), the surviving groups are listed in order, and they
+    // are staged four at a time, one per wave, so every staged tile is full
+    // of sources in range and the waves share the work evenly.  Inside a
+    // group the per-wave range test still applies.
+    auto sweep_groups = [&](auto form) {
+      if constexpr (Op::kCull) {
+        constexpr bool E = decltype(form)::value;
+        int *glist = reinterpret_cast<int *>(smem + SM::stage + SM::cull);
+        int *wcnt = glist + kBlock;
+        const int gA = j0 >> 6, gB = (j1 + 63) >> 6;
+        for (int w0 = gA; w0 < gB; w0 += kBlock) {
+            const int wn = min(kBlock, gB - w0);
+            __syncthreads();   // the previous window's list and tiles are consumed
+            bool live = false;
+            if ((int)threadIdx.x < wn) {
+                const double4 b = args.grp[w0 + threadIdx.x];
+                const double bc[3] = {b.x, b.y, b.z};
+                double d2 = 0.0;
+#pragma unroll
+                for (int c = 0; c < D; ++c) d2 += (bc[c] - anc[c]) * (bc[c] - anc[c]);
+                const float ms = (float)(fmax(0.0, sqrt(d2) - b.w) * kap);
                 if constexpr (E) {
                     const float gap = ms - 2.0f * rt;
-                    skip = gap > 0.0f && ms * gap > (float)args.cut2;
+                    live = !(gap > 0.0f && ms * gap > (float)args.cut2);
                 } else {
                     const float gap = ms - rt;
-                    skip = gap > 0.0f && gap * gap > (float)args.cut2;
-                }
-                if (__builtin_amdgcn_readfirstlane((int)skip)) continue;
-                // diagnostic count of the pairs a culling operator computes
-                if (args.exec != nullptr && lane == 0)
-                    atomicAdd(args.exec + Op::kCountSlot,
-                              (unsigned long long)(t1 - t0) * (unsigned long long)min(TI, ni - tile * TI));
-            }
-            typename Op::Acc acc[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r) Op::zero(acc[r]);
-#pragma unroll 2
-            for (int t = t0; t < t1; ++t) {
-                crd_t xj[E ? D + 1 : D];
-                lds_t fj[F > 0 ? F : 1];
-                const char *rec = smem + SM::rec * t;
-#pragma unroll
-                for (int c = 0; c < F; ++c) fj[c] = reinterpret_cast<const lds_t *>(rec)[c];
-#pragma unroll
-                for (int c = 0; c < (E ? D + 1 : D); ++c) xj[c] = reinterpret_cast<const crd_t *>(rec + SM::xoff)[c];
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    if constexpr (E) Ox::pair_x(args, tg[r], xm[r], xj, fj, acc[r]);
-                    else Op::pair(args, tg[r], xi[r], xj, fj, acc[r]);
+                    live = !(gap > 0.0f && gap * gap > (float)args.cut2);
                 }
             }
+            const unsigned long long bal = __ballot(live);
+            if (lane == 0) wcnt[wave] = __popcll(bal);
+            __syncthreads();
+            int base = 0, total = 0;
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                if constexpr (E) Ox::flush_x(args, acc[r], tot[r], xi[r], sc[r]);
-                else Op::flush(args, acc[r], tot[r]);
+            for (int w = 0; w < kWaves; ++w) {
+                base += w < wave ? wcnt[w] : 0;
+                total += wcnt[w];
+            }
+            if (live)
+                glist[base + __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u))] =
+                    w0 + threadIdx.x;
+            for (int k = 0; k < total; k += kWaves) {
+                const int ng = min(kWaves, total - k);
+                __syncthreads();   // the list is written; the previous tile is consumed
+                if (wave < ng) {
+                    const int j = glist[k + wave] * 64 + lane;
+                    if (j < j1) stage(form, j, threadIdx.x);
+                }
+                __syncthreads();
+                if (wave < ng) {
+                    const int g0 = glist[k + wave] * 64;
+                    range(form, 64 * wave, 64 * wave + min(64, j1 - g0));
+                }
             }
         }
+      }
+    };
+    auto run = [&](auto form) {
+        if constexpr (Op::kCull) {
+            if (args.grp != nullptr && (j0 & 63) == 0) {
+                sweep_groups(form);
+                return;
+            }
+        }
+        sweep(form);
     };
     if constexpr (kExp) {
         // rtw2 is the same in every wave (all hold the block's targets)
-        if (__builtin_amdgcn_readfirstlane((int)(rtw2 <= args.xr2))) sweep(std::true_type{});
-        else sweep(std::false_type{});
+        if (__builtin_amdgcn_readfirstlane((int)(rtw2 <= args.xr2))) run(std::true_type{});
+        else run(std::false_type{});
     } else {
-        sweep(std::false_type{});
+        run(std::false_type{});
     }
 
     // fixed-order combination of the four waves' totals

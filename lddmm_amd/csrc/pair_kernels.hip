@@ -124,6 +124,14 @@ __global__ __launch_bounds__(kRedBlock) void k_bwd_reduce(int splits, int n, con
     }
 }
 
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_group_bounds(const double *__restrict__ x, int n, int ng,
+                                                         double4 *__restrict__ out)
+{
+    const int g = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (g < ng) group_bounds_wave<D>(x, n, g, threadIdx.x & 63, out);
+}
+
 void check_dim_order(int dim, int order)
 {
     if (dim != 2 && dim != 3) throw Error(kErrArg, "dimension must be 2 or 3");
@@ -165,6 +173,8 @@ PairSplit plan_single(int ni, int nj, int target_blocks) { return make_split_ti(
 
 namespace {
 std::atomic<bool> g_pair_culling{true}, g_pair_expansion{true};
+std::atomic<int> g_cull_bits{kDefaultCullBits};
+std::atomic<bool> g_pair_groups{true};
 std::atomic<unsigned long long *> g_pair_counter{nullptr};
 }
 
@@ -172,11 +182,30 @@ void set_pair_counter(unsigned long long *dev_counter) { g_pair_counter.store(de
 unsigned long long *pair_counter() { return g_pair_counter.load(); }
 void set_pair_culling(bool on) { g_pair_culling.store(on); }
 bool pair_culling() { return g_pair_culling.load(); }
+void set_pair_cull_bits(int bits)
+{
+    if (bits < 24 || bits > 126) throw Error(kErrArg, "the culling level must be 24..126 (kernel values below 2^-level)");
+    g_cull_bits.store(bits);
+}
+int pair_cull_bits() { return g_pair_culling.load() ? g_cull_bits.load() : 0; }
+void set_pair_groups(bool on) { g_pair_groups.store(on); }
+bool pair_groups() { return g_pair_groups.load() && pair_cull_bits() > 0; }
 void set_pair_expansion(bool on) { g_pair_expansion.store(on); }
 bool pair_expansion() { return g_pair_expansion.load(); }
 
+int group_count(int n) { return (n + 63) / 64; }
+
+void launch_group_bounds(int dim, const double *x, int n, double4 *out, hipStream_t st)
+{
+    const int ng = group_count(n);
+    if (ng <= 0) return;
+    if (dim == 2) k_group_bounds<2><<<ceil_div(ng, kWaves), kBlock, 0, st>>>(x, n, ng, out);
+    else k_group_bounds<3><<<ceil_div(ng, kWaves), kBlock, 0, st>>>(x, n, ng, out);
+    LDDMM_LAUNCH_CHECK();
+}
+
 void launch_kv(int dim, int order, int prec, const double *qi, int ni, const double *qj, const double *a, int nj,
-               double width, const PairSplit &sp, double *part, hipStream_t st)
+               double width, const PairSplit &sp, double *part, hipStream_t st, const double4 *grp)
 {
     if (ni <= 0) return;
     dispatch_all(dim, order, prec, [&](auto Dc, auto Oc, auto Pc) {
@@ -191,10 +220,11 @@ void launch_kv(int dim, int order, int prec, const double *qi, int ni, const dou
                             nj,
                             (real)iw,
                             (real)iw2,
-                            cull_r2(pair_culling()),
+                            cull_r2(pair_cull_bits()),
                             expand_r2(pair_expansion()),
                             Op::KF::kappa(iw, iw2),
-                            pair_counter()};
+                            pair_counter(),
+                            grp};
         launch_pairs<Op>(A, ni, nj, sp, 0, part, st);
     });
 }
@@ -220,15 +250,15 @@ void launch_fwd_reduce(int dim, int splits, int n, const double *part, const dou
 }
 
 void launch_bwd_pairs(int dim, int order, int prec, const double *q, const double *a, const double *p, int n,
-                      double width, const PairSplit &sp, double *part, hipStream_t st)
+                      double width, const PairSplit &sp, double *part, hipStream_t st, const double4 *grp)
 {
     dispatch_all(dim, order, prec, [&](auto Dc, auto Oc, auto Pc) {
         constexpr int R = decltype(Pc)::fp32 ? bwd_r32(Oc()) : 1;
         using Op = OpBwd<Dc(), Oc(), decltype(Pc), R>;
         using real = typename Op::real;
         const double iw = 1.0 / width, iw2 = 1.0 / (width * width), kap = Op::KF::kappa(iw, iw2);
-        typename Op::Args A{q,   a,   p, n, (real)iw, (real)iw2, cull_r2(pair_culling()), expand_r2(pair_expansion()),
-                            kap, Op::KF::cmul(iw, iw2) / kap, pair_counter()};
+        typename Op::Args A{q,   a,   p, n, (real)iw, (real)iw2, cull_r2(pair_cull_bits()), expand_r2(pair_expansion()),
+                            kap, Op::KF::cmul(iw, iw2) / kap, pair_counter(), grp};
         launch_pairs<Op>(A, n, n, sp, 0, part, st);
     });
 }

@@ -42,6 +42,7 @@ struct OpKV {
         float xr2;    // expanded-form radius bound (expand_r2)
         double kap;   // KF::kappa
         unsigned long long *exec = nullptr;   // executed-pair count (pair_counter), or none
+        const double4 *grp = nullptr;   // bounds of the aligned 64-source groups, or none
     };
     __device__ __forceinline__ static double crd_scale(const Args &A) { return A.kap; }
     struct Tgt {
@@ -131,6 +132,7 @@ struct OpBwd {
         float xr2;         // expanded-form radius bound (expand_r2)
         double kap, pds;   // KF::kappa; dq flush factor KF::cmul / kappa
         unsigned long long *exec = nullptr;   // executed-pair count (pair_counter), or none
+        const double4 *grp = nullptr;   // bounds of the aligned 64-source groups, or none
     };
     __device__ __forceinline__ static double crd_scale(const Args &A) { return A.kap; }
     struct Tgt {

@@ -226,7 +226,7 @@ void launch_elm_geometry(int dim, const double *q, int nq, const int *elm, int n
 
 void launch_vf_pairs(int dim, int prec, const VfParams &vp, const double *dcen, const double *ddir, const double *dvol,
                      int ne, const double *tcen, const double *tdir, const double *tvol, int nt, const PairSplit &spd,
-                     const PairSplit &spt, double *part, hipStream_t st)
+                     const PairSplit &spt, double *part, hipStream_t st, const double4 *grp_d, const double4 *grp_t)
 {
     if (ne <= 0) return;
     if (prec != kFP64 && prec != kFP32) throw Error(kErrArg, "unknown precision policy");
@@ -235,12 +235,12 @@ void launch_vf_pairs(int dim, int prec, const VfParams &vp, const double *dcen, 
             using Op = OpVf<Dc(), Cc(), Uc(), decltype(Pc)>;
             using real = typename Op::real;
             const real ics2 = (real)inv_sq(vp.cen_width), ids2 = (real)inv_sq(vp.dir_width);
-            const float cut2 = cull_r2(pair_culling()), xr2 = expand_r2(pair_expansion());
+            const float cut2 = cull_r2(pair_cull_bits()), xr2 = expand_r2(pair_expansion());
             const double kap = cen_kappa<Cc()>(inv_sq(vp.cen_width));
-            typename Op::Args dd{dcen, ddir, ne, dcen, ddir, dvol, ne, ics2, ids2, cut2, xr2, kap, pair_counter()};
+            typename Op::Args dd{dcen, ddir, ne, dcen, ddir, dvol, ne, ics2, ids2, cut2, xr2, kap, pair_counter(), grp_d};
             launch_pairs<Op>(dd, ne, ne, spd, 0, part, st);
             if (nt > 0) {
-                typename Op::Args dt{dcen, ddir, ne, tcen, tdir, tvol, nt, ics2, ids2, cut2, xr2, kap, pair_counter()};
+                typename Op::Args dt{dcen, ddir, ne, tcen, tdir, tvol, nt, ics2, ids2, cut2, xr2, kap, pair_counter(), grp_t};
                 launch_pairs<Op>(dt, ne, nt, spt, spd.splits, part, st);
             }
         };

@@ -107,6 +107,7 @@ struct OpVf {
         float xr2;    // expanded-form radius bound (expand_r2)
         double kap;   // cen_kappa of the staged offsets
         unsigned long long *exec = nullptr;   // executed-pair count (pair_counter), or none
+        const double4 *grp = nullptr;   // bounds of the aligned 64-source groups, or none
     };
     __device__ __forceinline__ static double crd_scale(const Args &A) { return A.kap; }
     struct Tgt {
