@@ -55,6 +55,12 @@ FLOPS_PER_PAIR = {
     ("fwd_pairs", 3, -1): 15, ("fwd_pairs", 3, 1): 17, ("fwd_pairs", 2, -1): 10,
     ("bwd_pairs", 3, -1): 33, ("bwd_pairs", 3, 1): 35, ("bwd_pairs", 2, -1): 22,
     ("vf_pairs", 3, "GB"): 32, ("vf_pairs", 2, "GB"): 23,
+    # Matern order 3 (nu = 7/2): cubic polynomial in d for k (3 FMAs instead of the add of order 1),
+    # quadratic for the gradient factor (2 FMAs)
+    ("fwd_pairs", 3, 3): 22, ("bwd_pairs", 3, 3): 44,
+    # 'C' (Cauchy) position kernel: 1 + r^2/s^2 (an FMA), a reciprocal (0) instead of the exp, and
+    # one more multiply for the squared kernel of its gradient
+    ("vf_pairs", 3, "CB"): 35,
 }
 # hardware transcendental instructions per pair of the FP32 kernels (v_exp_f32,
 # plus v_sqrt_f32 for the Matern distance)
@@ -81,6 +87,17 @@ def workload(name):
         return G.sphere_cube(subdivisions=5, cells=58), dict(itr=300)
     if name == "circle_square":
         return G.circle_square(400), dict(itr=200)
+    if name == "sphere2cube_example":
+        # the reference's own example (Cpp/examples/sphere2cube/run_sphere2cube.m:1-31): its
+        # sphereData.mat / cubeData.mat arrays (tests/golden/sphere2cube.npz), Matern order 3
+        # of width 0.25, 'C' 0.1 x 'B' varifold, weight 100, T = 21; L-BFGS to |g| < 1e-2
+        from lddmm_amd import io
+        z = np.load(os.path.join(ROOT, "tests", "golden", "sphere2cube.npz"))
+        w = io.problem_from_arrays({k: z[k] for k in ("iniNdeMat", "elmVtxMat", "tgtCenPosMat", "tgtUniDirMat",
+                                                      "tgtElmVolVec")})
+        w.update(cen_type="C", cen_width=0.1, dir_type="B", dir_width=0.0, dis_wgt=100.0, knl_order=3,
+                 knl_width=0.25, time_stp=0.05, time_num=21)
+        return w, dict(itr=100000, tol=1e-2, mem=10000)
     raise SystemExit("unknown config " + name)
 
 
@@ -671,9 +688,10 @@ def main():
                   w["cen_width"], w["dir_type"], w["dir_width"], w["dis_wgt"], w["knl_order"], w["knl_width"],
                   w["time_stp"], T)
         regs = {}
-        a64, i64 = lddmm_amd.run(*common, itr, 0.0, 0.0, 0.5, 5, precision="fp64")
+        tol, mem = extra.get("tol", 0.0), extra.get("mem", 5)
+        a64, i64 = lddmm_amd.run(*common, itr, tol, 0.0, 0.5, mem, precision="fp64")
         e_ref = i64["fcn"]
-        a32, i32 = lddmm_amd.run(*common, 3 * itr, 0.0, 0.0, 0.5, 5, precision="fp32",
+        a32, i32 = lddmm_amd.run(*common, 3 * itr, 0.0, 0.0, 0.5, mem, precision="fp32",
                                  target_energy=e_ref * (1.0 + 1e-6))
         e32, _ = obj(a32)   # FP32-policy result re-evaluated in FP64
         for k, info in (("fp64", i64), ("fp32", i32)):
