@@ -283,6 +283,42 @@ def warm_start_leg(probs, targets, e_ref, B, B64, t_setup, itr, weights):
 WARM_WEIGHTS = os.path.join(ROOT, "lddmm_amd", "data", "warmstart_c4.npz")
 
 
+def launch_shapes_report(count, nmin, nmax, seed=4):
+    """FP32 K.alpha and adjoint pair-kernel time per evaluation of the
+    uneven batch (HIP events around each launch group, direct launches) at
+    the autotuned launch shape and at every shape of the grid source chunk
+    {512, 1024, 2048} x targets per lane {(1, 1), (2, 2), (2, 1), (4, 2),
+    (4, 1)}, at momenta 0.02 x N(0, 1)."""
+    probs = G.batch_problems(count, seed, nmin, nmax)
+    k = G.BATCH_KERNELS
+    T = k["time_num"]
+    t0 = time.perf_counter()
+    B = lddmm_amd.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
+                        k["knl_order"], k["time_stp"], T, precision="fp32", chunk=0)
+    t_setup = time.perf_counter() - t0
+    tuned = B.tuning()
+    B.set_graphs(False)
+    alps = [0.02 * G.normal(100 + i, B.sizes[i]).reshape(T - 1, B.n[i], 3) for i in range(B.P)]
+    rows = []
+    for sh in [tuned] + [(c, kv, bw) for c in (512, 1024, 2048) for kv, bw in ((1, 1), (2, 2), (2, 1), (4, 2), (4, 1))]:
+        B.set_tuning(*sh)
+        B.objgrd(alps)
+        lddmm_amd.profile_reset()
+        lddmm_amd.profile_enable(True)
+        for _ in range(2):
+            B.objgrd(alps)
+        pr = lddmm_amd.profile_read()
+        lddmm_amd.profile_enable(False)
+        fw, bw = pr["fwd_pairs"][0] / 2, pr["bwd_pairs"][0] / 2
+        rows.append(dict(chunk=sh[0], kv_lanes=sh[1], bwd_lanes=sh[2], fwd_pairs_ms=fw, bwd_pairs_ms=bw,
+                         sum_ms=fw + bw))
+        print("launch shape %s: K.alpha %.2f ms, adjoint %.2f ms per evaluation" % (sh, fw, bw), file=sys.stderr,
+              flush=True)
+    B.close()
+    return dict(problems=count, n_range=[nmin, nmax], setup_seconds=t_setup,
+                tuned=dict(chunk=tuned[0], kv_lanes=tuned[1], bwd_lanes=tuned[2]), shapes=rows)
+
+
 def batch_bench(count, nmin, nmax, itr, seed=4, warm=True, batched_fp64=False, budget_s=None, t_start=None):
     """The batched-registration configuration: `count` problems of the
     uneven generator (vertex counts log-uniform in [nmin, nmax]; 256 in
@@ -325,6 +361,7 @@ def batch_bench(count, nmin, nmax, itr, seed=4, warm=True, batched_fp64=False, b
     t0 = time.perf_counter()
     B = make(probs, "fp32")
     chunk, pairs = B.info()
+    tuning = B.tuning()
     t_setup = time.perf_counter() - t0
     a32, i32, r32 = B.run(3 * itr, 0.0, 0.0, 0.5, 5, precision="fp32", targets=targets, progress=100)
     t32 = i32[0]["seconds"] + t_setup
@@ -334,6 +371,7 @@ def batch_bench(count, nmin, nmax, itr, seed=4, warm=True, batched_fp64=False, b
     rel = (f64 - e_ref) / np.abs(e_ref)
     f_plain, _ = B64.objgrd(a_plain)   # consistency of the two FP64 evaluations
     out = dict(problems=count, n_range=[int(min(B.n)), int(max(B.n))], chunk=chunk,
+               fp32_targets_per_lane=dict(kv=tuning[1], bwd=tuning[2]),
                pairs_per_eval_total=float(pairs.sum()), batch_setup_seconds=t_setup,
                plain_fp64_sequential=dict(seconds=t_seq, registrations_per_s=count / t_seq,
                                           median_iterations=float(np.median([i["iterations"] for i in seq])),
@@ -498,6 +536,8 @@ def main():
                     help="only the batched FP32 registration of the uneven batch (for kernel traces)")
     ap.add_argument("--culling-report", action="store_true",
                     help="only the pair-kernel times with culling off and on")
+    ap.add_argument("--launch-shapes", action="store_true",
+                    help="only the batched FP32 pair-kernel times of the uneven batch at each launch shape")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: there is no multi-GPU path (problems would shard by "
@@ -509,6 +549,9 @@ def main():
         return
     if args.culling_report:
         print(json.dumps(culling_report()), flush=True)
+        return
+    if args.launch_shapes:
+        print(json.dumps(launch_shapes_report(args.batch, args.batch_nmin, args.batch_nmax)), flush=True)
         return
     if args.batch_profile:
         probs = G.batch_problems(args.batch, 4, args.batch_nmin, args.batch_nmax)

@@ -497,6 +497,8 @@ class Batch:
                                           C.c_int, C.POINTER(Options), _dp, C.POINTER(Result), C.POINTER(C.c_int)]
             L.lddmm_batch_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), _dp]
             L.lddmm_batch_set_graphs.argtypes = [C.c_void_p, C.c_int]
+            L.lddmm_batch_tuning.argtypes = [C.c_void_p] + [C.POINTER(C.c_int)] * 3
+            L.lddmm_batch_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
             L.lddmm_batch_destroy.argtypes = [C.c_void_p]
             L.lddmm_batch_destroy.restype = None
             L.lddmm_batch_create._set = True
@@ -558,6 +560,17 @@ class Batch:
         pairs = np.empty(self.P)
         _check(lib().lddmm_batch_info(self._h, C.byref(ch), _p(pairs)))
         return ch.value, pairs
+
+    def tuning(self):
+        """(chunk, kv_lanes, bwd_lanes): source points per work item and FP32
+        targets per lane of the K.alpha and adjoint pair launches."""
+        v = [C.c_int() for _ in range(3)]
+        _check(lib().lddmm_batch_tuning(self._h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def set_tuning(self, chunk, kv_lanes, bwd_lanes):
+        """Fix the launch shape (see tuning)."""
+        _check(lib().lddmm_batch_set_tuning(self._h, int(chunk), int(kv_lanes), int(bwd_lanes)))
 
     def close(self):
         if getattr(self, "_h", None):

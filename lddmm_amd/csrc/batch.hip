@@ -11,9 +11,12 @@
 // keeps the whole chip busy.  All sums are in a fixed order: results do not
 // depend on which other problems share the launch.
 #include <algorithm>
+#include <array>
+#include <tuple>
 #include <chrono>
 #include <cfloat>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -91,6 +94,16 @@ struct BatchView {
     double4 *grpv, *grpe, *grpt;
     int gpts, gvf;
 };
+
+// x[i] = scale * u_i with u_i a fixed pseudo-random value in [-1/2, 1/2):
+// stand-in momenta and adjoints for timing launches
+__global__ __launch_bounds__(kTile) void kb_fill_pattern(double *__restrict__ x, long long n, double scale)
+{
+    for (long long i = (long long)blockIdx.x * kTile + threadIdx.x; i < n; i += (long long)gridDim.x * kTile) {
+        const unsigned h = (unsigned)(i * 2654435761ULL) ^ (unsigned)(i >> 16);
+        x[i] = scale * ((double)(h >> 8) * (1.0 / 16777216.0) - 0.5);
+    }
+}
 
 // bounds of the aligned 64-point groups: q_t (which = 0), the deformed
 // element centres (1) or the target centres (2), one group per item
@@ -646,6 +659,18 @@ __global__ void kb_store_ring(const double *__restrict__ dots, const int *__rest
     }
 }
 
+// a runtime number of targets per lane (1, 2 or 4) onto a template lambda
+template <typename F>
+void with_lanes(int r, F &&f)
+{
+    switch (r) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    default: throw Error(kErrArg, "targets per lane must be 1, 2 or 4");
+    }
+}
+
 int cen_code(char t) { return t == 'G' ? 0 : 1; }
 int dir_code(char t) { return t == 'B' ? 0 : t == 'L' ? 1 : t == 'O' ? 2 : 3; }
 
@@ -835,6 +860,8 @@ Batch::Batch(const BatchSpec &spec, const double *q0, const int *elm, const doub
 
     chunk_ = spec_.chunk > 0 ? ((spec_.chunk + 63) / 64) * 64 : 1024;
     vchunk_ = 1024;
+    rkv_ = kKvR32;
+    rbwd_ = bwd_r32(spec_.order);
     build_descs();
     size_t part = 0, vpart = 0;
     for (int p = 0; p < P; ++p) {
@@ -847,6 +874,7 @@ Batch::Batch(const BatchSpec &spec, const double *q0, const int *elm, const doub
     size_t part_max = 0;
     for (int p = 0; p < P; ++p) part_max += (size_t)ceil_div(spec_.n[p], 256) * 2 * D * spec_.n[p];
     part = std::max(part, part_max);
+    part_cap_ = part;
 
     slab_.reserve(&desc_, P);
     slab_.reserve(&q0_, hq.size());
@@ -950,8 +978,8 @@ void Batch::build_tables(const std::vector<int> &prec_of)
         const int g = pr == kFP32 ? 1 : 0;
         const BatchDesc &d = desc_h_[p];
         probs.push_back(p);
-        // FP32 pair kernels keep kKvR32 / bwd_r32(order) targets per lane
-        const int tkv = g ? 64 * kKvR32 : 64, tbwd = g ? 64 * bwd_r32(spec_.order) : 64;
+        // FP32 pair kernels keep rkv_ / rbwd_ targets per lane (autotuned)
+        const int tkv = g ? 64 * rkv_ : 64, tbwd = g ? 64 * rbwd_ : 64;
         for (int s = 0; s < d.splits; ++s) {
             for (int t = 0; t < ceil_div(d.n, tkv); ++t) kv[g].push_back(make_int4(p, t, s, 0));
             for (int t = 0; t < ceil_div(d.n, tbwd); ++t) bwd[g].push_back(make_int4(p, t, s, 0));
@@ -1113,8 +1141,10 @@ void Batch::launch_eval(const double *alp, double *grd, double *f_dev)
             each_prec([&](int g, auto Pc) {
                 if (!tb.nkv[g]) return;
                 dispatch([&](auto Dc, auto Oc) {
-                    constexpr int R = decltype(Pc)::fp32 ? kKvR32 : 1;
-                    kb_kv<Dc(), Oc(), decltype(Pc), R><<<tb.nkv[g], kBlock, 0, st_>>>(v, t, tb.kv[g].p);
+                    with_lanes(decltype(Pc)::fp32 ? rkv_ : 1, [&](auto Rc) {
+                        if constexpr (decltype(Pc)::fp32 || Rc() == 1)
+                            kb_kv<Dc(), Oc(), decltype(Pc), Rc()><<<tb.nkv[g], kBlock, 0, st_>>>(v, t, tb.kv[g].p);
+                    });
                 });
                 LDDMM_LAUNCH_CHECK();
             });
@@ -1170,8 +1200,10 @@ void Batch::launch_eval(const double *alp, double *grd, double *f_dev)
             each_prec([&](int g, auto Pc) {
                 if (!tb.nbwd[g]) return;
                 dispatch([&](auto Dc, auto Oc) {
-                    constexpr int R = decltype(Pc)::fp32 ? bwd_r32(Oc()) : 1;
-                    kb_bwd<Dc(), Oc(), decltype(Pc), R><<<tb.nbwd[g], kBlock, 0, st_>>>(v, t, tb.bwd[g].p);
+                    with_lanes(decltype(Pc)::fp32 ? rbwd_ : 1, [&](auto Rc) {
+                        if constexpr ((decltype(Pc)::fp32 && Rc() <= 2) || Rc() == 1)
+                            kb_bwd<Dc(), Oc(), decltype(Pc), Rc()><<<tb.nbwd[g], kBlock, 0, st_>>>(v, t, tb.bwd[g].p);
+                    });
                 });
                 LDDMM_LAUNCH_CHECK();
             });
@@ -1183,88 +1215,176 @@ void Batch::launch_eval(const double *alp, double *grd, double *f_dev)
     }
 }
 
-// Chunk autotuner: times one batched forward pair launch per candidate
-// source chunk (the work-item size) on this batch and keeps the fastest.
-// Results are cached per batch signature for the life of the process.
+void Batch::upload_descs()
+{
+    LDDMM_HIP(hipMemcpyAsync(desc_, desc_h_.data(), desc_h_.size() * sizeof(BatchDesc), hipMemcpyHostToDevice, st_));
+    LDDMM_HIP(hipStreamSynchronize(st_));
+    tab_key_.clear();   // the work tables depend on the chunk and the targets per lane
+}
+
+void Batch::set_tuning(int chunk, int kv_lanes, int bwd_lanes)
+{
+    if (chunk <= 0) throw Error(kErrArg, "source chunk must be positive");
+    if (kv_lanes != 1 && kv_lanes != 2 && kv_lanes != 4)
+        throw Error(kErrArg, "K.alpha targets per lane must be 1, 2 or 4");
+    if (bwd_lanes != 1 && bwd_lanes != 2) throw Error(kErrArg, "adjoint targets per lane must be 1 or 2");
+    const int c = ((chunk + 63) / 64) * 64;
+    size_t need = 0;
+    for (int p = 0; p < spec_.P(); ++p) need += (size_t)ceil_div(spec_.n[p], c) * 2 * spec_.dim * spec_.n[p];
+    if (need > part_cap_) throw Error(kErrArg, "source chunk too small for the partial-sum buffer");
+    chunk_ = c;
+    rkv_ = kv_lanes;
+    rbwd_ = bwd_lanes;
+    build_descs();
+    upload_descs();
+}
+
+// Autotuner: for each candidate source chunk (the work-item size) and, in
+// FP32, each number of targets per lane (K.alpha 1, 2, 4; adjoint 1, 2),
+// times the batched K.alpha and adjoint pair launches of the first time
+// slice on this batch, with the culling state of the run, and keeps the
+// chunk whose best K.alpha plus best adjoint time is smallest, with those
+// two lane counts.  Results are cached per batch signature and culling
+// state for the life of the process.
 void Batch::autotune()
 {
+    using Key = std::tuple<int, int, int, int, long long, long long, int>;
     static std::mutex mu;
-    static std::map<std::tuple<int, int, int, int, long long, long long>, int> cache;
+    static std::map<Key, std::array<int, 3>> cache;
     long long sumsq = 0;
     for (int n : spec_.n) sumsq += (long long)n * n;
-    const auto key = std::make_tuple(spec_.dim, spec_.order, spec_.prec, spec_.P(), ntot_, sumsq);
+    const int cull_key = 2 * pair_cull_bits() + (pair_groups() ? 1 : 0);
+    const Key key = std::make_tuple(spec_.dim, spec_.order, spec_.prec, spec_.P(), ntot_, sumsq, cull_key);
     {
         std::lock_guard<std::mutex> g(mu);
         auto it = cache.find(key);
         if (it != cache.end()) {
-            chunk_ = it->second;
+            chunk_ = it->second[0];
+            rkv_ = it->second[1];
+            rbwd_ = it->second[2];
             build_descs();
-            LDDMM_HIP(hipMemcpyAsync(desc_, desc_h_.data(), desc_h_.size() * sizeof(BatchDesc),
-                                     hipMemcpyHostToDevice, st_));
-            LDDMM_HIP(hipStreamSynchronize(st_));
-            tab_key_.clear();
+            upload_descs();
             return;
         }
     }
     const int cands[] = {256, 512, 1024, 2048};
     const int P = spec_.P(), D = spec_.dim;
+    const bool f32 = spec_.prec == kFP32;
+    const int g = f32 ? 1 : 0;
     std::vector<int> all(P, spec_.prec);
     DArr<double> alp;
     alp.ensure((size_t)vtot_);
-    LDDMM_HIP(hipMemsetAsync(alp.p, 0, (size_t)vtot_ * sizeof(double), st_));
+    // nonzero stand-in momenta and adjoints: launch times under load depend
+    // on the operands (the clock follows the power drawn), so zeros mislead
+    kb_fill_pattern<<<1024, kTile, 0, st_>>>(alp.p, vtot_, 1e-2);
+    kb_fill_pattern<<<1024, kTile, 0, st_>>>(padj_, (long long)D * ntot_, 1.0);
+    LDDMM_LAUNCH_CHECK();
     for (int p = 0; p < P; ++p)
         LDDMM_HIP(hipMemcpyAsync(nde_ + (size_t)spec_.T * D * vo_[p], q0_ + (size_t)D * vo_[p],
                                  (size_t)D * spec_.n[p] * sizeof(double), hipMemcpyDeviceToDevice, st_));
+    BatchView v{desc_, nde_, padj_, part_, vpart_, alp.p, nullptr, q0_, tc_, tu_, ta_, elm_, ifo_, rows_,
+                spec_.T, spec_.h, spec_.w, ecen_, edir_, evol_, dc_, dd_, efwd_, evf_, pair_cull_bits(),
+                pair_expansion() ? 1 : 0, nullptr, grpv_, grpe_, grpt_,
+                pair_groups() && spec_.order == -1 ? 1 : 0, 0};
     hipEvent_t e0, e1;
     LDDMM_HIP(hipEventCreate(&e0));
     LDDMM_HIP(hipEventCreate(&e1));
-    float best = 1e30f;
-    int best_chunk = chunk_;
-    for (int c : cands) {
-        chunk_ = c;
-        build_descs();
-        LDDMM_HIP(hipMemcpyAsync(desc_, desc_h_.data(), P * sizeof(BatchDesc), hipMemcpyHostToDevice, st_));
-        tab_key_.clear();
-        build_tables(all);
-        const Tables &tb = *tab_;
-        const int g = spec_.prec == kFP32 ? 1 : 0;
-        BatchView v{desc_, nde_, padj_, part_, vpart_, alp.p, nullptr, q0_, tc_, tu_, ta_, elm_, ifo_, rows_,
-                    spec_.T, spec_.h, spec_.w, ecen_, edir_, evol_, dc_, dd_, efwd_, evf_, pair_cull_bits(),
-                    pair_expansion() ? 1 : 0, nullptr};
-        auto launch = [&] {
-            dispatch_order(spec_.order, [&](auto O) {
-                if (g) {
-                    if (D == 2) kb_kv<2, O(), PrecF32, kKvR32><<<tb.nkv[1], kBlock, 0, st_>>>(v, 0, tb.kv[1].p);
-                    else kb_kv<3, O(), PrecF32, kKvR32><<<tb.nkv[1], kBlock, 0, st_>>>(v, 0, tb.kv[1].p);
-                } else {
-                    if (D == 2) kb_kv<2, O(), PrecF64, 1><<<tb.nkv[0], kBlock, 0, st_>>>(v, 0, tb.kv[0].p);
-                    else kb_kv<3, O(), PrecF64, 1><<<tb.nkv[0], kBlock, 0, st_>>>(v, 0, tb.kv[0].p);
-                }
-            });
-            LDDMM_LAUNCH_CHECK();
-        };
-        launch();   // warm
-        LDDMM_HIP(hipEventRecord(e0, st_));
+    // the shortest of three single-launch times, after one untimed launch
+    auto timed = [&](auto &&launch) {
         launch();
-        launch();
-        LDDMM_HIP(hipEventRecord(e1, st_));
-        LDDMM_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
-        LDDMM_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) {
-            best = ms;
-            best_chunk = c;
+        float best_ms = 1e30f;
+        for (int rep = 0; rep < 3; ++rep) {
+            LDDMM_HIP(hipEventRecord(e0, st_));
+            launch();
+            LDDMM_HIP(hipEventRecord(e1, st_));
+            LDDMM_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            LDDMM_HIP(hipEventElapsedTime(&ms, e0, e1));
+            best_ms = std::min(best_ms, ms);
         }
-    }
+        return best_ms;
+    };
+    const bool log = std::getenv("LDDMM_BATCH_TUNE_LOG") && std::atoi(std::getenv("LDDMM_BATCH_TUNE_LOG")) != 0;
+    auto dispatch = [&](auto &&fn) {
+        dispatch_order(spec_.order, [&](auto O) {
+            if (D == 2) fn(std::integral_constant<int, 2>{}, O);
+            else fn(std::integral_constant<int, 3>{}, O);
+        });
+    };
+    auto each = [&](auto Pc) {
+        float best = 1e30f;
+        int best_c = chunk_, best_kv = rkv_, best_bwd = rbwd_;
+        bool bounds = false;
+        for (int c : cands) {
+            chunk_ = c;
+            build_descs();
+            upload_descs();
+            float t_kv = 1e30f, t_bwd = 1e30f;
+            int r_kv = 1, r_bwd = 1;
+            for (int r : {1, 2, 4}) {
+                if (!f32 && r > 1) break;
+                rkv_ = r;
+                rbwd_ = std::min(r, 2);
+                tab_key_.clear();
+                build_tables(all);
+                const Tables &tb = *tab_;
+                if (v.gpts && f32 && !bounds) {   // group bounds of q_0 for the culling
+                    if (D == 2) kb_group_bounds<2><<<tb.nvtile, kTile, 0, st_>>>(v, 0, 0, tb.vtile.p);
+                    else kb_group_bounds<3><<<tb.nvtile, kTile, 0, st_>>>(v, 0, 0, tb.vtile.p);
+                    LDDMM_LAUNCH_CHECK();
+                    bounds = true;
+                }
+                const float kv = timed([&] {
+                    dispatch([&](auto Dc, auto Oc) {
+                        with_lanes(r, [&](auto Rc) {
+                            if constexpr (decltype(Pc)::fp32 || Rc() == 1)
+                                kb_kv<Dc(), Oc(), decltype(Pc), Rc()><<<tb.nkv[g], kBlock, 0, st_>>>(v, 0,
+                                                                                                    tb.kv[g].p);
+                        });
+                    });
+                    LDDMM_LAUNCH_CHECK();
+                });
+                if (kv < t_kv) {
+                    t_kv = kv;
+                    r_kv = r;
+                }
+                if (log) std::fprintf(stderr, "batch tune: chunk %d, K.alpha lanes %d: %.3f ms\n", c, r, kv);
+                if (r > 2) continue;
+                const float bw = timed([&] {
+                    dispatch([&](auto Dc, auto Oc) {
+                        with_lanes(r, [&](auto Rc) {
+                            if constexpr ((decltype(Pc)::fp32 && Rc() <= 2) || Rc() == 1)
+                                kb_bwd<Dc(), Oc(), decltype(Pc), Rc()><<<tb.nbwd[g], kBlock, 0, st_>>>(v, 0,
+                                                                                                      tb.bwd[g].p);
+                        });
+                    });
+                    LDDMM_LAUNCH_CHECK();
+                });
+                if (bw < t_bwd) {
+                    t_bwd = bw;
+                    r_bwd = r;
+                }
+                if (log) std::fprintf(stderr, "batch tune: chunk %d, adjoint lanes %d: %.3f ms\n", c, r, bw);
+            }
+            if (t_kv + t_bwd < best) {
+                best = t_kv + t_bwd;
+                best_c = c;
+                best_kv = f32 ? r_kv : kKvR32;
+                best_bwd = f32 ? r_bwd : bwd_r32(spec_.order);
+            }
+        }
+        chunk_ = best_c;
+        rkv_ = best_kv;
+        rbwd_ = best_bwd;
+    };
+    if (f32) each(PrecF32{});
+    else each(PrecF64{});
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    chunk_ = best_chunk;
     build_descs();
-    LDDMM_HIP(hipMemcpyAsync(desc_, desc_h_.data(), P * sizeof(BatchDesc), hipMemcpyHostToDevice, st_));
-    LDDMM_HIP(hipStreamSynchronize(st_));
-    tab_key_.clear();
+    upload_descs();
     std::lock_guard<std::mutex> gl(mu);
-    cache[key] = chunk_;
+    cache[key] = {chunk_, rkv_, rbwd_};
 }
 
 std::vector<double> Batch::to_device_layout(const double *h) const

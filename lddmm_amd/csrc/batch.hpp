@@ -82,6 +82,14 @@ public:
     long long varnum(int p) const { return (long long)spec_.dim * (spec_.T - 1) * spec_.n[p]; }
     const BatchSpec &spec() const { return spec_; }
     int chunk() const { return chunk_; }
+    // targets per lane of the FP32 K.alpha and adjoint launches (autotuned)
+    int kv_lanes() const { return rkv_; }
+    int bwd_lanes() const { return rbwd_; }
+    // fixes the source chunk (rounded up to a multiple of 64; the partial
+    // buffer holds chunks down to min(256, the chunk at creation)) and the
+    // FP32 targets per lane (K.alpha 1, 2 or 4; adjoint 1 or 2); the stream
+    // must be idle
+    void set_tuning(int chunk, int kv_lanes, int bwd_lanes);
     double pairs_per_eval(int p) const;
 
     // f_dev[p], grd (device, concatenated per-problem [T-1][D][N_p] blocks)
@@ -102,7 +110,9 @@ private:
     BatchSpec spec_;
     hipStream_t st_;
     int chunk_ = 1024, vchunk_ = 1024, rows_ = 1;
+    int rkv_ = 2, rbwd_ = 2;   // FP32 targets per lane: K.alpha (1, 2, 4), adjoint (1, 2)
     long long ntot_ = 0, etot_ = 0, ttot_ = 0, vtot_ = 0;
+    size_t part_cap_ = 0;   // doubles in part_
     std::vector<long long> vo_, eo_, to_;
     std::vector<int> vperm_;   // per problem: internal slot -> caller's vertex (local indices)
     std::vector<BatchDesc> desc_h_;
@@ -129,6 +139,7 @@ private:
     void clear_graphs();
     void launch_eval(const double *alp, double *grd, double *f_dev);
     void build_descs();
+    void upload_descs();
     void build_tables(const std::vector<int> &prec_of);
     void compute_target_terms();
     void autotune();

@@ -676,6 +676,27 @@ int lddmm_batch_info(lddmm_batch *b, int *chunk, double *pairsPerEval)
     });
 }
 
+int lddmm_batch_tuning(lddmm_batch *b, int *chunk, int *kvLanes, int *bwdLanes)
+{
+    return guarded([&] {
+        require(b != nullptr, "null argument");
+        if (chunk) *chunk = b->b->chunk();
+        if (kvLanes) *kvLanes = b->b->kv_lanes();
+        if (bwdLanes) *bwdLanes = b->b->bwd_lanes();
+        return 0;
+    });
+}
+
+int lddmm_batch_set_tuning(lddmm_batch *b, int chunk, int kvLanes, int bwdLanes)
+{
+    return guarded([&] {
+        require(b != nullptr, "null argument");
+        b->st.sync();
+        b->b->set_tuning(chunk, kvLanes, bwdLanes);
+        return 0;
+    });
+}
+
 int lddmm_batch_set_graphs(lddmm_batch *b, int on)
 {
     return guarded([&] {

@@ -64,6 +64,51 @@ def test_batched_fp32_objgrd_error(gpu):
         obj.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_every_launch_shape_gives_the_same_objgrd(gpu, precision):
+    """The autotuned launch shape (source chunk, FP32 targets per lane of the
+    K.alpha and adjoint launches) is one of the candidates, and every shape
+    gives the same energies and gradients.  In FP64 the chunk changes only
+    the order in which split sums are added.  In FP32 the targets per lane
+    also set the target tile, whose centre anchors the FP32 offsets and
+    whose bounds decide the grouped culling, so shapes differ at FP32
+    rounding: each stays within the FP32 bounds of the FP64 values."""
+    probs, k = small_batch(), kernels()
+    B = gpu.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"], k["knl_order"],
+                  k["time_stp"], k["time_num"], precision=precision, chunk=0)
+    ch, kv, bw = B.tuning()
+    print("%s autotuned shape: chunk %d, K.alpha lanes %d, adjoint lanes %d" % (precision, ch, kv, bw), flush=True)
+    assert ch in (256, 512, 1024, 2048) and kv in (1, 2, 4) and bw in (1, 2)
+    alps = [0.05 * G.normal(70 + i, B.sizes[i]).reshape(k["time_num"] - 1, B.n[i], 3) for i in range(B.P)]
+    f0, g0 = B.objgrd(alps)
+    ref = []
+    if precision == "fp32":
+        for p, a in zip(probs, alps):
+            obj, _ = single(gpu, p, k)
+            ref.append(obj(a))
+            obj.close()
+    for shape in [(256, 1, 1), (1024, 2, 2), (512, 4, 1), (2048, 4, 2), (64, 1, 2)]:
+        if shape[0] < 256:
+            with pytest.raises(gpu.LddmmError):   # the partial buffer holds chunks down to 256
+                B.set_tuning(*shape)
+            continue
+        B.set_tuning(*shape)
+        assert B.tuning() == shape
+        f, g = B.objgrd(alps)
+        for i in range(B.P):
+            e_rel, g_rel = abs(f[i] - f0[i]) / abs(f0[i]), rel_err(g[i].reshape(-1), g0[i].reshape(-1))
+            if precision == "fp64":
+                assert e_rel <= 1e-11 and g_rel <= 1e-10, (shape, i, e_rel, g_rel)
+            else:
+                assert e_rel <= 1e-7 and g_rel <= 1e-5, (shape, i, e_rel, g_rel)
+                fd, gd = ref[i]
+                assert abs(f[i] - fd) <= 1e-6 * abs(fd) and rel_err(g[i].reshape(-1), gd) <= 1e-4, (shape, i)
+    for bad in [(256, 3, 1), (256, 2, 4), (0, 1, 1)]:
+        with pytest.raises(gpu.LddmmError):
+            B.set_tuning(*bad)
+    B.close()
+
+
 @pytest.mark.parametrize("mem,itr", [(5, 8), (1, 6), (12, 16)])
 def test_batched_run_matches_single_runs(gpu, mem, itr):
     """The batched L-BFGS (compact form of the two-loop recursion, dot
