@@ -199,6 +199,8 @@ __global__ __launch_bounds__(kBlock) void kb_bwd(BatchView v, int t, const int4 
     pairs_block<Op>(A, d.n, it.y, j0, j1, v.part + d.po + (size_t)it.z * 2 * D * d.n, smem);
 }
 
+#include "mx_bwd.hpp"
+
 template <int D>
 __global__ __launch_bounds__(kTile) void kb_fwd_reduce(BatchView v, int t, const int4 *__restrict__ items)
 {
@@ -1203,6 +1205,8 @@ void Batch::launch_eval(const double *alp, double *grd, double *f_dev)
                     with_lanes(decltype(Pc)::fp32 ? rbwd_ : 1, [&](auto Rc) {
                         if constexpr ((decltype(Pc)::fp32 && Rc() <= 2) || Rc() == 1)
                             kb_bwd<Dc(), Oc(), decltype(Pc), Rc()><<<tb.nbwd[g], kBlock, 0, st_>>>(v, t, tb.bwd[g].p);
+                        else if constexpr (decltype(Pc)::fp32 && Rc() == 4 && Dc() == 3 && Oc() == -1)
+                            kb_bwd_mx<3><<<tb.nbwd[g], kBlock, 0, st_>>>(v, t, tb.bwd[g].p);
                     });
                 });
                 LDDMM_LAUNCH_CHECK();
@@ -1227,7 +1231,9 @@ void Batch::set_tuning(int chunk, int kv_lanes, int bwd_lanes)
     if (chunk <= 0) throw Error(kErrArg, "source chunk must be positive");
     if (kv_lanes != 1 && kv_lanes != 2 && kv_lanes != 4)
         throw Error(kErrArg, "K.alpha targets per lane must be 1, 2 or 4");
-    if (bwd_lanes != 1 && bwd_lanes != 2) throw Error(kErrArg, "adjoint targets per lane must be 1 or 2");
+    if (bwd_lanes != 1 && bwd_lanes != 2 && !(bwd_lanes == 4 && mx_ok()))
+        throw Error(kErrArg, "adjoint targets per lane must be 1 or 2 (4: the matrix-core adjoint of 3D Gaussian "
+                             "batches)");
     const int c = ((chunk + 63) / 64) * 64;
     size_t need = 0;
     for (int p = 0; p < spec_.P(); ++p) need += (size_t)ceil_div(spec_.n[p], c) * 2 * spec_.dim * spec_.n[p];
@@ -1267,7 +1273,7 @@ void Batch::autotune()
             return;
         }
     }
-    const int cands[] = {256, 512, 1024, 2048};
+    const int cands[] = {256, 512, 1024, 2048, 4096};
     const int P = spec_.P(), D = spec_.dim;
     const bool f32 = spec_.prec == kFP32;
     const int g = f32 ? 1 : 0;
@@ -1324,7 +1330,7 @@ void Batch::autotune()
             for (int r : {1, 2, 4}) {
                 if (!f32 && r > 1) break;
                 rkv_ = r;
-                rbwd_ = std::min(r, 2);
+                rbwd_ = r > 2 && !mx_ok() ? 2 : r;
                 tab_key_.clear();
                 build_tables(all);
                 const Tables &tb = *tab_;
@@ -1349,13 +1355,15 @@ void Batch::autotune()
                     r_kv = r;
                 }
                 if (log) std::fprintf(stderr, "batch tune: chunk %d, K.alpha lanes %d: %.3f ms\n", c, r, kv);
-                if (r > 2) continue;
+                if (r > 2 && !mx_ok()) continue;
                 const float bw = timed([&] {
                     dispatch([&](auto Dc, auto Oc) {
                         with_lanes(r, [&](auto Rc) {
                             if constexpr ((decltype(Pc)::fp32 && Rc() <= 2) || Rc() == 1)
                                 kb_bwd<Dc(), Oc(), decltype(Pc), Rc()><<<tb.nbwd[g], kBlock, 0, st_>>>(v, 0,
                                                                                                       tb.bwd[g].p);
+                            else if constexpr (decltype(Pc)::fp32 && Rc() == 4 && Dc() == 3 && Oc() == -1)
+                                kb_bwd_mx<3><<<tb.nbwd[g], kBlock, 0, st_>>>(v, 0, tb.bwd[g].p);
                         });
                     });
                     LDDMM_LAUNCH_CHECK();

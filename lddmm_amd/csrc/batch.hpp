@@ -110,7 +110,7 @@ private:
     BatchSpec spec_;
     hipStream_t st_;
     int chunk_ = 1024, vchunk_ = 1024, rows_ = 1;
-    int rkv_ = 2, rbwd_ = 2;   // FP32 targets per lane: K.alpha (1, 2, 4), adjoint (1, 2)
+    int rkv_ = 2, rbwd_ = 2;   // FP32 targets per lane: K.alpha (1, 2, 4), adjoint (1, 2; 4 = matrix cores)
     long long ntot_ = 0, etot_ = 0, ttot_ = 0, vtot_ = 0;
     size_t part_cap_ = 0;   // doubles in part_
     std::vector<long long> vo_, eo_, to_;
@@ -143,6 +143,8 @@ private:
     void build_tables(const std::vector<int> &prec_of);
     void compute_target_terms();
     void autotune();
+    // the matrix-core adjoint (four targets per lane's tiles of 256) applies
+    bool mx_ok() const { return spec_.dim == 3 && spec_.order == -1; }
 };
 
 }  // namespace lddmm

@@ -36,7 +36,7 @@ def test_batched_objgrd_matches_single_problems(gpu, chunk):
     B = gpu.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"], k["knl_order"],
                   k["time_stp"], k["time_num"], chunk=chunk)
     ch, pairs = B.info()
-    assert ch in ((256, 512, 1024, 2048) if chunk == 0 else (chunk,))
+    assert ch in ((256, 512, 1024, 2048, 4096) if chunk == 0 else (chunk,))
     alps = [0.05 * G.normal(40 + i, B.sizes[i]).reshape(k["time_num"] - 1, B.n[i], 3) for i in range(B.P)]
     f, g = B.objgrd(alps)
     for i, p in enumerate(probs):
@@ -78,7 +78,7 @@ def test_every_launch_shape_gives_the_same_objgrd(gpu, precision):
                   k["time_stp"], k["time_num"], precision=precision, chunk=0)
     ch, kv, bw = B.tuning()
     print("%s autotuned shape: chunk %d, K.alpha lanes %d, adjoint lanes %d" % (precision, ch, kv, bw), flush=True)
-    assert ch in (256, 512, 1024, 2048) and kv in (1, 2, 4) and bw in (1, 2)
+    assert ch in (256, 512, 1024, 2048, 4096) and kv in (1, 2, 4) and bw in (1, 2, 4)
     alps = [0.05 * G.normal(70 + i, B.sizes[i]).reshape(k["time_num"] - 1, B.n[i], 3) for i in range(B.P)]
     f0, g0 = B.objgrd(alps)
     ref = []
@@ -87,7 +87,9 @@ def test_every_launch_shape_gives_the_same_objgrd(gpu, precision):
             obj, _ = single(gpu, p, k)
             ref.append(obj(a))
             obj.close()
-    for shape in [(256, 1, 1), (1024, 2, 2), (512, 4, 1), (2048, 4, 2), (64, 1, 2)]:
+    # adjoint lanes 4: the matrix-core adjoint of 3D Gaussian batches (FP32),
+    # with its vector fallback for the tiles of the small problems
+    for shape in [(256, 1, 1), (1024, 2, 2), (512, 4, 1), (2048, 4, 2), (1024, 2, 4), (4096, 1, 4), (64, 1, 2)]:
         if shape[0] < 256:
             with pytest.raises(gpu.LddmmError):   # the partial buffer holds chunks down to 256
                 B.set_tuning(*shape)
@@ -103,7 +105,7 @@ def test_every_launch_shape_gives_the_same_objgrd(gpu, precision):
                 assert e_rel <= 1e-7 and g_rel <= 1e-5, (shape, i, e_rel, g_rel)
                 fd, gd = ref[i]
                 assert abs(f[i] - fd) <= 1e-6 * abs(fd) and rel_err(g[i].reshape(-1), gd) <= 1e-4, (shape, i)
-    for bad in [(256, 3, 1), (256, 2, 4), (0, 1, 1)]:
+    for bad in [(256, 3, 1), (256, 2, 3), (0, 1, 1)]:
         with pytest.raises(gpu.LddmmError):
             B.set_tuning(*bad)
     B.close()
