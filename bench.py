@@ -287,8 +287,9 @@ def launch_shapes_report(count, nmin, nmax, seed=4):
     """FP32 K.alpha and adjoint pair-kernel time per evaluation of the
     uneven batch (HIP events around each launch group, direct launches) at
     the autotuned launch shape and at every shape of the grid source chunk
-    {512, 1024, 2048} x targets per lane {(1, 1), (2, 2), (2, 1), (4, 2),
-    (4, 1)}, at momenta 0.02 x N(0, 1)."""
+    {1024, 2048, 4096} x targets per lane (K.alpha, adjoint) {(1, 1), (2, 2),
+    (2, 1), (4, 2), (1, 4), (2, 4), (4, 4)} (adjoint 4: the matrix-core
+    adjoint of 3D Gaussian batches), at momenta 0.02 x N(0, 1)."""
     probs = G.batch_problems(count, seed, nmin, nmax)
     k = G.BATCH_KERNELS
     T = k["time_num"]
@@ -300,7 +301,8 @@ def launch_shapes_report(count, nmin, nmax, seed=4):
     B.set_graphs(False)
     alps = [0.02 * G.normal(100 + i, B.sizes[i]).reshape(T - 1, B.n[i], 3) for i in range(B.P)]
     rows = []
-    for sh in [tuned] + [(c, kv, bw) for c in (512, 1024, 2048) for kv, bw in ((1, 1), (2, 2), (2, 1), (4, 2), (4, 1))]:
+    for sh in [tuned] + [(c, kv, bw) for c in (1024, 2048, 4096)
+                         for kv, bw in ((1, 1), (2, 2), (2, 1), (4, 2), (1, 4), (2, 4), (4, 4))]:
         B.set_tuning(*sh)
         B.objgrd(alps)
         lddmm_amd.profile_reset()
@@ -525,6 +527,8 @@ def main():
     ap.add_argument("--batch-nmin", type=int, default=500)
     ap.add_argument("--batch-nmax", type=int, default=20000)
     ap.add_argument("--batch-chunk", type=int, default=512, help="source chunk of --batch-profile")
+    ap.add_argument("--batch-lanes", default="2,2",
+                    help="FP32 targets per lane of --batch-profile, K.alpha,adjoint (adjoint 4: matrix cores)")
     ap.add_argument("--batched-fp64", action="store_true", help="also the batched FP64 run of the batch leg")
     ap.add_argument("--no-warm-start", action="store_true", help="skip the learned warm start of the batch leg")
     ap.add_argument("--budget", type=float, default=470.0,
@@ -561,13 +565,16 @@ def main():
         # timing launches can pick differently from pass to pass
         B = lddmm_amd.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"],
                             k["knl_order"], k["time_stp"], k["time_num"], precision="fp32", chunk=args.batch_chunk)
+        kv_lanes, bwd_lanes = (int(x) for x in args.batch_lanes.split(","))
+        B.set_tuning(args.batch_chunk, kv_lanes, bwd_lanes)
         chunk, pairs = B.info()
         # direct launches (as whenever rocprofv3 drives the process): its kernel
         # tracing has crashed the host process during hipGraph capture here;
         # graphs do not change kernel times
         B.set_graphs(False)
         _, info, rounds = B.run(300, 0.0, 0.0, 0.5, 5, precision="fp32", rel_energy_tol=1e-6)
-        print(json.dumps(dict(problems=args.batch, chunk=chunk, pairs_per_eval_total=float(pairs.sum()),
+        print(json.dumps(dict(problems=args.batch, chunk=chunk, lanes=[kv_lanes, bwd_lanes],
+                              pairs_per_eval_total=float(pairs.sum()),
                               seconds=info[0]["seconds"], rounds=rounds,
                               evaluations=int(sum(i["evaluations"] for i in info)))), flush=True)
         return

@@ -188,23 +188,107 @@ struct KernelFnT<ORDER, double, D> {
     }
 };
 
-// FP32, 2D Matern: the Bessel factors and the polynomial in FP64 from the
-// FP32 squared distance (unscaled), rounded to FP32.
+// FP32 Bessel factors p0 = x^2 K0(x), p1 = x K1(x) of the 2D Matern kernels.
+// 0 < x <= 1: the ascending series (A&S 9.6.13, 9.6.11) in t = x^2/4,
+//   K0 = -ln(x/2) I0 + sum_k psi(k+1) t^k/(k!)^2,
+//   x K1 = 1 + ln(x/2) x I1 - t sum_k (psi(k+1) + psi(k+2)) t^k/(k!(k+1)!),
+// x I1 = 2t sum_k t^k/(k!(k+1)!), through k = 5 (the next terms are below
+// 1e-9 at t = 1/4); the coefficients are those closed forms, rounded to FP32.
+// x > 1: p1 = sqrt(x) f1(u) e^-x and p0 = x sqrt(x) f0(u) e^-x with
+// f_n(u) = sqrt(x) e^x K_n(x), u = 1/x, polynomials of degree 10 in u fitted
+// by lddmm_amd/fit_bessel_f32.py (relative error 5e-9; 9e-8 with FP32
+// Horner).  x below 2^-60 gives the limits p0 = 0, p1 = 1.  Against scipy
+// over x in (0, 90], absolute error <= 1.7e-7 (tests/test_bessel_f32.py).
+__device__ __forceinline__ void bessel_p01_f32(float x, float &p0, float &p1)
+{
+    if (x <= 1.0f) {
+        const float t = 0.25f * x * x;
+        const float lg = (__builtin_amdgcn_logf(x) - 1.0f) * 0.693147181f;   // ln(x/2)
+        const float i0 = 1.0f + t * (1.0f + t * (2.5e-1f + t * (2.777777778e-2f + t * (1.736111111e-3f +
+                                                                                      t * 6.944444444e-5f))));
+        const float s0 = -5.772156649e-1f +
+                         t * (4.227843351e-1f + t * (2.306960838e-1f + t * (3.489215746e-2f + t * (2.614787619e-3f +
+                                                                                                 t * 1.184803936e-4f))));
+        const float i1 = 1.0f + t * (5.0e-1f + t * (8.333333333e-2f + t * (6.944444444e-3f + t * (3.472222222e-4f +
+                                                                                              t * 1.157407407e-5f))));
+        const float s1 = -1.544313298e-1f +
+                         t * (6.727843351e-1f + t * (1.815751670e-1f + t * (1.918218984e-2f + t * (1.115359492e-3f +
+                                                                                                 t * 4.142247689e-5f))));
+        const bool tiny = !(x >= 8.673617380e-19f);   // 2^-60, and x = 0 (log2 0 = -inf)
+        p0 = tiny ? 0.0f : 4.0f * t * (s0 - lg * i0);
+        p1 = tiny ? 1.0f : 1.0f + t * (2.0f * lg * i1 - s1);
+    } else {
+        const float u = __builtin_amdgcn_rcpf(x);
+        const float f0 =
+            1.253314137e+00f +
+            u * (-1.566625983e-01f +
+                 u * (8.804495633e-02f +
+                      u * (-9.030350298e-02f +
+                           u * (1.252581924e-01f +
+                                u * (-1.841103584e-01f +
+                                     u * (2.345902026e-01f +
+                                          u * (-2.255823016e-01f +
+                                               u * (1.470358670e-01f + u * (-5.692518502e-02f + u * 9.803697467e-03f)))))))));
+        const float f1 =
+            1.253314137e+00f +
+            u * (4.699910283e-01f +
+                 u * (-1.467864066e-01f +
+                      u * (1.268443316e-01f +
+                           u * (-1.633526385e-01f +
+                                u * (2.327853292e-01f +
+                                     u * (-2.936871648e-01f +
+                                          u * (2.822842300e-01f +
+                                               u * (-1.846503466e-01f + u * (7.185639441e-02f + u * -1.244540792e-02f)))))))));
+        const float se = __builtin_amdgcn_sqrtf(x) * __builtin_amdgcn_exp2f(-x * 1.442695041f);
+        p1 = se * f1;
+        p0 = x * se * f0;
+    }
+}
+
+// FP32, 2D Matern (Cpp/src/computeKernel.cpp:60-149, Cpp/src/dqKernel.cpp:
+// 91-185): coordinates scaled by kappa = 1/sigma, so d = sqrt(r2) and
+// d^2 = r2; k and the gradient factor from bessel_p01_f32, the constant of
+// each order's gradient factor (and -1/sigma^2) moved into cmul.
 template <int ORDER>
 struct KernelFnT<ORDER, float, 2> {
-    static constexpr bool kScaled = true;   // anchored (kappa = 1)
-    __host__ __device__ static double kappa(double, double) { return 1.0; }
-    __host__ __device__ static double cmul(double, double) { return 1.0; }
+    static constexpr bool kScaled = true;
+    __host__ __device__ static double kappa(double iw, double) { return iw; }
+    __host__ __device__ static double cmul(double, double iw2)
+    {
+        switch (ORDER) {
+        case 0: return 0.0;   // order 0 has no dq (Cpp/src/dqKernel.cpp:39)
+        case 1: return -0.5 * iw2;
+        case 2: return -0.125 * iw2;
+        case 3: return -iw2 / 48.0;
+        default: return -iw2 / 384.0;
+        }
+    }
+    __device__ __forceinline__ static void eval(float r2, float, float, float &k, float &cf)
+    {
+        float p0, p1;
+        bessel_p01_f32(__builtin_amdgcn_sqrtf(r2), p0, p1);
+        if constexpr (ORDER == 0) {
+            k = p1;
+            cf = 0.0f;
+        } else if constexpr (ORDER == 1) {
+            k = 0.5f * p0 + p1;
+            cf = p1;
+        } else if constexpr (ORDER == 2) {
+            k = 0.5f * p0 + (1.0f + 0.125f * r2) * p1;
+            cf = p0 + 2.0f * p1;
+        } else if constexpr (ORDER == 3) {
+            k = ((24.0f + r2) * p0 + (48.0f + 8.0f * r2) * p1) * (1.0f / 48.0f);
+            cf = 4.0f * p0 + (8.0f + r2) * p1;
+        } else {
+            k = ((192.0f + 12.0f * r2) * p0 + (384.0f + r2 * (72.0f + r2)) * p1) * (1.0f / 384.0f);
+            cf = (24.0f + r2) * p0 + (48.0f + 8.0f * r2) * p1;
+        }
+    }
     __device__ __forceinline__ static float value(float r2, float iw, float iw2)
     {
-        return (float)KernelFnD<2, ORDER>::value((double)r2, (double)iw, (double)iw2);
-    }
-    __device__ __forceinline__ static void eval(float r2, float iw, float iw2, float &k, float &cf)
-    {
-        double kd, cd;
-        KernelFnD<2, ORDER>::eval((double)r2, (double)iw, (double)iw2, kd, cd);
-        k = (float)kd;
-        cf = (float)cd;
+        float k, cf;
+        eval(r2, iw, iw2, k, cf);
+        return k;
     }
 };
 

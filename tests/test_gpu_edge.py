@@ -49,12 +49,15 @@ def batch_vs_single(gpu, probs, k, precision="fp64", tol_f=1e-12, tol_g=1e-11):
     B.close()
 
 
-@pytest.mark.parametrize("order", [-1, 2])
-def test_batch_of_curves(gpu, order):
+@pytest.mark.parametrize("order,precision", [(-1, "fp64"), (2, "fp64"), (3, "fp32")])
+def test_batch_of_curves(gpu, order, precision):
     k = dict(cen_type="G", cen_width=0.1, dir_type="B", dir_width=0.0, dis_wgt=100.0, knl_order=order,
              time_stp=0.2, time_num=6)
-    print("batch of 2D curves, order %d" % order, flush=True)
-    batch_vs_single(gpu, curve_problems(5, 31), k)
+    print("batch of 2D curves, order %d, %s" % (order, precision), flush=True)
+    # FP32 (2D Matern: FP32 Bessel factors): the batch's source chunks differ from
+    # the single-problem split, so the FP32 error budget applies
+    tol = (1e-12, 1e-11) if precision == "fp64" else (1e-6, 1e-4)
+    batch_vs_single(gpu, curve_problems(5, 31), k, precision=precision, tol_f=tol[0], tol_g=tol[1])
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])

@@ -182,8 +182,13 @@ def make_problem(kind):
     if kind.startswith("circle_square"):
         w = G.circle_square(200)
         w["q0"] = w["q0"] * 0.95
-        if kind == "circle_square_matern":
-            w["knl_order"] = 2   # 2D Matern nu = 3 (Bessel factors)
+        if kind.startswith("circle_square_matern"):
+            # 2D Matern nu = p + 1 (Bessel factors), order p = 2 unless the kind names it
+            w["knl_order"] = int(kind[-1]) if kind[-1].isdigit() else 2
+            if kind.endswith("example3"):
+                # the reference's circle->square example: order 3, width 0.025, 'C' x 'B'
+                # (Cpp/examples/circle2square/run_circle2square.m:31-39)
+                w["knl_width"], w["cen_type"], w["cen_width"] = 0.025, "C", 0.1
     elif kind == "sphere_cube":
         w = G.sphere_cube(subdivisions=2, cells=10)
     else:
@@ -245,7 +250,9 @@ def test_compute_node_position_matches_trajectory(gpu):
         assert rel_err(nde[t + 1], q) <= 1e-12
 
 
-@pytest.mark.parametrize("kind", ["circle_square", "circle_square_matern", "sphere_cube", "superquadric"])
+@pytest.mark.parametrize("kind", ["circle_square", "circle_square_matern", "circle_square_matern0",
+                                  "circle_square_matern1", "circle_square_matern3", "circle_square_matern4",
+                                  "circle_square_matern_example3", "sphere_cube", "superquadric"])
 def test_objgrd_fp32_policy_error(gpu, kind):
     """FP32 pair arithmetic (FP64 differences and cross-tile sums) against
     the FP64 oracle: relative error of the energy and of the gradient
