@@ -13,56 +13,90 @@
 //     sf_i = -p_i . P_a + a_i . P_w,   dq_i = x_i sf_i - (-p_i . Q_a + a_i . Q_w).
 //
 // Per block: 256 targets, one wave per 64 (four row tiles of 16).  Per 32
-// sources and row tile, the exponents t_ij = |x_j|^2 - 2 y_i . x_j - S come
-// from two v_mfma_f32_16x16x4f32 (sources x (x, |x|^2 - S) against targets
-// x (-2 y, 1); y are the targets' offsets from the wave's centroid, so the
-// expanded form keeps its error bound of |y| <= rt), the kernel values
-// k' = exp2(-t) = 2^S exp2(-|x_j|^2 + 2 y_i . x_j) from v_exp_f32, and the
+// sources and row tile, the exponents t_ij = |x_j'|^2 - 2 y_i . x_j' - S come
+// from two v_mfma_f32_16x16x4f32 (sources x (x', |x'|^2 - S) against targets
+// x (-2 y, 1); x' and y are offsets from the wave's centroid of its targets,
+// so the expanded form keeps its error bound of |y| <= rt), the kernel values
+// k' = exp2(-t) = 2^S exp2(-|x_j - x_i|^2 + |y_i|^2) from v_exp_f32, and the
 // products with the columns from v_mfma_f32_16x16x32_f16 on a split of both
 // factors into FP16 high and low halves (hi.hi + hi.lo + lo.hi: 22 bits of
-// each product, FP32 accumulation).  The wave's scale 2^S keeps k' below
-// 2^15, and power-of-two column scales from the chunk's maxima keep the
-// columns below 2^14, so both halves stay normal down to 2^-29 of the
-// largest value; the scales are powers of two from each staged tile's
-// maxima.  The FP32 sums of one staged tile (256 sources) are added to FP64
-// totals with the scales removed; exp2(-|y_i|^2) and the combination with
-// the target's own vectors happen once per block in FP64, one thread per
-// target.  With the group bounds of the culling (BatchView grpv), a wave
-// skips each 64-source group whose kernel values against all its targets
-// are below the culling level (the exact bound on r^2, as the direct form).
+// each product, FP32 accumulation); the low half of k' is formed by one
+// mixed-precision FMA per value (k' minus its FP16 high half, exact).  The
+// wave's scale 2^S keeps k' below 2^15, and power-of-two column scales from
+// each staged tile's maxima keep the columns below 2^14, so both halves stay
+// normal down to 2^-29 of the largest value.  The FP32 sums run over the
+// whole source chunk: when a staged tile's column scales differ from the
+// previous tile's, the sums are rescaled by the (power-of-two, exact) ratio.
+// exp2(-|y_i|^2), the scales and the combination with the target's own
+// vectors are applied once per block in FP64, one thread per target.
+//
+// The wave's exponent operand (x', |x'|^2 - S) of every staged source is
+// formed once per staged tile, by the staging thread for all four waves,
+// into per-wave rows of LDS (component-major, padded so that the 64 lanes'
+// reads hit distinct banks).
+//
+// With the group bounds of the culling (BatchView grpv), the chunk's aligned
+// 64-source groups are tested against the block's targets before staging,
+// the surviving groups are staged four to a tile, and each wave skips the
+// staged groups whose kernel values against all its targets are below the
+// culling level (the exact bound on r^2, as the direct form).
 //
 // A block whose wave tiles are wider than the expanded form's bound (small
 // problems, narrow kernels), or a run with the expanded form switched off,
-// takes the vector path (pairs_block with four targets per lane) instead.
+// takes the vector path instead (pairs_block, one target per lane, four
+// tiles of 64 targets).
 #pragma once
 
 typedef float mx_v4f __attribute__((ext_vector_type(4)));
 typedef _Float16 mx_v8h __attribute__((ext_vector_type(8)));
+typedef __fp16 mx_h2 __attribute__((ext_vector_type(2)));
 
 constexpr int kMxTI = 256;          // targets per block
 constexpr int kMxTJ = 256;          // sources per staged tile
 constexpr int kMxRow = kMxTJ + 8;   // halves per column row (16-byte pad: the rows of one read hit distinct banks)
 constexpr int kMxCols = 24;         // columns per source (two tiles of 16, the last 8 zero)
+constexpr int kMxAP = kMxTJ + 4;    // floats per component row of a wave's exponent operand
 
 struct MxSmem {
-    float4 crd[kMxTJ];                  // block-anchored scaled FP32 offsets of the staged sources
-    _Float16 colh[2][16][kMxRow];       // columns, high halves: [column tile][column][source]
-    _Float16 coll[2][16][kMxRow];       // low halves
-    double dred[kWaves][4];             // block reductions
+    _Float16 colh[2][16][kMxRow];   // columns, high halves: [column tile][column][source]
+    _Float16 coll[2][16][kMxRow];   // low halves
+    float aop[kWaves][4][kMxAP];    // per wave: x' (3) and |x'|^2 - S of the staged sources
+    float wfr[kWaves][4];           // per wave: its centroid (block frame, scaled) and S
+    double dred[kWaves][4];         // block reductions
+    int glist[kMxTJ];               // surviving groups of a window
+    int wcnt[kWaves];
     int flag[kWaves];
+    float wr2[kWaves];
 };
 
-using OpBwdMxFallback = OpBwd<3, -1, PrecF32, 4>;
+using OpBwdMxFallback = OpBwd<3, -1, PrecF32, 1>;
 constexpr size_t kMxSmemBytes =
     sizeof(MxSmem) > PairSmem<OpBwdMxFallback>::bytes ? sizeof(MxSmem) : PairSmem<OpBwdMxFallback>::bytes;
 
-// 2^(top - e) for m = f 2^e, f in [1/2, 1): m times it lies below 2^top
-__device__ __forceinline__ float mx_pow2_below(float m, int top)
+// 2^(top - e) for m = f 2^e, f in [1/2, 1): m times it lies below 2^top;
+// `keep` when m is zero (no constraint) or not finite
+__device__ __forceinline__ float mx_pow2_below(float m, int top, float keep)
 {
-    if (!(m > 0.0f) || !(m < 3.0e38f)) return 1.0f;
+    if (!(m > 0.0f) || !(m < 3.0e38f)) return keep;
     int e;
     (void)frexpf(m, &e);
     return ldexpf(1.0f, top - e);
+}
+
+// e minus the low or the high FP16 half of h, in FP32 (v_fma_mix_f32: the
+// FP16 operand converted on the fly); exact when that half is e truncated
+// to FP16
+__device__ __forceinline__ float mx_sub_h16_lo(float e, mx_h2 h)
+{
+    float r;
+    asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(e));
+    return r;
+}
+__device__ __forceinline__ float mx_sub_h16_hi(float e, mx_h2 h)
+{
+    float r;
+    asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(e));
+    return r;
 }
 
 // totals of three values over the block: one barrier; S.dred is free again
@@ -85,7 +119,7 @@ __device__ __forceinline__ void mx_block_sum3(double (&v)[3], MxSmem &S, int lan
 }
 
 template <int D>
-__global__ __launch_bounds__(kBlock, 2) void kb_bwd_mx(BatchView v, int t, const int4 *__restrict__ items)
+__global__ __launch_bounds__(kBlock, 3) void kb_bwd_mx(BatchView v, int t, const int4 *__restrict__ items)
 {
     static_assert(D == 3, "matrix-core adjoint: 3D only");
     using KF = KernelFnT<-1, float, 3>;
@@ -115,9 +149,12 @@ __global__ __launch_bounds__(kBlock, 2) void kb_bwd_mx(BatchView v, int t, const
     mx_block_sum3(anc, S, lane, wave, false);
 #pragma unroll
     for (int c = 0; c < 3; ++c) anc[c] /= (double)tcount;
-    float xh[3], dw[3], y[3], y2 = 0.0f;
+    float xh[3], dw[3], y[3], y2 = 0.0f, xh2 = 0.0f;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) xh[c] = real ? (float)((qi[c] - anc[c]) * kap) : 0.0f;
+    for (int c = 0; c < 3; ++c) {
+        xh[c] = real ? (float)((qi[c] - anc[c]) * kap) : 0.0f;
+        xh2 += xh[c] * xh[c];
+    }
     // the wave's frame: offsets from the centroid of its targets
 #pragma unroll
     for (int c = 0; c < 3; ++c) dw[c] = (float)(wave_sum(real ? (double)xh[c] : 0.0) / (double)max(wcount, 1));
@@ -128,8 +165,17 @@ __global__ __launch_bounds__(kBlock, 2) void kb_bwd_mx(BatchView v, int t, const
     }
     const float rtw2 = wave_max(real ? y2 : 0.0f);
     const bool ok = wcount == 0 || rtw2 <= expand_r2(v.expand != 0);
+    // the wave's kernel scale: k' = 2^S exp2(-|x_j - x_i|^2 + |y_i|^2) <= 2^(S + rt^2) <= 2^15
+    const float Sk = floorf(15.0f - rtw2);
+    const float wr2 = wave_max(real ? xh2 : 0.0f);   // squared wave radius about the block anchor
     __syncthreads();   // dred reads done
-    if (lane == 0) S.flag[wave] = ok ? 1 : 0;
+    if (lane == 0) {
+        S.flag[wave] = ok ? 1 : 0;
+        S.wr2[wave] = wr2;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) S.wfr[wave][c] = dw[c];
+        S.wfr[wave][3] = Sk;
+    }
     __syncthreads();
     const bool all_ok = S.flag[0] && S.flag[1] && S.flag[2] && S.flag[3];
     if (!all_ok) {
@@ -137,12 +183,14 @@ __global__ __launch_bounds__(kBlock, 2) void kb_bwd_mx(BatchView v, int t, const
         using Op = OpBwdMxFallback;
         typename Op::Args A{q, a, p, n, (float)d.iw, (float)d.iw2, cull_r2(v.cull), expand_r2(v.expand != 0), kap,
                             pds, v.exec, grp};
-        pairs_block<Op>(A, n, tile, j0, j1, out, smem);
+        for (int s = 0; s < kMxTI / 64 && (4 * tile + s) * 64 < n; ++s) {
+            pairs_block<Op>(A, n, 4 * tile + s, j0, j1, out, smem);
+            __syncthreads();
+        }
         return;
     }
+    const float rtb = __builtin_sqrtf(fmaxf(fmaxf(S.wr2[0], S.wr2[1]), fmaxf(S.wr2[2], S.wr2[3])));
 
-    // the wave's kernel scale: k' = 2^S exp2(-|x_j|^2 + 2 y.x_j) <= 2^(S + rt^2) <= 2^15
-    const float Sk = floorf(15.0f - rtw2);
     const double k2 = ldexp(1.0, -(int)Sk);
     // culling: the wave's centroid (unscaled) and radius (scaled)
     double aw[3];
@@ -158,46 +206,48 @@ __global__ __launch_bounds__(kBlock, 2) void kb_bwd_mx(BatchView v, int t, const
         const float y0 = __shfl(y[0], src, 64), y1 = __shfl(y[1], src, 64), yz = __shfl(y[2], src, 64);
         bs[mt] = g == 0 ? -2.0f * y0 : g == 1 ? -2.0f * y1 : g == 2 ? -2.0f * yz : 1.0f;
     }
-    // source operand of the exponent products: lane (m, k) holds component
-    // k of source 8(m/4) + m%4 (+4 for the second half), in the wave's frame
-    // (x' = x - dw, and |x'|^2 - S); the selection is arithmetic, so the
-    // loop has no branch
-    const float sel0 = g == 0 ? 1.0f : 0.0f, sel1 = g == 1 ? 1.0f : 0.0f, sel2 = g == 2 ? 1.0f : 0.0f,
-                sel3 = g == 3 ? 1.0f : 0.0f;
-    auto aval = [&](const float4 cc) {
-        const float x0 = cc.x - dw[0], x1 = cc.y - dw[1], x2 = cc.z - dw[2];
-        const float n2 = __builtin_fmaf(x0, x0, __builtin_fmaf(x1, x1, __builtin_fmaf(x2, x2, -Sk)));
-        return __builtin_fmaf(sel0, x0, __builtin_fmaf(sel1, x1, __builtin_fmaf(sel2, x2, sel3 * n2)));
-    };
+    // lane (m, k) of the source operand holds component k of source
+    // 8(m/4) + m%4 (+4 for the second product) of the K-block
     const int js0 = 8 * (nn >> 2) + (nn & 3), co0 = 8 * g;
 
     mx_v4f C[4][2];
-    double acc[4][2][4];
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-        for (int nt = 0; nt < 2; ++nt) {
-            C[mt][nt] = mx_v4f{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int r = 0; r < 4; ++r) acc[mt][nt][r] = 0.0;
-        }
+    for (int mt = 0; mt < 4; ++mt) C[mt][0] = C[mt][1] = mx_v4f{0.0f, 0.0f, 0.0f, 0.0f};
     const mx_v4f zero4 = {0.0f, 0.0f, 0.0f, 0.0f};
+    // column scales of the sums in C: offsets, a, a - p (powers of two)
+    float sx = 1.0f, sa = 1.0f, sw = 1.0f;
+    // factor of column `col`'s sum for the scales (x, a, w)
+    auto colf = [](int col, float fx, float fa, float fw) -> float {
+        return col < 3 ? fw : col < 6 ? fa : col < 15 ? fw * fx : col < 24 ? fa * fx : 0.0f;
+    };
     unsigned long long executed = 0;
 
-    for (int jt = j0; jt < j1; jt += kMxTJ) {
-        const int cnt = min(kMxTJ, j1 - jt);
-        __syncthreads();   // the previous tile is consumed, dred free
+    // One staged tile.  grouped: slot s holds source 64 glist[k + s/64] + s%64
+    // of the surviving-group list (ng groups from position k); otherwise
+    // slot s holds source k + s.
+    auto tile_run = [&](bool grouped, int k, int ng) {
+        __syncthreads();   // the previous tile is consumed, dred free; the group list is written
+        // valid sources of each 64-slot quarter, 8 bits each (uniform)
+        unsigned vpack = 0u;
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) {
+            const int vqq = grouped ? (qq < ng ? min(64, j1 - 64 * S.glist[k + qq]) : 0)
+                                    : max(0, min(64, j1 - k - 64 * qq));
+            vpack |= (unsigned)vqq << (8 * qq);
+        }
+        const int tj = threadIdx.x;
+        const int jsrc = lane < (int)((vpack >> (8 * wave)) & 0xFFu)
+                             ? (grouped ? 64 * S.glist[k + wave] + lane : k + tj)
+                             : -1;
         // staging, first pass: the source's offsets and vectors, and the
         // tile's maxima of |x|, |a|, |a - p|
-        const int tj = threadIdx.x;
         float xs[3] = {0.0f, 0.0f, 0.0f}, av[3] = {0.0f, 0.0f, 0.0f}, wv[3] = {0.0f, 0.0f, 0.0f};
         double mx3[3] = {0.0, 0.0, 0.0};
-        if (tj < cnt) {
-            const int j = jt + tj;
+        if (jsrc >= 0) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const double ac = a[(size_t)c * n + j], pc = p[(size_t)c * n + j];
-                xs[c] = (float)((q[(size_t)c * n + j] - anc[c]) * kap);
+                const double ac = a[(size_t)c * n + jsrc], pc = p[(size_t)c * n + jsrc];
+                xs[c] = (float)((q[(size_t)c * n + jsrc] - anc[c]) * kap);
                 av[c] = (float)ac;
                 wv[c] = (float)(ac - pc);
                 mx3[0] = fmax(mx3[0], (double)fabsf(xs[c]));
@@ -205,10 +255,30 @@ __global__ __launch_bounds__(kBlock, 2) void kb_bwd_mx(BatchView v, int t, const
                 mx3[2] = fmax(mx3[2], (double)fabsf(wv[c]));
             }
         }
-        S.crd[tj] = make_float4(xs[0], xs[1], xs[2], 0.0f);
+        // the wave exponent operands of this source, for every wave
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const float x0 = xs[0] - S.wfr[w][0], x1 = xs[1] - S.wfr[w][1], x2 = xs[2] - S.wfr[w][2];
+            S.aop[w][0][tj] = x0;
+            S.aop[w][1][tj] = x1;
+            S.aop[w][2][tj] = x2;
+            S.aop[w][3][tj] = __builtin_fmaf(x0, x0, __builtin_fmaf(x1, x1, __builtin_fmaf(x2, x2, -S.wfr[w][3])));
+        }
         mx_block_sum3(mx3, S, lane, wave, true);
-        const float sx = mx_pow2_below((float)mx3[0], 0), sa = mx_pow2_below((float)mx3[1], 14),
-                    sw = mx_pow2_below((float)mx3[2], 14);
+        const float nx = mx_pow2_below((float)mx3[0], 0, sx), na = mx_pow2_below((float)mx3[1], 14, sa),
+                    nw = mx_pow2_below((float)mx3[2], 14, sw);
+        if (nx != sx || na != sa || nw != sw) {   // uniform: block maxima
+            const float fx = nx / sx, fa = na / sa, fw = nw / sw;
+            const float f0 = colf(nn, fx, fa, fw), f1 = colf(16 + nn, fx, fa, fw);
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                C[mt][0] *= f0;
+                C[mt][1] *= f1;
+            }
+            sx = nx;
+            sa = na;
+            sw = nw;
+        }
         // second pass: the 24 scaled columns as FP16 halves
         {
             float col[kMxCols];
@@ -223,105 +293,123 @@ __global__ __launch_bounds__(kBlock, 2) void kb_bwd_mx(BatchView v, int t, const
                 }
             }
 #pragma unroll
-            for (int k = 0; k < 32; ++k) {
-                const float vv = k < kMxCols ? col[k < kMxCols ? k : 0] : 0.0f;
+            for (int kk = 0; kk < 32; ++kk) {
+                const float vv = kk < kMxCols ? col[kk < kMxCols ? kk : 0] : 0.0f;
                 const _Float16 h = (_Float16)vv;
-                S.colh[k >> 4][k & 15][tj] = h;
-                S.coll[k >> 4][k & 15][tj] = (_Float16)(vv - (float)h);
+                S.colh[kk >> 4][kk & 15][tj] = h;
+                S.coll[kk >> 4][kk & 15][tj] = (_Float16)(vv - (float)h);
             }
         }
-        // FP64 factor of the lane's two columns: the inverse column scale and 2^-S
-        auto cinv = [&](int col) -> double {
-            if (col < 3) return k2 / sw;
-            if (col < 6) return k2 / sa;
-            if (col < 15) return k2 / ((double)sw * sx);
-            if (col < 24) return k2 / ((double)sa * sx);
-            return 0.0;
-        };
-        const double cs0 = cinv(nn), cs1 = cinv(16 + nn);
-        // the wave's live 64-source groups of the tile (bit q: group q)
-        unsigned live = 0xFu;
-        if (grp != nullptr && wcount > 0) {
-            bool keep = true;
-            if (lane < 4) {
-                const int gi = (jt >> 6) + lane;
-                if (64 * lane < cnt) {
-                    const double4 b = grp[gi];
-                    const double dx = b.x - aw[0], dy = b.y - aw[1], dz = b.z - aw[2];
-                    const float ms = (float)(fmax(0.0, sqrt(dx * dx + dy * dy + dz * dz) - b.w) * kap);
-                    const float gap = ms - rtw;
-                    keep = !(gap > 0.0f && gap * gap > cut2);
-                }
+        // the wave's live quarters of the tile (bit qq: quarter qq)
+        unsigned live = 0u;
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq)
+            if ((vpack >> (8 * qq)) & 0xFFu) live |= 1u << qq;
+        if (grouped && wcount > 0) {
+            bool keep = false;
+            if (lane < ng) {
+                const double4 b = grp[S.glist[k + lane]];
+                const double dx = b.x - aw[0], dy = b.y - aw[1], dz = b.z - aw[2];
+                const float ms = (float)(fmax(0.0, sqrt(dx * dx + dy * dy + dz * dz) - b.w) * kap);
+                const float gap = ms - rtw;
+                keep = !(gap > 0.0f && gap * gap > cut2);
             }
-            live = (unsigned)__ballot(lane < 4 && keep) & 0xFu;
+            live &= (unsigned)__ballot(keep) & 0xFu;
         }
-        __syncthreads();
-        if (wcount > 0 && live != 0u) {
-            const int nkb = (cnt + 31) >> 5;
-            // the live K-blocks of the tile as a bit mask (uniform)
-            unsigned kmask = 0u;
+        __syncthreads();   // columns and exponent operands are staged
+        if (wcount == 0 || live == 0u) return;
+        // the live K-blocks of the tile as a bit mask (uniform)
+        unsigned kmask = 0u;
 #pragma unroll
-            for (int k = 0; k < 8; ++k)
-                if (k < nkb && ((live >> (k >> 1)) & 1u)) kmask |= 1u << k;
-            while (kmask != 0u) {
-                const int kb = __builtin_ctz(kmask);
-                kmask &= kmask - 1u;
-                executed += (unsigned long long)min(32, cnt - 32 * kb) * (unsigned long long)wcount;
-                const float A0 = aval(S.crd[kb * 32 + js0]), A1 = aval(S.crd[kb * 32 + js0 + 4]);
-                // all eight exponent products of the K-block first, so the
-                // exponentials of one row tile overlap the products of the next
-                mx_v4f s0[4], s1[4];
+        for (int kb = 0; kb < 8; ++kb)
+            if (((live >> (kb >> 1)) & 1u) && (int)((vpack >> (8 * (kb >> 1))) & 0xFFu) > 32 * (kb & 1))
+                kmask |= 1u << kb;
+        while (kmask != 0u) {
+            const int kb = __builtin_ctz(kmask);
+            kmask &= kmask - 1u;
+            const int vk = min(32, (int)((vpack >> (8 * (kb >> 1))) & 0xFFu) - 32 * (kb & 1));
+            executed += (unsigned long long)vk * (unsigned long long)wcount;
+            const float A0 = S.aop[wave][g][kb * 32 + js0], A1 = S.aop[wave][g][kb * 32 + js0 + 4];
+            const mx_v8h h0c = *reinterpret_cast<const mx_v8h *>(&S.colh[0][nn][kb * 32 + co0]);
+            const mx_v8h l0c = *reinterpret_cast<const mx_v8h *>(&S.coll[0][nn][kb * 32 + co0]);
+            const mx_v8h h1c = *reinterpret_cast<const mx_v8h *>(&S.colh[1][nn][kb * 32 + co0]);
+            const mx_v8h l1c = *reinterpret_cast<const mx_v8h *>(&S.coll[1][nn][kb * 32 + co0]);
 #pragma unroll
-                for (int mt = 0; mt < 4; ++mt) {
-                    s0[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(A0, bs[mt], zero4, 0, 0, 0);
-                    s1[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(A1, bs[mt], zero4, 0, 0, 0);
+            for (int mt = 0; mt < 4; ++mt) {
+                const mx_v4f s0 = __builtin_amdgcn_mfma_f32_16x16x4f32(A0, bs[mt], zero4, 0, 0, 0);
+                const mx_v4f s1 = __builtin_amdgcn_mfma_f32_16x16x4f32(A1, bs[mt], zero4, 0, 0, 0);
+                // exponents: lane (n, g) has target n against sources 8g .. 8g+7;
+                // k = hi + lo: hi truncated to FP16 (so k - hi is exact in
+                // FP32), lo rounded to nearest
+                mx_v8h ah, al;
+#pragma unroll
+                for (int r = 0; r < 4; r += 2) {
+                    const float e0 = __builtin_amdgcn_exp2f(-s0[r]), e1 = __builtin_amdgcn_exp2f(-s0[r + 1]);
+                    const float f0 = __builtin_amdgcn_exp2f(-s1[r]), f1 = __builtin_amdgcn_exp2f(-s1[r + 1]);
+                    const mx_h2 he = __builtin_amdgcn_cvt_pkrtz(e0, e1), hf = __builtin_amdgcn_cvt_pkrtz(f0, f1);
+                    ah[r] = he[0];
+                    ah[r + 1] = he[1];
+                    ah[4 + r] = hf[0];
+                    ah[5 + r] = hf[1];
+                    al[r] = (_Float16)mx_sub_h16_lo(e0, he);
+                    al[r + 1] = (_Float16)mx_sub_h16_hi(e1, he);
+                    al[4 + r] = (_Float16)mx_sub_h16_lo(f0, hf);
+                    al[5 + r] = (_Float16)mx_sub_h16_hi(f1, hf);
                 }
-                const mx_v8h h0c = *reinterpret_cast<const mx_v8h *>(&S.colh[0][nn][kb * 32 + co0]);
-                const mx_v8h l0c = *reinterpret_cast<const mx_v8h *>(&S.coll[0][nn][kb * 32 + co0]);
-                const mx_v8h h1c = *reinterpret_cast<const mx_v8h *>(&S.colh[1][nn][kb * 32 + co0]);
-                const mx_v8h l1c = *reinterpret_cast<const mx_v8h *>(&S.coll[1][nn][kb * 32 + co0]);
-#pragma unroll
-                for (int mt = 0; mt < 4; ++mt) {
-                    // exponents: lane (n, g) has target n against sources 8g .. 8g+7;
-                    // k = hi + lo: hi truncated to FP16 (so k - hi is exact in
-                    // FP32), lo rounded to nearest
-                    mx_v8h ah, al;
-#pragma unroll
-                    for (int r = 0; r < 4; r += 2) {
-                        const float e0 = __builtin_amdgcn_exp2f(-s0[mt][r]), e1 = __builtin_amdgcn_exp2f(-s0[mt][r + 1]);
-                        const float f0 = __builtin_amdgcn_exp2f(-s1[mt][r]), f1 = __builtin_amdgcn_exp2f(-s1[mt][r + 1]);
-                        const auto he = __builtin_amdgcn_cvt_pkrtz(e0, e1), hf = __builtin_amdgcn_cvt_pkrtz(f0, f1);
-                        ah[r] = he[0];
-                        ah[r + 1] = he[1];
-                        ah[4 + r] = hf[0];
-                        ah[5 + r] = hf[1];
-                        al[r] = (_Float16)(e0 - (float)he[0]);
-                        al[r + 1] = (_Float16)(e1 - (float)he[1]);
-                        al[4 + r] = (_Float16)(f0 - (float)hf[0]);
-                        al[5 + r] = (_Float16)(f1 - (float)hf[1]);
-                    }
-                    C[mt][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, h0c, C[mt][0], 0, 0, 0);
-                    C[mt][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, h1c, C[mt][1], 0, 0, 0);
-                    C[mt][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, l0c, C[mt][0], 0, 0, 0);
-                    C[mt][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, l1c, C[mt][1], 0, 0, 0);
-                    C[mt][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, h0c, C[mt][0], 0, 0, 0);
-                    C[mt][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, h1c, C[mt][1], 0, 0, 0);
-                }
+                C[mt][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, h0c, C[mt][0], 0, 0, 0);
+                C[mt][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, h1c, C[mt][1], 0, 0, 0);
+                C[mt][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, l0c, C[mt][0], 0, 0, 0);
+                C[mt][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, l1c, C[mt][1], 0, 0, 0);
+                C[mt][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, h0c, C[mt][0], 0, 0, 0);
+                C[mt][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, h1c, C[mt][1], 0, 0, 0);
             }
-            // the tile's FP32 sums into the FP64 totals, scales removed
-#pragma unroll
-            for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    acc[mt][0][r] += (double)C[mt][0][r] * cs0;
-                    acc[mt][1][r] += (double)C[mt][1][r] * cs1;
-                }
-#pragma unroll
-            for (int mt = 0; mt < 4; ++mt) C[mt][0] = C[mt][1] = zero4;
         }
+    };
+
+    if (grp != nullptr && (j0 & 63) == 0) {
+        // the chunk's groups against the block (anchor, radius rtb), a window
+        // of up to kBlock groups at a time, survivors listed in order
+        const int gA = j0 >> 6, gB = (j1 + 63) >> 6;
+        for (int w0 = gA; w0 < gB; w0 += kBlock) {
+            const int wn = min(kBlock, gB - w0);
+            __syncthreads();   // the previous window's list and tiles are consumed
+            bool glive = false;
+            if ((int)threadIdx.x < wn) {
+                const double4 b = grp[w0 + threadIdx.x];
+                const double dx = b.x - anc[0], dy = b.y - anc[1], dz = b.z - anc[2];
+                const float ms = (float)(fmax(0.0, sqrt(dx * dx + dy * dy + dz * dz) - b.w) * kap);
+                const float gap = ms - rtb;
+                glive = !(gap > 0.0f && gap * gap > cut2);
+            }
+            const unsigned long long bal = __ballot(glive);
+            if (lane == 0) S.wcnt[wave] = __popcll(bal);
+            __syncthreads();
+            int base = 0, total = 0;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                base += w < wave ? S.wcnt[w] : 0;
+                total += S.wcnt[w];
+            }
+            if (glive)
+                S.glist[base + __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u))] =
+                    w0 + threadIdx.x;
+            for (int k = 0; k < total; k += 4) tile_run(true, k, min(4, total - k));
+        }
+    } else {
+        for (int jt = j0; jt < j1; jt += kMxTJ) tile_run(false, jt, 0);
     }
     if (v.exec != nullptr && lane == 0 && executed != 0) atomicAdd(v.exec + 1, executed);
 
+    // FP64 factors of the lane's two columns: the inverse column scales and 2^-S
+    auto cinv = [&](int col) -> double {
+        if (col < 3) return k2 / sw;
+        if (col < 6) return k2 / sa;
+        if (col < 15) return k2 / ((double)sw * sx);
+        if (col < 24) return k2 / ((double)sa * sx);
+        return 0.0;
+    };
+    const double cs0 = cinv(nn), cs1 = cinv(16 + nn);
     // combination with the targets' own vectors, one thread per target:
     // lane (n, g) holds columns n and 16 + n of targets 16 mt + 4 g + r
     static_assert(sizeof(double) * kWaves * 64 * kMxCols <= kMxSmemBytes, "epilogue buffer");
@@ -332,8 +420,8 @@ __global__ __launch_bounds__(kBlock, 2) void kb_bwd_mx(BatchView v, int t, const
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int tl = 16 * mt + 4 * g + r;
-            E[((size_t)wave * 64 + tl) * kMxCols + nn] = acc[mt][0][r];
-            if (nn < kMxCols - 16) E[((size_t)wave * 64 + tl) * kMxCols + 16 + nn] = acc[mt][1][r];
+            E[((size_t)wave * 64 + tl) * kMxCols + nn] = (double)C[mt][0][r] * cs0;
+            if (nn < kMxCols - 16) E[((size_t)wave * 64 + tl) * kMxCols + 16 + nn] = (double)C[mt][1][r] * cs1;
         }
     __syncthreads();
     if (!real) return;
