@@ -93,6 +93,10 @@ struct BatchView {
     // gvf (point kernels / varifold) are set
     double4 *grpv, *grpe, *grpt;
     int gpts, gvf;
+    // per group of q_t (one time slice, BatchDesc gvo / T): the largest
+    // |alpha_t| and |alpha_t - p| components, the column scales of the
+    // matrix-core adjoint (kb_group_maxima)
+    float2 *grpm = nullptr;
 };
 
 // x[i] = scale * u_i with u_i a fixed pseudo-random value in [-1/2, 1/2):
@@ -120,6 +124,31 @@ __global__ __launch_bounds__(kTile) void kb_group_bounds(BatchView v, int t, int
     } else {
         group_bounds_wave<D>(v.tc + (size_t)D * d.to, d.nt, it.y, threadIdx.x, v.grpt + d.gto);
     }
+}
+
+// per aligned 64-point group of alpha_t and the current adjoint p: the
+// largest |alpha| and |alpha - p| components (FP32, rounded as the
+// matrix-core adjoint stages them), one group per item
+template <int D>
+__global__ __launch_bounds__(kTile) void kb_group_maxima(BatchView v, int t, const int4 *__restrict__ items)
+{
+    const int4 it = items[blockIdx.x];
+    const BatchDesc d = v.desc[it.x];
+    const int i = it.y * 64 + threadIdx.x;
+    const double *a = v.alp + (size_t)(v.T - 1) * D * d.vo + (size_t)t * D * d.n;
+    const double *p = v.padj + (size_t)D * d.vo;
+    float am = 0.0f, wm = 0.0f;
+    if (i < d.n) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            const double ac = a[(size_t)c * d.n + i], pc = p[(size_t)c * d.n + i];
+            am = fmaxf(am, fabsf((float)ac));
+            wm = fmaxf(wm, fabsf((float)(ac - pc)));
+        }
+    }
+    am = wave_max(am);
+    wm = wave_max(wm);
+    if (threadIdx.x == 0) v.grpm[d.gvo / v.T + it.y] = make_float2(am, wm);
 }
 
 template <int NC>
@@ -905,6 +934,7 @@ Batch::Batch(const BatchSpec &spec, const double *q0, const int *elm, const doub
             gt += (spec_.nt[p] + 63) / 64;
         }
         slab_.reserve(&grpv_, (size_t)std::max(gv, 1LL));
+        slab_.reserve(&grpm_, (size_t)std::max(gv / spec_.T, 1LL));
         slab_.reserve(&grpe_, (size_t)std::max(ge, 1LL));
         slab_.reserve(&grpt_, (size_t)std::max(gt, 1LL));
     }
@@ -1115,6 +1145,7 @@ void Batch::launch_eval(const double *alp, double *grd, double *f_dev)
                 spec_.h, spec_.w, ecen_, edir_, evol_, dc_, dd_, efwd_, evf_, pair_cull_bits(),
                 pair_expansion() ? 1 : 0, pair_counter(), grpv_, grpe_, grpt_,
                 pair_groups() && spec_.order == -1 ? 1 : 0, pair_groups() && spec_.vp.cen_type == 'G' ? 1 : 0};
+    v.grpm = grpm_;
 
     // q_0 of the listed problems
     if (D == 2) kb_init_flow<2><<<tb.nvtile, kTile, 0, st_>>>(v, tb.vtile.p);
@@ -1199,6 +1230,10 @@ void Batch::launch_eval(const double *alp, double *grd, double *f_dev)
     for (int t = T - 2; t >= 0; --t) {
         {
             ProfScope ps(kProfBwdPairs, st_);
+            if (rbwd_ == 4 && mx_ok() && v.gpts && tb.nbwd[1]) {   // column scales of the matrix-core adjoint
+                kb_group_maxima<3><<<tb.nvtile, kTile, 0, st_>>>(v, t, tb.vtile.p);
+                LDDMM_LAUNCH_CHECK();
+            }
             each_prec([&](int g, auto Pc) {
                 if (!tb.nbwd[g]) return;
                 dispatch([&](auto Dc, auto Oc) {
@@ -1292,6 +1327,7 @@ void Batch::autotune()
                 spec_.T, spec_.h, spec_.w, ecen_, edir_, evol_, dc_, dd_, efwd_, evf_, pair_cull_bits(),
                 pair_expansion() ? 1 : 0, nullptr, grpv_, grpe_, grpt_,
                 pair_groups() && spec_.order == -1 ? 1 : 0, 0};
+    v.grpm = grpm_;
     hipEvent_t e0, e1;
     LDDMM_HIP(hipEventCreate(&e0));
     LDDMM_HIP(hipEventCreate(&e1));

@@ -124,6 +124,7 @@ private:
     double *ecen_ = nullptr, *edir_ = nullptr, *evol_ = nullptr, *dc_ = nullptr, *dd_ = nullptr;
     double *efwd_ = nullptr, *evf_ = nullptr, *tt_ = nullptr;
     double4 *grpv_ = nullptr, *grpe_ = nullptr, *grpt_ = nullptr;   // group bounds (FP32 culling)
+    float2 *grpm_ = nullptr;   // group maxima of alpha_t and alpha_t - p (matrix-core adjoint)
     // work tables for the current active set
     struct Tables;
     std::unique_ptr<Tables> tab_;

@@ -24,9 +24,13 @@
 // mixed-precision FMA per value (k' minus its FP16 high half, exact).  The
 // wave's scale 2^S keeps k' below 2^15, and power-of-two column scales from
 // each staged tile's maxima keep the columns below 2^14, so both halves stay
-// normal down to 2^-29 of the largest value.  The FP32 sums run over the
-// whole source chunk: when a staged tile's column scales differ from the
-// previous tile's, the sums are rescaled by the (power-of-two, exact) ratio.
+// normal down to 2^-29 of the largest value.  A tile's column scales come
+// from its four groups' data, without a pass over the sources: the group
+// bounds of q_t bound the offsets, and the group maxima of |alpha_t| and
+// |alpha_t - p| (kb_group_maxima, launched before each step) bound the
+// vectors.  The FP32 sums run over the whole source chunk: when a staged
+// tile's column scales differ from the previous tile's, the sums are
+// rescaled by the (power-of-two, exact) ratio.
 // exp2(-|y_i|^2), the scales and the combination with the target's own
 // vectors are applied once per block in FP64, one thread per target.
 //
@@ -35,16 +39,16 @@
 // into per-wave rows of LDS (component-major, padded so that the 64 lanes'
 // reads hit distinct banks).
 //
-// With the group bounds of the culling (BatchView grpv), the chunk's aligned
-// 64-source groups are tested against the block's targets before staging,
-// the surviving groups are staged four to a tile, and each wave skips the
-// staged groups whose kernel values against all its targets are below the
-// culling level (the exact bound on r^2, as the direct form).
+// The chunk's aligned 64-source groups are tested against the block's
+// targets (group bounds of the culling, BatchView grpv) before staging, the
+// surviving groups are staged four to a tile, and each wave skips the staged
+// groups whose kernel values against all its targets are below the culling
+// level (the exact bound on r^2, as the direct form).
 //
 // A block whose wave tiles are wider than the expanded form's bound (small
-// problems, narrow kernels), or a run with the expanded form switched off,
-// takes the vector path instead (pairs_block, one target per lane, four
-// tiles of 64 targets).
+// problems, narrow kernels), a run with the expanded form or the source
+// groups switched off, takes the vector path instead (pairs_block, one
+// target per lane, four tiles of 64 targets).
 #pragma once
 
 typedef float mx_v4f __attribute__((ext_vector_type(4)));
@@ -54,12 +58,12 @@ typedef __fp16 mx_h2 __attribute__((ext_vector_type(2)));
 constexpr int kMxTI = 256;          // targets per block
 constexpr int kMxTJ = 256;          // sources per staged tile
 constexpr int kMxRow = kMxTJ + 8;   // halves per column row (16-byte pad: the rows of one read hit distinct banks)
-constexpr int kMxCols = 24;         // columns per source (two tiles of 16, the last 8 zero)
+constexpr int kMxCols = 24;         // columns per source (two tiles of 16; lanes 8-15 of the second repeat 16-23)
 constexpr int kMxAP = kMxTJ + 4;    // floats per component row of a wave's exponent operand
 
 struct MxSmem {
-    _Float16 colh[2][16][kMxRow];   // columns, high halves: [column tile][column][source]
-    _Float16 coll[2][16][kMxRow];   // low halves
+    _Float16 colh[kMxCols][kMxRow];   // columns, high halves: [column][source]
+    _Float16 coll[kMxCols][kMxRow];   // low halves
     float aop[kWaves][4][kMxAP];    // per wave: x' (3) and |x'|^2 - S of the staged sources
     float wfr[kWaves][4];           // per wave: its centroid (block frame, scaled) and S
     double dred[kWaves][4];         // block reductions
@@ -70,8 +74,9 @@ struct MxSmem {
 };
 
 using OpBwdMxFallback = OpBwd<3, -1, PrecF32, 1>;
-constexpr size_t kMxSmemBytes =
-    sizeof(MxSmem) > PairSmem<OpBwdMxFallback>::bytes ? sizeof(MxSmem) : PairSmem<OpBwdMxFallback>::bytes;
+constexpr size_t kMxEpilogueBytes = sizeof(double) * kWaves * 64 * kMxCols;   // FP64 column sums per target
+constexpr size_t mx_max(size_t x, size_t y) { return x > y ? x : y; }
+constexpr size_t kMxSmemBytes = mx_max(mx_max(sizeof(MxSmem), PairSmem<OpBwdMxFallback>::bytes), kMxEpilogueBytes);
 
 // 2^(top - e) for m = f 2^e, f in [1/2, 1): m times it lies below 2^top;
 // `keep` when m is zero (no constraint) or not finite
@@ -177,7 +182,8 @@ __global__ __launch_bounds__(kBlock, 3) void kb_bwd_mx(BatchView v, int t, const
         S.wfr[wave][3] = Sk;
     }
     __syncthreads();
-    const bool all_ok = S.flag[0] && S.flag[1] && S.flag[2] && S.flag[3];
+    const bool all_ok = S.flag[0] && S.flag[1] && S.flag[2] && S.flag[3] && grp != nullptr && v.grpm != nullptr &&
+                        (j0 & 63) == 0;
     if (!all_ok) {
         __syncthreads();   // flags read before the vector path reuses the memory
         using Op = OpBwdMxFallback;
@@ -196,7 +202,7 @@ __global__ __launch_bounds__(kBlock, 3) void kb_bwd_mx(BatchView v, int t, const
     double aw[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) aw[c] = anc[c] + (double)dw[c] / kap;
-    const float rtw = __builtin_sqrtf(rtw2), cut2 = cull_r2(v.cull);
+    const float rtw = __builtin_sqrtf(rtw2), cut2 = cull_r2(v.cull), kapf = (float)kap;
     const int nn = lane & 15, g = lane >> 4;
     // target operand of the exponent products, per row tile: (-2 y, 1)
     float bs[4];
@@ -222,52 +228,37 @@ __global__ __launch_bounds__(kBlock, 3) void kb_bwd_mx(BatchView v, int t, const
     };
     unsigned long long executed = 0;
 
-    // One staged tile.  grouped: slot s holds source 64 glist[k + s/64] + s%64
-    // of the surviving-group list (ng groups from position k); otherwise
-    // slot s holds source k + s.
-    auto tile_run = [&](bool grouped, int k, int ng) {
-        __syncthreads();   // the previous tile is consumed, dred free; the group list is written
-        // valid sources of each 64-slot quarter, 8 bits each (uniform)
+    // per group of q_t (this problem, this step): the largest |alpha_t| and |alpha_t - p|
+    const float2 *gmax = v.grpm + d.gvo / v.T;
+
+    // One staged tile: slot s holds source 64 glist[k + s/64] + s%64 of the
+    // surviving-group list (ng groups from position k).
+    auto tile_run = [&](int k, int ng) {
+        __syncthreads();   // the previous tile is consumed; the group list is written
+        // valid sources of each 64-slot quarter, 8 bits each, and the
+        // tile's bounds on |x|, |a|, |a - p| from its groups' data (uniform)
         unsigned vpack = 0u;
+        float xb = 0.0f, ab = 0.0f, wb = 0.0f;
 #pragma unroll
         for (int qq = 0; qq < 4; ++qq) {
-            const int vqq = grouped ? (qq < ng ? min(64, j1 - 64 * S.glist[k + qq]) : 0)
-                                    : max(0, min(64, j1 - k - 64 * qq));
-            vpack |= (unsigned)vqq << (8 * qq);
-        }
-        const int tj = threadIdx.x;
-        const int jsrc = lane < (int)((vpack >> (8 * wave)) & 0xFFu)
-                             ? (grouped ? 64 * S.glist[k + wave] + lane : k + tj)
-                             : -1;
-        // staging, first pass: the source's offsets and vectors, and the
-        // tile's maxima of |x|, |a|, |a - p|
-        float xs[3] = {0.0f, 0.0f, 0.0f}, av[3] = {0.0f, 0.0f, 0.0f}, wv[3] = {0.0f, 0.0f, 0.0f};
-        double mx3[3] = {0.0, 0.0, 0.0};
-        if (jsrc >= 0) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const double ac = a[(size_t)c * n + jsrc], pc = p[(size_t)c * n + jsrc];
-                xs[c] = (float)((q[(size_t)c * n + jsrc] - anc[c]) * kap);
-                av[c] = (float)ac;
-                wv[c] = (float)(ac - pc);
-                mx3[0] = fmax(mx3[0], (double)fabsf(xs[c]));
-                mx3[1] = fmax(mx3[1], (double)fabsf(av[c]));
-                mx3[2] = fmax(mx3[2], (double)fabsf(wv[c]));
+            if (qq < ng) {
+                const int gi = __builtin_amdgcn_readfirstlane(S.glist[k + qq]);
+                vpack |= (unsigned)min(64, j1 - 64 * gi) << (8 * qq);
+                const double4 b = grp[gi];
+                const double dx = b.x - anc[0], dy = b.y - anc[1], dz = b.z - anc[2];
+                xb = fmaxf(xb, (__builtin_sqrtf((float)(dx * dx + dy * dy + dz * dz)) + (float)b.w) * kapf);
+                const float2 m = gmax[gi];
+                ab = fmaxf(ab, m.x);
+                wb = fmaxf(wb, m.y);
             }
         }
-        // the wave exponent operands of this source, for every wave
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) {
-            const float x0 = xs[0] - S.wfr[w][0], x1 = xs[1] - S.wfr[w][1], x2 = xs[2] - S.wfr[w][2];
-            S.aop[w][0][tj] = x0;
-            S.aop[w][1][tj] = x1;
-            S.aop[w][2][tj] = x2;
-            S.aop[w][3][tj] = __builtin_fmaf(x0, x0, __builtin_fmaf(x1, x1, __builtin_fmaf(x2, x2, -S.wfr[w][3])));
-        }
-        mx_block_sum3(mx3, S, lane, wave, true);
-        const float nx = mx_pow2_below((float)mx3[0], 0, sx), na = mx_pow2_below((float)mx3[1], 14, sa),
-                    nw = mx_pow2_below((float)mx3[2], 14, sw);
-        if (nx != sx || na != sa || nw != sw) {   // uniform: block maxima
+        // column scales of this tile: kept while the tile fits them with at
+        // most 8 bits of headroom unused, else the tile's own; the sums so
+        // far follow a change exactly (powers of two)
+        auto keep_or = [](float cur, float want) { return want < cur || want > 256.0f * cur ? want : cur; };
+        const float nx = keep_or(sx, mx_pow2_below(xb, 0, sx)), na = keep_or(sa, mx_pow2_below(ab, 14, sa)),
+                    nw = keep_or(sw, mx_pow2_below(wb, 14, sw));
+        if (nx != sx || na != sa || nw != sw) {
             const float fx = nx / sx, fa = na / sa, fw = nw / sw;
             const float f0 = colf(nn, fx, fa, fw), f1 = colf(16 + nn, fx, fa, fw);
 #pragma unroll
@@ -279,61 +270,92 @@ __global__ __launch_bounds__(kBlock, 3) void kb_bwd_mx(BatchView v, int t, const
             sa = na;
             sw = nw;
         }
-        // second pass: the 24 scaled columns as FP16 halves
-        {
-            float col[kMxCols];
+        const int tj = threadIdx.x;
+        const int jsrc =
+            lane < (int)((vpack >> (8 * wave)) & 0xFFu) ? 64 * __builtin_amdgcn_readfirstlane(S.glist[k + wave]) + lane : -1;
+        // staging: the source's offsets and vectors (zero for an empty slot)
+        float xs[3] = {0.0f, 0.0f, 0.0f}, av[3] = {0.0f, 0.0f, 0.0f}, wv[3] = {0.0f, 0.0f, 0.0f};
+        if (jsrc >= 0) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                col[c] = wv[c] * sw;
-                col[3 + c] = av[c] * sa;
+                const double ac = a[(size_t)c * n + jsrc], pc = p[(size_t)c * n + jsrc];
+                xs[c] = (float)((q[(size_t)c * n + jsrc] - anc[c]) * kap);
+                av[c] = (float)ac;
+                wv[c] = (float)(ac - pc);
+            }
+        }
+        // the wave exponent operands of this source, for every wave
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const float x0 = xs[0] - S.wfr[w][0], x1 = xs[1] - S.wfr[w][1], x2 = xs[2] - S.wfr[w][2];
+            S.aop[w][0][tj] = x0;
+            S.aop[w][1][tj] = x1;
+            S.aop[w][2][tj] = x2;
+            S.aop[w][3][tj] = __builtin_fmaf(x0, x0, __builtin_fmaf(x1, x1, __builtin_fmaf(x2, x2, -S.wfr[w][3])));
+        }
+        // the 24 scaled columns as FP16 halves, both rounded toward zero: hi
+        // so that v - hi is exact in FP32, lo within 2^-22 of v
+        {
+            float xsc[3], wsc[3], asc[3], col[kMxCols];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                xsc[c] = xs[c] * sx;
+                wsc[c] = wv[c] * sw;
+                asc[c] = av[c] * sa;
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                col[c] = wsc[c];
+                col[3 + c] = asc[c];
 #pragma unroll
                 for (int e = 0; e < 3; ++e) {
-                    col[6 + 3 * c + e] = (wv[c] * sw) * (xs[e] * sx);
-                    col[15 + 3 * c + e] = (av[c] * sa) * (xs[e] * sx);
+                    col[6 + 3 * c + e] = wsc[c] * xsc[e];
+                    col[15 + 3 * c + e] = asc[c] * xsc[e];
                 }
             }
 #pragma unroll
-            for (int kk = 0; kk < 32; ++kk) {
-                const float vv = kk < kMxCols ? col[kk < kMxCols ? kk : 0] : 0.0f;
-                const _Float16 h = (_Float16)vv;
-                S.colh[kk >> 4][kk & 15][tj] = h;
-                S.coll[kk >> 4][kk & 15][tj] = (_Float16)(vv - (float)h);
+            for (int kk = 0; kk < kMxCols; kk += 2) {
+                const mx_h2 h = __builtin_amdgcn_cvt_pkrtz(col[kk], col[kk + 1]);
+                const mx_h2 l = __builtin_amdgcn_cvt_pkrtz(mx_sub_h16_lo(col[kk], h), mx_sub_h16_hi(col[kk + 1], h));
+                S.colh[kk][tj] = h[0];
+                S.colh[kk + 1][tj] = h[1];
+                S.coll[kk][tj] = l[0];
+                S.coll[kk + 1][tj] = l[1];
             }
         }
         // the wave's live quarters of the tile (bit qq: quarter qq)
         unsigned live = 0u;
-#pragma unroll
-        for (int qq = 0; qq < 4; ++qq)
-            if ((vpack >> (8 * qq)) & 0xFFu) live |= 1u << qq;
-        if (grouped && wcount > 0) {
+        if (wcount > 0) {
             bool keep = false;
             if (lane < ng) {
                 const double4 b = grp[S.glist[k + lane]];
                 const double dx = b.x - aw[0], dy = b.y - aw[1], dz = b.z - aw[2];
-                const float ms = (float)(fmax(0.0, sqrt(dx * dx + dy * dy + dz * dz) - b.w) * kap);
+                const float ms = fmaxf(0.0f, __builtin_sqrtf((float)(dx * dx + dy * dy + dz * dz)) - (float)b.w) * kapf;
                 const float gap = ms - rtw;
                 keep = !(gap > 0.0f && gap * gap > cut2);
             }
-            live &= (unsigned)__ballot(keep) & 0xFu;
+            live = (unsigned)__ballot(keep) & 0xFu;
         }
         __syncthreads();   // columns and exponent operands are staged
-        if (wcount == 0 || live == 0u) return;
+        if (live == 0u) return;
         // the live K-blocks of the tile as a bit mask (uniform)
         unsigned kmask = 0u;
 #pragma unroll
         for (int kb = 0; kb < 8; ++kb)
             if (((live >> (kb >> 1)) & 1u) && (int)((vpack >> (8 * (kb >> 1))) & 0xFFu) > 32 * (kb & 1))
                 kmask |= 1u << kb;
+        // second column tile: lanes 8-15 repeat columns 16-23 (their sums are not used)
+        const int row1 = 16 + (nn & 7);
         while (kmask != 0u) {
             const int kb = __builtin_ctz(kmask);
             kmask &= kmask - 1u;
             const int vk = min(32, (int)((vpack >> (8 * (kb >> 1))) & 0xFFu) - 32 * (kb & 1));
             executed += (unsigned long long)vk * (unsigned long long)wcount;
             const float A0 = S.aop[wave][g][kb * 32 + js0], A1 = S.aop[wave][g][kb * 32 + js0 + 4];
-            const mx_v8h h0c = *reinterpret_cast<const mx_v8h *>(&S.colh[0][nn][kb * 32 + co0]);
-            const mx_v8h l0c = *reinterpret_cast<const mx_v8h *>(&S.coll[0][nn][kb * 32 + co0]);
-            const mx_v8h h1c = *reinterpret_cast<const mx_v8h *>(&S.colh[1][nn][kb * 32 + co0]);
-            const mx_v8h l1c = *reinterpret_cast<const mx_v8h *>(&S.coll[1][nn][kb * 32 + co0]);
+            const mx_v8h h0c = *reinterpret_cast<const mx_v8h *>(&S.colh[nn][kb * 32 + co0]);
+            const mx_v8h l0c = *reinterpret_cast<const mx_v8h *>(&S.coll[nn][kb * 32 + co0]);
+            const mx_v8h h1c = *reinterpret_cast<const mx_v8h *>(&S.colh[row1][kb * 32 + co0]);
+            const mx_v8h l1c = *reinterpret_cast<const mx_v8h *>(&S.coll[row1][kb * 32 + co0]);
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt) {
                 const mx_v4f s0 = __builtin_amdgcn_mfma_f32_16x16x4f32(A0, bs[mt], zero4, 0, 0, 0);
@@ -366,7 +388,7 @@ __global__ __launch_bounds__(kBlock, 3) void kb_bwd_mx(BatchView v, int t, const
         }
     };
 
-    if (grp != nullptr && (j0 & 63) == 0) {
+    {
         // the chunk's groups against the block (anchor, radius rtb), a window
         // of up to kBlock groups at a time, survivors listed in order
         const int gA = j0 >> 6, gB = (j1 + 63) >> 6;
@@ -377,7 +399,7 @@ __global__ __launch_bounds__(kBlock, 3) void kb_bwd_mx(BatchView v, int t, const
             if ((int)threadIdx.x < wn) {
                 const double4 b = grp[w0 + threadIdx.x];
                 const double dx = b.x - anc[0], dy = b.y - anc[1], dz = b.z - anc[2];
-                const float ms = (float)(fmax(0.0, sqrt(dx * dx + dy * dy + dz * dz) - b.w) * kap);
+                const float ms = fmaxf(0.0f, __builtin_sqrtf((float)(dx * dx + dy * dy + dz * dz)) - (float)b.w) * kapf;
                 const float gap = ms - rtb;
                 glive = !(gap > 0.0f && gap * gap > cut2);
             }
@@ -394,10 +416,8 @@ __global__ __launch_bounds__(kBlock, 3) void kb_bwd_mx(BatchView v, int t, const
                 S.glist[base + __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
                                                          __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u))] =
                     w0 + threadIdx.x;
-            for (int k = 0; k < total; k += 4) tile_run(true, k, min(4, total - k));
+            for (int k = 0; k < total; k += 4) tile_run(k, min(4, total - k));
         }
-    } else {
-        for (int jt = j0; jt < j1; jt += kMxTJ) tile_run(false, jt, 0);
     }
     if (v.exec != nullptr && lane == 0 && executed != 0) atomicAdd(v.exec + 1, executed);
 
@@ -412,7 +432,7 @@ __global__ __launch_bounds__(kBlock, 3) void kb_bwd_mx(BatchView v, int t, const
     const double cs0 = cinv(nn), cs1 = cinv(16 + nn);
     // combination with the targets' own vectors, one thread per target:
     // lane (n, g) holds columns n and 16 + n of targets 16 mt + 4 g + r
-    static_assert(sizeof(double) * kWaves * 64 * kMxCols <= kMxSmemBytes, "epilogue buffer");
+    static_assert(kMxEpilogueBytes <= kMxSmemBytes, "epilogue buffer");
     double *E = reinterpret_cast<double *>(smem);   // [wave][target][column], 48 KiB
     __syncthreads();   // the last tile is consumed
 #pragma unroll
