@@ -529,6 +529,8 @@ def main():
     ap.add_argument("--batch-chunk", type=int, default=512, help="source chunk of --batch-profile")
     ap.add_argument("--batch-lanes", default="2,2",
                     help="FP32 targets per lane of --batch-profile, K.alpha,adjoint (adjoint 4: matrix cores)")
+    ap.add_argument("--batch-evals", type=int, default=0,
+                    help="--batch-profile: this many objective evaluations at fixed momenta instead of a run")
     ap.add_argument("--batched-fp64", action="store_true", help="also the batched FP64 run of the batch leg")
     ap.add_argument("--no-warm-start", action="store_true", help="skip the learned warm start of the batch leg")
     ap.add_argument("--budget", type=float, default=470.0,
@@ -572,6 +574,22 @@ def main():
         # tracing has crashed the host process during hipGraph capture here;
         # graphs do not change kernel times
         B.set_graphs(False)
+        if args.batch_evals > 0:
+            # fixed momenta 0.02 N(0, 1) (as --launch-shapes), one untimed evaluation first
+            T = k["time_num"]
+            alps = [0.02 * G.normal(100 + i, B.sizes[i]).reshape(T - 1, B.n[i], 3) for i in range(B.P)]
+            B.objgrd(alps)
+            lddmm_amd.profile_reset()
+            lddmm_amd.profile_enable(True)
+            for _ in range(args.batch_evals):
+                B.objgrd(alps)
+            pr = lddmm_amd.profile_read()
+            lddmm_amd.profile_enable(False)
+            print(json.dumps(dict(problems=args.batch, n_range=[int(min(B.n)), int(max(B.n))], chunk=chunk,
+                                  lanes=[kv_lanes, bwd_lanes], evaluations=args.batch_evals,
+                                  fwd_pairs_ms=pr["fwd_pairs"][0] / args.batch_evals,
+                                  bwd_pairs_ms=pr["bwd_pairs"][0] / args.batch_evals)), flush=True)
+            return
         _, info, rounds = B.run(300, 0.0, 0.0, 0.5, 5, precision="fp32", rel_energy_tol=1e-6)
         print(json.dumps(dict(problems=args.batch, chunk=chunk, lanes=[kv_lanes, bwd_lanes],
                               pairs_per_eval_total=float(pairs.sum()),
