@@ -239,8 +239,7 @@ def warm_start_leg(probs, targets, e_ref, B, B64, t_setup, itr, weights):
     (including the start's varifold gradient), inference, the per-problem
     start scale (choose_start: the multiple of the prediction, zero included,
     with the lowest objective), the batched FP32 registration to the plain
-    energies (at most 3 times the plain iteration limit, the cap of the
-    zero-start batched run) and the
+    energies (at most 1.25 times the plain iteration limit) and the
     fallback, a zero-momentum batched FP32 run for every problem the
     warm-started run did not bring to its target.  B and B64 are the batch in
     FP32 and FP64 (the latter re-evaluates results)."""
@@ -252,7 +251,7 @@ def warm_start_leg(probs, targets, e_ref, B, B64, t_setup, itr, weights):
     warm = [ws.predict(p, T) for p in probs]
     t_inf = time.perf_counter() - t0
     starts, scale = W.choose_start(B, warm)
-    aw, iw, rw = B.run(3 * itr, 0.0, 0.0, 0.5, 5, alp0=starts, precision="fp32", targets=targets)
+    aw, iw, rw = B.run(int(1.25 * itr), 0.0, 0.0, 0.5, 5, alp0=starts, precision="fp32", targets=targets)
     missed = [i for i, r in enumerate(iw) if not r["reached_target"]]
     its_fb = []
     if missed:

@@ -237,9 +237,10 @@ int lddmm_batch_run(lddmm_batch *b, const double *iniAlpStks, double *optAlpStks
 /* chosen source chunk and the pair interactions per evaluation of each problem */
 int lddmm_batch_info(lddmm_batch *b, int *chunk, double *pairsPerEval);
 /* the launch shape in use: source points per work item, and targets per
- * lane of the FP32 K.alpha (1, 2, 4) and adjoint (1, 2) pair launches.  With
- * chunk = 0 at creation all three are chosen by timing those launches on the
- * batch (cached per batch signature for the life of the process). */
+ * lane of the FP32 K.alpha (1, 2, 4) and adjoint (1, 2; 4 in 3D Gaussian
+ * batches: the matrix-core adjoint, 256 targets per block) pair launches.
+ * With chunk = 0 at creation all three are chosen by timing those launches on
+ * the batch (cached per batch signature for the life of the process). */
 int lddmm_batch_tuning(lddmm_batch *b, int *chunk, int *kvLanes, int *bwdLanes);
 /* fixes the launch shape (chunk rounded up to a multiple of 64; an error when
  * the partial-sum buffer sized at creation cannot hold it) */
