@@ -237,21 +237,37 @@ __global__ __launch_bounds__(kBlock, 3) void kb_bwd_mx(BatchView v, int t, const
         __syncthreads();   // the previous tile is consumed; the group list is written
         // valid sources of each 64-slot quarter, 8 bits each, and the
         // tile's bounds on |x|, |a|, |a - p| from its groups' data (uniform)
+        // lane qq < ng reads quarter qq's group once: its valid sources, the
+        // bound of its offsets from the block anchor, its maxima of |a| and
+        // |a - p|, and the wave's culling test against it; the wave combines
+        // the four lanes' values (uniform)
+        int vq = 0;
+        float xq = 0.0f, aq = 0.0f, wq = 0.0f;
+        bool keep = false;
+        if (lane < ng) {
+            const int gi = S.glist[k + lane];
+            vq = min(64, j1 - 64 * gi);
+            const double4 b = grp[gi];
+            const double dx = b.x - anc[0], dy = b.y - anc[1], dz = b.z - anc[2];
+            xq = (__builtin_sqrtf((float)(dx * dx + dy * dy + dz * dz)) + (float)b.w) * kapf;
+            const float2 m = gmax[gi];
+            aq = m.x;
+            wq = m.y;
+            const double ex = b.x - aw[0], ey = b.y - aw[1], ez = b.z - aw[2];
+            const float ms = fmaxf(0.0f, __builtin_sqrtf((float)(ex * ex + ey * ey + ez * ez)) - (float)b.w) * kapf;
+            const float gap = ms - rtw;
+            keep = !(gap > 0.0f && gap * gap > cut2);
+        }
         unsigned vpack = 0u;
         float xb = 0.0f, ab = 0.0f, wb = 0.0f;
 #pragma unroll
         for (int qq = 0; qq < 4; ++qq) {
-            if (qq < ng) {
-                const int gi = __builtin_amdgcn_readfirstlane(S.glist[k + qq]);
-                vpack |= (unsigned)min(64, j1 - 64 * gi) << (8 * qq);
-                const double4 b = grp[gi];
-                const double dx = b.x - anc[0], dy = b.y - anc[1], dz = b.z - anc[2];
-                xb = fmaxf(xb, (__builtin_sqrtf((float)(dx * dx + dy * dy + dz * dz)) + (float)b.w) * kapf);
-                const float2 m = gmax[gi];
-                ab = fmaxf(ab, m.x);
-                wb = fmaxf(wb, m.y);
-            }
+            vpack |= (unsigned)__builtin_amdgcn_readlane(vq, qq) << (8 * qq);
+            xb = fmaxf(xb, __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xq), qq)));
+            ab = fmaxf(ab, __int_as_float(__builtin_amdgcn_readlane(__float_as_int(aq), qq)));
+            wb = fmaxf(wb, __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wq), qq)));
         }
+        const unsigned live = wcount > 0 ? (unsigned)__ballot(keep) & 0xFu : 0u;   // bit qq: quarter qq
         // column scales of this tile: kept while the tile fits them with at
         // most 8 bits of headroom unused, else the tile's own; the sums so
         // far follow a change exactly (powers of two)
@@ -322,19 +338,6 @@ __global__ __launch_bounds__(kBlock, 3) void kb_bwd_mx(BatchView v, int t, const
                 S.coll[kk][tj] = l[0];
                 S.coll[kk + 1][tj] = l[1];
             }
-        }
-        // the wave's live quarters of the tile (bit qq: quarter qq)
-        unsigned live = 0u;
-        if (wcount > 0) {
-            bool keep = false;
-            if (lane < ng) {
-                const double4 b = grp[S.glist[k + lane]];
-                const double dx = b.x - aw[0], dy = b.y - aw[1], dz = b.z - aw[2];
-                const float ms = fmaxf(0.0f, __builtin_sqrtf((float)(dx * dx + dy * dy + dz * dz)) - (float)b.w) * kapf;
-                const float gap = ms - rtw;
-                keep = !(gap > 0.0f && gap * gap > cut2);
-            }
-            live = (unsigned)__ballot(keep) & 0xFu;
         }
         __syncthreads();   // columns and exponent operands are staged
         if (live == 0u) return;
