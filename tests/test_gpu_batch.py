@@ -238,26 +238,31 @@ def test_batched_objgrd_matches_oracle(gpu):
     B.close()
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_nan_in_one_problem_leaves_the_others_unchanged(gpu, precision):
+@pytest.mark.parametrize("precision,shape,bad_p", [("fp64", None, 2), ("fp32", None, 2), ("fp32", (1024, 2, 4), 5)])
+def test_nan_in_one_problem_leaves_the_others_unchanged(gpu, precision, shape, bad_p):
     """A NaN in one problem's initial momentum: its objective and gradient
     are NaN, so its first direction has no negative slope and it stops with
     status 1 (not a descent direction, Cpp/src/lineSearch.cpp:39-46) at its
-    start; every other problem runs exactly as in a batch without the NaN."""
+    start; every other problem runs exactly as in a batch without the NaN.
+    The third case puts the NaN into the 2,562-point problem under the
+    matrix-core adjoint (adjoint lanes 4), whose column scales come from
+    per-group maxima that skip NaN."""
     probs, k = small_batch(), kernels()
     B = gpu.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"], k["knl_order"],
                   k["time_stp"], k["time_num"], chunk=256)
+    if shape is not None:
+        B.set_tuning(*shape)
     zero = [np.zeros((k["time_num"] - 1, B.n[i], 3)) for i in range(B.P)]
     a_ref, i_ref, _ = B.run(8, 0.0, 0.0, 0.5, 5, alp0=zero, precision=precision)
     bad = [z.copy() for z in zero]
-    bad[2][1, 3, 0] = np.nan
+    bad[bad_p][1, 3, 0] = np.nan
     a_nan, i_nan, _ = B.run(8, 0.0, 0.0, 0.5, 5, alp0=bad, precision=precision)
-    print("NaN problem (%s): status %d, iterations %d" % (precision, i_nan[2]["status"], i_nan[2]["iterations"]),
-          flush=True)
-    assert i_nan[2]["status"] == 1 and i_nan[2]["iterations"] == 0
-    assert np.isnan(a_nan[2][1, 3, 0]) and np.count_nonzero(np.isnan(a_nan[2])) == 1   # the start is returned
+    print("NaN problem %d (%s, shape %s): status %d, iterations %d" %
+          (bad_p, precision, shape, i_nan[bad_p]["status"], i_nan[bad_p]["iterations"]), flush=True)
+    assert i_nan[bad_p]["status"] == 1 and i_nan[bad_p]["iterations"] == 0
+    assert np.isnan(a_nan[bad_p][1, 3, 0]) and np.count_nonzero(np.isnan(a_nan[bad_p])) == 1   # the start is returned
     for i in range(B.P):
-        if i == 2:
+        if i == bad_p:
             continue
         assert i_nan[i]["status"] == i_ref[i]["status"] == 0
         assert i_nan[i]["iterations"] == i_ref[i]["iterations"] and i_nan[i]["fcn"] == i_ref[i]["fcn"]
