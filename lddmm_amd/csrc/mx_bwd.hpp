@@ -235,8 +235,6 @@ __global__ __launch_bounds__(kBlock, 3) void kb_bwd_mx(BatchView v, int t, const
     // surviving-group list (ng groups from position k).
     auto tile_run = [&](int k, int ng) {
         __syncthreads();   // the previous tile is consumed; the group list is written
-        // valid sources of each 64-slot quarter, 8 bits each, and the
-        // tile's bounds on |x|, |a|, |a - p| from its groups' data (uniform)
         // lane qq < ng reads quarter qq's group once: its valid sources, the
         // bound of its offsets from the block anchor, its maxima of |a| and
         // |a - p|, and the wave's culling test against it; the wave combines
