@@ -1376,6 +1376,10 @@ void Batch::autotune()
                     LDDMM_LAUNCH_CHECK();
                     bounds = true;
                 }
+                if (r == 4 && mx_ok() && v.gpts && f32) {   // column scales of the matrix-core adjoint
+                    kb_group_maxima<3><<<tb.nvtile, kTile, 0, st_>>>(v, 0, tb.vtile.p);
+                    LDDMM_LAUNCH_CHECK();
+                }
                 const float kv = timed([&] {
                     dispatch([&](auto Dc, auto Oc) {
                         with_lanes(r, [&](auto Rc) {
