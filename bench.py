@@ -238,8 +238,9 @@ def warm_start_leg(probs, targets, e_ref, B, B64, t_setup, itr, weights):
     parameters and training record in `weights`).  Timed: the features
     (including the start's varifold gradient), inference, the per-problem
     start scale (choose_start: the multiple of the prediction, zero included,
-    with the lowest objective), the batched FP32 registration to the plain
-    energies (at most 1.25 times the plain iteration limit) and the
+    with the lowest objective, from energy-only evaluations of the batch on
+    the device), the batched FP32 registration to the plain energies from
+    that start (at most 1.25 times the plain iteration limit) and the
     fallback, a zero-momentum batched FP32 run for every problem the
     warm-started run did not bring to its target.  B and B64 are the batch in
     FP32 and FP64 (the latter re-evaluates results)."""
@@ -248,11 +249,17 @@ def warm_start_leg(probs, targets, e_ref, B, B64, t_setup, itr, weights):
     T = k["time_num"]
     ws = W.WarmStart.load(weights)
     t0 = time.perf_counter()
-    warm = [ws.predict(p, T) for p in probs]
-    t_inf = time.perf_counter() - t0
-    starts, scale = W.choose_start(B, warm)
-    aw, iw, rw = B.run(int(1.25 * itr), 0.0, 0.0, 0.5, 5, alp0=starts, precision="fp32", targets=targets)
+    base = [ws.start_vector(p) for p in probs]
+    t1 = time.perf_counter()
+    scale, _, _ = W.choose_start(B, base)
+    t2 = time.perf_counter()
+    aw, iw, rw = B.run_scaled(base, scale, int(1.25 * itr), 0.0, 0.0, 0.5, 5, precision="fp32", targets=targets)
+    t3 = time.perf_counter()
     missed = [i for i, r in enumerate(iw) if not r["reached_target"]]
+    # where the warm-started run left each missed problem: its multiple and
+    # the relative gap of its energy (batch precision) to the target
+    missed_at = [dict(problem=i, n=int(B.n[i]), start_multiple=float(scale[i]),
+                      rel_gap=float((iw[i]["fcn"] - targets[i]) / abs(targets[i]))) for i in missed]
     its_fb = []
     if missed:
         Bf = lddmm_amd.Batch([probs[i] for i in missed], k["cen_type"], k["cen_width"], k["dir_type"],
@@ -262,16 +269,21 @@ def warm_start_leg(probs, targets, e_ref, B, B64, t_setup, itr, weights):
         for j, i in enumerate(missed):
             aw[i] = af[j]
         its_fb = [r["iterations"] for r in ifb]
-    t_warm = time.perf_counter() - t0 + t_setup
+    t4 = time.perf_counter()
+    t_warm = t4 - t0 + t_setup
+    t_inf = t1 - t0
     ws.close()
     f, _ = B64.objgrd(aw)
     rel_w = (f - e_ref) / np.abs(e_ref)
     count = len(probs)
     m = ws.meta
     return dict(seconds=t_warm, registrations_per_s=count / t_warm, rounds=rw, inference_seconds=t_inf,
+                phase_seconds=dict(setup=t_setup, features_and_inference=t_inf, choose_start=t2 - t1,
+                                   warm_run=t3 - t2, fallback=t4 - t3),
+                start_multiples={str(c): int(np.sum(scale == c)) for c in np.unique(scale)},
                 median_iterations=float(np.median([r["iterations"] for r in iw])),
                 warm_start_used=int(np.sum(scale > 0)), reached_target=count - len(missed),
-                fallback_problems=len(missed), fallback_iterations=its_fb,
+                fallback_problems=len(missed), fallback_iterations=its_fb, fallback_from=missed_at,
                 within_1e6=int(np.sum(rel_w <= 1e-6 + 1e-8)), max_rel_energy_vs_plain=float(rel_w.max()),
                 training=dict(problems=int(m.get("problems", 0)), rows=int(m.get("rows", 0)),
                               solver="plain FP64 L-BFGS (batched evaluation), other seeds than the benchmark's",

@@ -234,6 +234,22 @@ int lddmm_batch_objgrd(lddmm_batch *b, double *fcnVals, double *grdStks, const d
 int lddmm_batch_run(lddmm_batch *b, const double *iniAlpStks, double *optAlpStks, int itrMax, double tolVal,
                     double wolfe1, double wolfe2, int memNum, const lddmm_options *opts,
                     const double *targetEnergies, lddmm_result *results, int *rounds);
+/* Energies at momentum stacks that are constant in time (no counterpart in
+ * the reference; for choosing a start of Cpp/src/LDDMM.cpp:87, iniAlpVec, by
+ * objective value): fcnVals[k * P + p] is problem p's energy, in the batch's
+ * precision, at alpha_t = scales[k] u_p for every t, with u the concatenated
+ * D x N_p blocks baseVecs (host layout).  Energies only: no gradient is
+ * formed. */
+int lddmm_batch_objective_scan(lddmm_batch *b, const double *baseVecs, const double *scales, int nScales,
+                               double *fcnVals);
+/* lddmm_batch_run from the constant-in-time start alpha_t = startScales[p] u_p
+ * (baseVecs as above; a zero multiple is the zero start), built on the
+ * device rather than passed as full stacks (no counterpart in the reference;
+ * the start is Cpp/src/LDDMM.cpp:87's iniAlpVec). */
+int lddmm_batch_run_scaled(lddmm_batch *b, const double *baseVecs, const double *startScales, double *optAlpStks,
+                           int itrMax, double tolVal, double wolfe1, double wolfe2, int memNum,
+                           const lddmm_options *opts, const double *targetEnergies, lddmm_result *results,
+                           int *rounds);
 /* chosen source chunk and the pair interactions per evaluation of each problem */
 int lddmm_batch_info(lddmm_batch *b, int *chunk, double *pairsPerEval);
 /* the launch shape in use: source points per work item, and targets per

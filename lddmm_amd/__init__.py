@@ -495,6 +495,10 @@ class Batch:
             L.lddmm_batch_objgrd.argtypes = [C.c_void_p, _dp, _dp, _dp]
             L.lddmm_batch_run.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_double,
                                           C.c_int, C.POINTER(Options), _dp, C.POINTER(Result), C.POINTER(C.c_int)]
+            L.lddmm_batch_objective_scan.argtypes = [C.c_void_p, _dp, _dp, C.c_int, _dp]
+            L.lddmm_batch_run_scaled.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_double, C.c_double,
+                                                 C.c_double, C.c_int, C.POINTER(Options), _dp, C.POINTER(Result),
+                                                 C.POINTER(C.c_int)]
             L.lddmm_batch_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), _dp]
             L.lddmm_batch_set_graphs.argtypes = [C.c_void_p, C.c_int]
             L.lddmm_batch_tuning.argtypes = [C.c_void_p] + [C.POINTER(C.c_int)] * 3
@@ -538,14 +542,45 @@ class Batch:
     def run(self, itr_max, tol, wolfe1, wolfe2, mem, alp0=None, precision="fp64", rel_energy_tol=0.0,
             fp64_refine=True, refine_rel_tol=0.0, targets=None, progress=0):
         a0 = None if alp0 is None else _f64(np.concatenate([np.ravel(x) for x in alp0]))
+        return self._run(lambda *r: lib().lddmm_batch_run(self._h, _p(a0), *r), itr_max, tol, wolfe1, wolfe2, mem,
+                         precision, rel_energy_tol, fp64_refine, refine_rel_tol, targets, progress)
+
+    def _vecs(self, base):
+        base = [np.asarray(u, dtype=np.float64) for u in base]
+        if len(base) != self.P or any(u.shape != (self.n[p], self.dim) for p, u in enumerate(base)):
+            raise ValueError("one (N_p, D) vector array per problem expected")
+        return _f64(np.concatenate([np.ravel(u) for u in base]))
+
+    def objective_scan(self, base, scales):
+        """Energies (len(scales), P), in the batch's precision, at the momenta
+        alpha_t = c u_p for every time slice and each multiple c; base: one
+        (N_p, D) array per problem.  Energies only (lddmm_batch_objective_scan)."""
+        u = self._vecs(base)
+        sc = _f64(scales)
+        f = np.empty((len(sc), self.P))
+        _check(lib().lddmm_batch_objective_scan(self._h, _p(u), _p(sc), len(sc), _p(f)))
+        return f
+
+    def run_scaled(self, base, scales, itr_max, tol, wolfe1, wolfe2, mem, precision="fp64", rel_energy_tol=0.0,
+                   fp64_refine=True, refine_rel_tol=0.0, targets=None, progress=0):
+        """run() from the start alpha_t = scales[p] u_p (every time slice),
+        built on the device (lddmm_batch_run_scaled)."""
+        u = self._vecs(base)
+        sc = _f64(scales)
+        if sc.shape != (self.P,):
+            raise ValueError("one start multiple per problem expected")
+        return self._run(lambda *r: lib().lddmm_batch_run_scaled(self._h, _p(u), _p(sc), *r), itr_max, tol, wolfe1,
+                         wolfe2, mem, precision, rel_energy_tol, fp64_refine, refine_rel_tol, targets, progress)
+
+    def _run(self, call, itr_max, tol, wolfe1, wolfe2, mem, precision, rel_energy_tol, fp64_refine, refine_rel_tol,
+             targets, progress):
         out = np.empty(int(self.offsets[-1]))
         opts = Options(PRECISIONS[precision], 0 if fp64_refine else 1, rel_energy_tol, refine_rel_tol, 0.0, 0, 0,
                        int(progress))
         res = (Result * self.P)()
         rounds = C.c_int()
         tg = None if targets is None else _f64(targets)
-        _check(lib().lddmm_batch_run(self._h, _p(a0), _p(out), itr_max, tol, wolfe1, wolfe2, mem, C.byref(opts),
-                                     _p(tg), res, C.byref(rounds)))
+        _check(call(_p(out), itr_max, tol, wolfe1, wolfe2, mem, C.byref(opts), _p(tg), res, C.byref(rounds)))
         infos = [dict(fcn=r.fcn, grad_norm=r.grad_norm, seconds=r.seconds, pairs_per_eval=r.pairs_per_eval,
                       iterations=r.iterations, evaluations=r.evaluations, status=r.status,
                       fp32_iterations=r.fp32_iterations, reached_target=bool(r.reached_target)) for r in res]

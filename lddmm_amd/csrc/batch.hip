@@ -109,6 +109,21 @@ __global__ __launch_bounds__(kTile) void kb_fill_pattern(double *__restrict__ x,
     }
 }
 
+// x = the momentum stack of every problem p with each time slice s[p] u_p
+// (u: D x N_p blocks in the device layout); blockIdx.y is the problem
+__global__ __launch_bounds__(kTile) void kb_const_momentum(const BatchDesc *__restrict__ desc, int D, int T1,
+                                                           const double *__restrict__ u, const double *__restrict__ s,
+                                                           double *__restrict__ x)
+{
+    const BatchDesc d = desc[blockIdx.y];
+    const long long m = (long long)D * d.n;
+    const double c = s[blockIdx.y];
+    const double *up = u + (long long)D * d.vo;
+    double *xp = x + (long long)D * T1 * d.vo;
+    for (long long i = (long long)blockIdx.x * kTile + threadIdx.x; i < m * T1; i += (long long)gridDim.x * kTile)
+        xp[i] = c * up[i % m];
+}
+
 // bounds of the aligned 64-point groups: q_t (which = 0), the deformed
 // element centres (1) or the target centres (2), one group per item
 template <int D>
@@ -1227,6 +1242,7 @@ void Batch::launch_eval(const double *alp, double *grd, double *f_dev)
         kb_energy<<<tb.nprobs, kBlock, 0, st_>>>(v, tt_, tb.probs.p, f_dev);
         LDDMM_LAUNCH_CHECK();
     }
+    if (grd == nullptr) return;   // energies only
     for (int t = T - 2; t >= 0; --t) {
         {
             ProfScope ps(kProfBwdPairs, st_);
@@ -1449,6 +1465,26 @@ std::vector<double> Batch::to_device_layout(const double *h) const
                         h[off + (size_t)t * D * n + (size_t)vperm_[vo_[p] + i] * D + c];
     }
     return out;
+}
+
+std::vector<double> Batch::vec_to_device_layout(const double *h) const
+{
+    const int D = spec_.dim;
+    std::vector<double> out((size_t)D * ntot_);
+    for (int p = 0; p < spec_.P(); ++p) {
+        const int n = spec_.n[p];
+        const size_t off = (size_t)D * vo_[p];
+        for (int i = 0; i < n; ++i)
+            for (int c = 0; c < D; ++c) out[off + (size_t)c * n + i] = h[off + (size_t)vperm_[vo_[p] + i] * D + c];
+    }
+    return out;
+}
+
+void Batch::const_momentum(const double *u, const double *s, double *x)
+{
+    if (spec_.P() > 65535) throw Error(kErrArg, "a constant-in-time start takes at most 65535 problems");
+    kb_const_momentum<<<dim3(32, spec_.P()), kTile, 0, st_>>>(desc_, spec_.dim, spec_.T - 1, u, s, x);
+    LDDMM_LAUNCH_CHECK();
 }
 
 void Batch::to_host_layout(const std::vector<double> &dv, double *h) const

@@ -93,12 +93,18 @@ public:
     double pairs_per_eval(int p) const;
 
     // f_dev[p], grd (device, concatenated per-problem [T-1][D][N_p] blocks)
-    // for the problems with prec_of[p] >= 0 (their pair arithmetic)
+    // for the problems with prec_of[p] >= 0 (their pair arithmetic); grd =
+    // nullptr evaluates the energies only (no adjoint sweep)
     void eval(const double *alp, double *grd, double *f_dev, const std::vector<int> &prec_of);
     // host layout (caller's vertex order) <-> device layout (internal
     // spatial order of every problem, as in Objective)
     std::vector<double> to_device_layout(const double *host_alp) const;
     void to_host_layout(const std::vector<double> &dev, double *host_alp) const;
+    // the same for one vector per vertex (D x N_p blocks, no time axis)
+    std::vector<double> vec_to_device_layout(const double *host_vec) const;
+    // x = the momentum stack whose every time slice is s[p] u_p: u the
+    // device-layout D x N_p blocks, s one multiple per problem (both device)
+    void const_momentum(const double *u, const double *s, double *x);
     const double *target_terms() const { return tt_; }
 
     BatchRunResult run(double *x, const BatchLbfgsOptions &o);

@@ -623,6 +623,43 @@ int lddmm_batch_objgrd(lddmm_batch *b, double *fcnVals, double *grdStks, const d
     });
 }
 
+// runs the batch from the device-layout start x and returns the results in
+// the host layout
+static void batch_run_from(lddmm_batch *b, DevBuf<double> &x, double *optAlpStks, int itrMax, double tolVal,
+                           double wolfe1, double wolfe2, int memNum, const lddmm_options *opts,
+                           const double *targetEnergies, lddmm_result *results, int *rounds)
+{
+    Batch &B = *b->b;
+    BatchLbfgsOptions bo;
+    bo.base = make_opts(itrMax, tolVal, wolfe1, wolfe2, memNum, 0, opts);
+    if (opts) {
+        bo.prec = precision_of(opts);
+        bo.progress = opts->progress;
+    }
+    if (targetEnergies) bo.targets.assign(targetEnergies, targetEnergies + B.P());
+    BatchRunResult r = B.run(x.p, bo);
+    std::vector<double> hx((size_t)B.varnum_total());
+    x.download(hx.data(), b->st.s);
+    b->st.sync();
+    B.to_host_layout(hx, optAlpStks);
+    if (rounds) *rounds = r.rounds;
+    if (results) {
+        for (int p = 0; p < B.P(); ++p) {
+            lddmm_result &q = results[p];
+            std::memset(&q, 0, sizeof(q));
+            q.fcn = r.prob[p].f;
+            q.grad_norm = r.prob[p].gnorm;
+            q.seconds = r.seconds;
+            q.pairs_per_eval = B.pairs_per_eval(p);
+            q.iterations = r.prob[p].iters;
+            q.evaluations = r.prob[p].evals;
+            q.status = r.prob[p].status;
+            q.fp32_iterations = r.prob[p].fp32_iters;
+            q.reached_target = r.prob[p].reached_target;
+        }
+    }
+}
+
 int lddmm_batch_run(lddmm_batch *b, const double *iniAlpStks, double *optAlpStks, int itrMax, double tolVal,
                     double wolfe1, double wolfe2, int memNum, const lddmm_options *opts,
                     const double *targetEnergies, lddmm_result *results, int *rounds)
@@ -630,37 +667,66 @@ int lddmm_batch_run(lddmm_batch *b, const double *iniAlpStks, double *optAlpStks
     return guarded([&] {
         require(b && optAlpStks, "null argument");
         Batch &B = *b->b;
-        BatchLbfgsOptions bo;
-        bo.base = make_opts(itrMax, tolVal, wolfe1, wolfe2, memNum, 0, opts);
-        if (opts) {
-            bo.prec = precision_of(opts);
-            bo.progress = opts->progress;
+        DevBuf<double> x((size_t)B.varnum_total());
+        if (iniAlpStks) {
+            const std::vector<double> hx = B.to_device_layout(iniAlpStks);
+            x.upload(hx.data(), b->st.s);
+            b->st.sync();
+        } else if (x.n) {
+            LDDMM_HIP(hipMemsetAsync(x.p, 0, x.n * sizeof(double), b->st.s));
         }
-        if (targetEnergies) bo.targets.assign(targetEnergies, targetEnergies + B.P());
-        std::vector<double> hx((size_t)B.varnum_total(), 0.0);
-        if (iniAlpStks) hx = B.to_device_layout(iniAlpStks);
-        DevBuf<double> x(hx.size());
-        x.upload(hx.data(), b->st.s);
-        BatchRunResult r = B.run(x.p, bo);
-        x.download(hx.data(), b->st.s);
+        batch_run_from(b, x, optAlpStks, itrMax, tolVal, wolfe1, wolfe2, memNum, opts, targetEnergies, results,
+                       rounds);
+        return 0;
+    });
+}
+
+int lddmm_batch_objective_scan(lddmm_batch *b, const double *baseVecs, const double *scales, int nScales,
+                               double *fcnVals)
+{
+    return guarded([&] {
+        require(b && baseVecs && scales && fcnVals, "null argument");
+        require(nScales > 0, "nScales must be positive");
+        Batch &B = *b->b;
+        const int P = B.P();
+        const std::vector<double> hu = B.vec_to_device_layout(baseVecs);
+        std::vector<double> hs((size_t)nScales * P);
+        for (int k = 0; k < nScales; ++k) std::fill(hs.begin() + (size_t)k * P, hs.begin() + (size_t)(k + 1) * P, scales[k]);
+        DevBuf<double> u(hu.size()), s(hs.size()), x((size_t)B.varnum_total()), f((size_t)P),
+            fall((size_t)nScales * P);
+        u.upload(hu.data(), b->st.s);
+        s.upload(hs.data(), b->st.s);
+        const std::vector<int> prec(P, B.spec().prec);
+        // one set of buffers for every multiple, so that the evaluation's
+        // captured graph is replayed rather than captured per multiple
+        for (int k = 0; k < nScales; ++k) {
+            B.const_momentum(u.p, s.p + (size_t)k * P, x.p);
+            B.eval(x.p, nullptr, f.p, prec);
+            LDDMM_HIP(hipMemcpyAsync(fall.p + (size_t)k * P, f.p, (size_t)P * sizeof(double), hipMemcpyDeviceToDevice,
+                                     b->st.s));
+        }
+        fall.download(fcnVals, b->st.s);
         b->st.sync();
-        B.to_host_layout(hx, optAlpStks);
-        if (rounds) *rounds = r.rounds;
-        if (results) {
-            for (int p = 0; p < B.P(); ++p) {
-                lddmm_result &q = results[p];
-                std::memset(&q, 0, sizeof(q));
-                q.fcn = r.prob[p].f;
-                q.grad_norm = r.prob[p].gnorm;
-                q.seconds = r.seconds;
-                q.pairs_per_eval = B.pairs_per_eval(p);
-                q.iterations = r.prob[p].iters;
-                q.evaluations = r.prob[p].evals;
-                q.status = r.prob[p].status;
-                q.fp32_iterations = r.prob[p].fp32_iters;
-                q.reached_target = r.prob[p].reached_target;
-            }
-        }
+        return 0;
+    });
+}
+
+int lddmm_batch_run_scaled(lddmm_batch *b, const double *baseVecs, const double *startScales, double *optAlpStks,
+                           int itrMax, double tolVal, double wolfe1, double wolfe2, int memNum,
+                           const lddmm_options *opts, const double *targetEnergies, lddmm_result *results,
+                           int *rounds)
+{
+    return guarded([&] {
+        require(b && baseVecs && startScales && optAlpStks, "null argument");
+        Batch &B = *b->b;
+        const std::vector<double> hu = B.vec_to_device_layout(baseVecs);
+        DevBuf<double> u(hu.size()), s((size_t)B.P()), x((size_t)B.varnum_total());
+        u.upload(hu.data(), b->st.s);
+        s.upload(startScales, b->st.s);
+        B.const_momentum(u.p, s.p, x.p);
+        b->st.sync();
+        batch_run_from(b, x, optAlpStks, itrMax, tolVal, wolfe1, wolfe2, memNum, opts, targetEnergies, results,
+                       rounds);
         return 0;
     });
 }

@@ -175,7 +175,7 @@ def vertex_area(q0, elm):
     a = 0.5 * np.linalg.norm(np.cross(q[e[:, 1]] - q[e[:, 0]], q[e[:, 2]] - q[e[:, 0]]), axis=1)
     out = np.zeros(len(q))
     for c in range(3):
-        np.add.at(out, e[:, c], a / 3.0)
+        out += np.bincount(e[:, c], weights=a / 3.0, minlength=len(q))
     return out
 
 
@@ -233,10 +233,15 @@ class WarmStart:
         f, A = self.features(problem)
         return self.net.forward((f - self.fmean) / self.fstd).astype(np.float64) * self.yscale, A
 
+    def start_vector(self, problem):
+        """(N, 3): the predicted momentum of every time slice (density x
+        vertex area)."""
+        m, A = self.density(problem)
+        return m * A[:, None]
+
     def predict(self, problem, time_num):
         """(T-1, N, 3) initial momentum stack for one problem."""
-        m, A = self.density(problem)
-        return np.repeat((m * A[:, None])[None], time_num - 1, axis=0)
+        return np.repeat(self.start_vector(problem)[None], time_num - 1, axis=0)
 
     def save(self, path, **extra):
         meta = dict(self.meta, **extra)
@@ -259,15 +264,17 @@ class WarmStart:
         self.net.close()
 
 
-def choose_start(batch, warm, scales=(0.25, 0.5, 0.75, 1.0, 1.25, 1.5)):
-    """Per problem, the multiple c * warm (c in scales, or c = 0: zero
-    momentum) with the lowest objective, from one batched evaluation per
-    scale.  Returns (starts, chosen scale per problem)."""
-    f_best, _ = batch.objgrd([np.zeros_like(a) for a in warm])
-    c_best = np.zeros(len(warm))
-    for c in scales:
-        f, _ = batch.objgrd([c * a for a in warm])
-        better = f < f_best
-        f_best = np.where(better, f, f_best)
-        c_best = np.where(better, c, c_best)
-    return [c * a for c, a in zip(c_best, warm)], c_best
+def choose_start(batch, base, scales=(0.25, 0.5, 0.75, 1.0, 1.25, 1.5)):
+    """Per problem, the multiple c of the constant-in-time start `base` (one
+    (N_p, 3) array per problem; c in scales, or c = 0: zero momentum) with the
+    lowest objective, from one energy-only batched evaluation per multiple on
+    the device (Batch.objective_scan); a later multiple replaces an earlier
+    one only when strictly lower.  Returns (chosen multiple per problem, its
+    objective, the zero-start objective)."""
+    cs = np.concatenate([[0.0], np.asarray(scales, dtype=np.float64)])
+    f = batch.objective_scan(base, cs)
+    best = np.zeros(f.shape[1], dtype=np.int64)
+    cols = np.arange(f.shape[1])
+    for k in range(1, len(cs)):
+        best = np.where(f[k] < f[best, cols], k, best)
+    return cs[best], f[best, cols], f[0]

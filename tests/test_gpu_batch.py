@@ -238,6 +238,47 @@ def test_batched_objgrd_matches_oracle(gpu):
     B.close()
 
 
+@pytest.mark.parametrize("precision,shape", [("fp64", None), ("fp32", (1024, 2, 4))])
+def test_objective_scan_and_scaled_start(gpu, precision, shape):
+    """Momenta constant in time, built on the device from one (N, 3) array
+    per problem: lddmm_batch_objective_scan's energies (no adjoint sweep) at
+    multiples 0, 0.5, 1, 2 equal lddmm_batch_objgrd's at the same stacks, and
+    in FP64 the oracle's objgrd (Cpp/src/objgrd.cpp:26-90) within 1e-10 for the
+    problems of up to 700 points; lddmm_batch_run_scaled runs exactly as
+    lddmm_batch_run from those stacks (a zero multiple is the zero start)."""
+    probs, k = small_batch(), kernels()
+    T1 = k["time_num"] - 1
+    B = gpu.Batch(probs, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"], k["knl_order"],
+                  k["time_stp"], k["time_num"], precision=precision, chunk=256)
+    if shape is not None:
+        B.set_tuning(*shape)
+    base = [0.05 * G.normal(180 + i, 3 * B.n[i]).reshape(B.n[i], 3) for i in range(B.P)]
+    scales = [0.0, 0.5, 1.0, 2.0]
+    fs = B.objective_scan(base, scales)
+    print("objective scan (%s, shape %s): %s" % (precision, shape, np.round(fs[:, :3], 6).tolist()), flush=True)
+    for j, c in enumerate(scales):
+        stacks = [np.repeat((c * u)[None], T1, axis=0) for u in base]
+        f, _ = B.objgrd(stacks)
+        assert np.all(np.abs(fs[j] - f) <= 1e-13 * np.abs(f)), (c, fs[j], f)
+        if precision == "fp64" and c in (0.0, 1.0):
+            for i, p in enumerate(probs):
+                if B.n[i] <= 700:
+                    ef, _ = oracle_problem(p, k).objgrd(stacks[i].reshape(-1))
+                    assert abs(fs[j][i] - ef) <= 1e-10 * abs(ef), (c, i)
+    start = np.array([0.0, 1.0, 0.5, 2.0, 1.0, 0.5])
+    a1, i1, r1 = B.run_scaled(base, start, 6, 0.0, 0.0, 0.5, 5, precision=precision)
+    a2, i2, r2 = B.run(6, 0.0, 0.0, 0.5, 5, alp0=[np.repeat((c * u)[None], T1, axis=0) for c, u in zip(start, base)],
+                       precision=precision)
+    assert r1 == r2
+    for i in range(B.P):
+        assert (i1[i]["status"], i1[i]["iterations"], i1[i]["evaluations"]) == \
+            (i2[i]["status"], i2[i]["iterations"], i2[i]["evaluations"]), i
+        assert i1[i]["fcn"] == i2[i]["fcn"] and np.array_equal(a1[i], a2[i]), i
+    with pytest.raises(ValueError):
+        B.run_scaled(base, start[:-1], 2, 0.0, 0.0, 0.5, 5)
+    B.close()
+
+
 @pytest.mark.parametrize("precision,shape,bad_p", [("fp64", None, 2), ("fp32", None, 2), ("fp32", (1024, 2, 4), 5)])
 def test_nan_in_one_problem_leaves_the_others_unchanged(gpu, precision, shape, bad_p):
     """A NaN in one problem's initial momentum: its objective and gradient

@@ -155,11 +155,9 @@ def test_trained_warm_start_gain_at_full_time_grid(gpu, ws):
     _, i0, _ = B.run(300, 0.0, 0.0, 0.5, 5, rel_energy_tol=1e-6)
     targets = np.array([r["fcn"] for r in i0]) * (1 + 1e-6)
     _, iz, _ = B.run(900, 0.0, 0.0, 0.5, 5, precision="fp32", targets=targets)
-    warm = [w.predict(p, T) for p in test]
-    starts, scale = ws.choose_start(B, warm)
-    f0, _ = B.objgrd([np.zeros_like(a) for a in warm])
-    fs, _ = B.objgrd(starts)
-    _, iw, _ = B.run(375, 0.0, 0.0, 0.5, 5, alp0=starts, precision="fp32", targets=targets)
+    base = [w.start_vector(p) for p in test]
+    scale, fs, f0 = ws.choose_start(B, base)
+    _, iw, _ = B.run_scaled(base, scale, 375, 0.0, 0.0, 0.5, 5, precision="fp32", targets=targets)
     itz = np.array([r["iterations"] for r in iz])
     reached = np.array([r["reached_target"] for r in iw])
     costw = np.array([r["iterations"] for r in iw]) + np.where(reached, 0, itz)
