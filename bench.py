@@ -156,15 +156,7 @@ def cpu_baseline(w, ifo, pairs_per_eval):
     vf_pairs_full = ed * ed + ed * et + et * et     # the reference recomputes the target-target term
     t_eval = T1 * (2 * t_mult + t_dq) + t_vf * vf_pairs_full / vf_pairs_sample
     sample_s = t_mult + t_dq + t_vf
-    model = "unknown"
-    try:
-        with open("/proc/cpuinfo") as fh:
-            for line in fh:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
+    model = _cpu_model()
     what = ("the reference's Cpp/src multiplyKernel, dqKernel and varifold (oracle/_ref)" if L is not None
             else "oracle/lddmm_oracle.c")
     return dict(value=pairs_per_eval / t_eval, unit="pair-interactions/s", cores=1,
@@ -249,7 +241,7 @@ def warm_start_leg(probs, targets, e_ref, B, B64, t_setup, itr, weights):
     T = k["time_num"]
     ws = W.WarmStart.load(weights)
     t0 = time.perf_counter()
-    base = [ws.start_vector(p) for p in probs]
+    base = ws.start_vectors(probs)
     t1 = time.perf_counter()
     scale, _, _ = W.choose_start(B, base)
     t2 = time.perf_counter()
@@ -428,6 +420,116 @@ def batch_bench(count, nmin, nmax, itr, seed=4, warm=True, batched_fp64=False, b
     return out
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baselines_report(threads=16, rows=1500):
+    """CPU baselines of the single-problem configurations, beside the GPU on
+    the same host.
+
+    circle->square (400 landmarks, 2D, 200 L-BFGS iterations): the plain FP64
+    registration on the GPU, and end to end on one core the oracle's full run
+    (the FP64 C restatement of Cpp/src objgrd, LBFGS, getDirection and
+    lineSearch; kind "port": the reference's own driver is not built), with
+    final energies and evaluation counts.
+
+    The single large problem (seeded 100,002-vertex superquadric, Gaussian
+    kernel of width 0.2): one matrix-free forward Euler step v = K(q) alpha
+    (1.0e10 point pairs).  GPU: the multiplyKernel entry point in FP64 (host
+    arrays in and out).  CPU: the reference's own multiplyKernel (oracle/_ref,
+    its cross form; the oracle's where oracle/_ref was not built) on `rows`
+    target rows against all sources, on one core and on `threads` host threads
+    at once (each its own rows; the reference has no threading, so the
+    all-core figure is this build's parallelisation of the reference's
+    kernel), extrapolated to every row."""
+    import concurrent.futures as cf
+    import oracle
+    out = dict(host=dict(cpu=_cpu_model(), logical_cpus=os.cpu_count()))
+
+    # circle->square, 200 iterations, GPU and the oracle's full run on one core
+    w, extra = workload("circle_square")
+    n = w["q0"].shape[0]
+    ifo = lddmm_amd.getElmIfo(w["elm"], n, one_based=False)
+    common = (w["q0"], None, w["elm"], ifo, w["tgt_cen"], w["tgt_vol"], w["tgt_dir"], w["cen_type"], w["cen_width"],
+              w["dir_type"], w["dir_width"], w["dis_wgt"], w["knl_order"], w["knl_width"], w["time_stp"],
+              w["time_num"])
+    lddmm_amd.run(*common, 5, 0.0, 0.0, 0.5, 5)   # first-call set-up
+    t0 = time.perf_counter()
+    _, ig = lddmm_amd.run(*common, extra["itr"], 0.0, 0.0, 0.5, 5)
+    t_gpu = time.perf_counter() - t0
+    P = oracle.OracleProblem(w["q0"], w["elm"], w["tgt_cen"], w["tgt_dir"], w["tgt_vol"], w["cen_type"],
+                             w["cen_width"], w["dir_type"], w["dir_width"], w["dis_wgt"], w["knl_order"],
+                             w["knl_width"], w["time_stp"], w["time_num"])
+    t0 = time.perf_counter()
+    _, f_cpu, tr = P.run(itr_max=extra["itr"], tol=0.0, wolfe1=0.0, wolfe2=0.5, mem=5)
+    t_cpu = time.perf_counter() - t0
+    out["circle_square"] = dict(
+        iterations=extra["itr"], gpu_seconds=t_gpu, gpu_energy=ig["fcn"], gpu_evaluations=ig["evaluations"],
+        cpu=dict(value=t_cpu, unit="s per registration", cores=1, kind="port",
+                 sample="the oracle's full %d-iteration FP64 run, end to end" % extra["itr"]),
+        cpu_energy=f_cpu, cpu_evaluations=int(tr["total_evals"]),
+        rel_energy_gpu_vs_cpu=abs(ig["fcn"] - f_cpu) / abs(f_cpu), gpu_speedup=t_cpu / t_gpu)
+    print("circle->square: GPU %.2f s, CPU %.1f s" % (t_gpu, t_cpu), file=sys.stderr, flush=True)
+
+    # the large problem: one forward Euler step K(q) alpha
+    w = G.superquadric_pair(100, 31, knl_width=0.2)
+    q = np.ascontiguousarray(w["q0"], dtype=np.float64)
+    n, d = q.shape
+    a = 0.01 * G.normal(7, q.size).reshape(q.shape)
+    lddmm_amd.multiplyKernel(q, a, w["knl_order"], w["knl_width"])   # set-up
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        lddmm_amd.multiplyKernel(q, a, w["knl_order"], w["knl_width"])
+    t_gpu = (time.perf_counter() - t0) / reps
+    L = oracle.ref(3)
+    nthr = max(1, min(int(threads), os.cpu_count() or 1))
+
+    def slice_rows(k):
+        qi = np.ascontiguousarray(q[k * rows:(k + 1) * rows])
+        v = np.empty_like(qi)
+        if L is not None:
+            L.ref_multiply_kernel_cross(v, qi, q, a, w["knl_order"], w["knl_width"], len(qi), n, d)
+        else:
+            v = oracle.multiply_kernel_cross(qi, q, a, w["knl_order"], w["knl_width"])
+        return v
+
+    t0 = time.perf_counter()
+    slice_rows(0)
+    t_one = time.perf_counter() - t0
+    with cf.ThreadPoolExecutor(nthr) as ex:
+        t0 = time.perf_counter()
+        list(ex.map(slice_rows, range(nthr)))
+        t_all = time.perf_counter() - t0
+    pairs = float(n) * n
+    step_one = t_one * pairs / (rows * n)
+    step_all = t_all * pairs / (nthr * rows * n)
+    what = ("the reference's Cpp/src multiplyKernel (oracle/_ref)" if L is not None
+            else "oracle/lddmm_oracle.c multiply_kernel_cross")
+    out["large_single_step"] = dict(
+        n=n, pairs=pairs, gpu_seconds=t_gpu, gpu_pairs_per_s=pairs / t_gpu,
+        cpu_one_core=dict(value=pairs / step_one, unit="pair-interactions/s", cores=1,
+                          kind="reference" if L is not None else "port",
+                          sample="%s, %d target rows x %d sources (%.1f s), scaled to the full step = %.1f s"
+                                 % (what, rows, n, t_one, step_one), seconds_per_step=step_one),
+        cpu_all_cores=dict(value=pairs / step_all, unit="pair-interactions/s", cores=nthr,
+                           kind="reference" if L is not None else "port",
+                           sample="%d host threads at once, %d rows each (%.1f s wall), scaled to the full "
+                                  "step = %.1f s" % (nthr, rows, t_all, step_all), seconds_per_step=step_all),
+        gpu_speedup_vs_one_core=step_one / t_gpu, gpu_speedup_vs_all_cores=step_all / t_gpu)
+    print("large step: GPU %.3f s, CPU 1 core %.0f s, %d cores %.1f s" % (t_gpu, step_one, nthr, step_all),
+          file=sys.stderr, flush=True)
+    return out
+
+
 def culling_report(count=32, seed=4, nmin=500, nmax=5000, reps=5):
     """Culling of far FP32 Gaussian tile pairs (kernel values below 2^-48):
     mean time per launch group of the pair kernels with culling off and on,
@@ -556,6 +658,8 @@ def main():
                     help="only the pair-kernel times with culling off and on")
     ap.add_argument("--launch-shapes", action="store_true",
                     help="only the batched FP32 pair-kernel times of the uneven batch at each launch shape")
+    ap.add_argument("--cpu-baselines", action="store_true",
+                    help="only the CPU baselines of circle->square (full run) and of one step of the large problem")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: there is no multi-GPU path (problems would shard by "
@@ -567,6 +671,9 @@ def main():
         return
     if args.culling_report:
         print(json.dumps(culling_report()), flush=True)
+        return
+    if args.cpu_baselines:
+        print(json.dumps(cpu_baselines_report()), flush=True)
         return
     if args.launch_shapes:
         print(json.dumps(launch_shapes_report(args.batch, args.batch_nmin, args.batch_nmax)), flush=True)

@@ -239,6 +239,22 @@ class WarmStart:
         m, A = self.density(problem)
         return m * A[:, None]
 
+    def start_vectors(self, problems, threads=8):
+        """start_vector of every problem: the features of several problems at
+        once on host threads (each library call runs on a stream of its
+        own), then one network pass over all their rows (the network is
+        row-wise, so the result is that of start_vector)."""
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(max(1, int(threads))) as ex:
+            fa = list(ex.map(self.features, problems))
+        X = np.concatenate([f for f, _ in fa])
+        m = self.net.forward((X - self.fmean) / self.fstd).astype(np.float64) * self.yscale
+        out, o = [], 0
+        for f, A in fa:
+            out.append(m[o:o + len(f)] * A[:, None])
+            o += len(f)
+        return out
+
     def predict(self, problem, time_num):
         """(T-1, N, 3) initial momentum stack for one problem."""
         return np.repeat(self.start_vector(problem)[None], time_num - 1, axis=0)

@@ -155,7 +155,9 @@ def test_trained_warm_start_gain_at_full_time_grid(gpu, ws):
     _, i0, _ = B.run(300, 0.0, 0.0, 0.5, 5, rel_energy_tol=1e-6)
     targets = np.array([r["fcn"] for r in i0]) * (1 + 1e-6)
     _, iz, _ = B.run(900, 0.0, 0.0, 0.5, 5, precision="fp32", targets=targets)
-    base = [w.start_vector(p) for p in test]
+    base = w.start_vectors(test, threads=4)
+    for p, u in zip(test, base):   # threaded features and one pass over all rows: the per-problem result
+        assert np.array_equal(u, w.start_vector(p))
     scale, fs, f0 = ws.choose_start(B, base)
     _, iw, _ = B.run_scaled(base, scale, 375, 0.0, 0.0, 0.5, 5, precision="fp32", targets=targets)
     itz = np.array([r["iterations"] for r in iz])
