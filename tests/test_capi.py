@@ -61,6 +61,21 @@ def test_get_elm_ifo_host_only():
     assert np.array_equal(lddmm_amd.getElmIfo(seg + 1, 50), oracle.get_elm_ifo(seg, 50))
 
 
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_batch_start_entries_check_arguments_on_the_host():
+    """lddmm_batch_objective_scan and lddmm_batch_run_scaled reject null
+    arguments with an error status and message before any device work (so
+    this runs without a GPU)."""
+    L = ctypes.CDLL(LIB)
+    L.lddmm_last_error.restype = ctypes.c_char_p
+    v = ctypes.c_double(1.0)
+    assert L.lddmm_batch_objective_scan(None, ctypes.byref(v), ctypes.byref(v), 1, ctypes.byref(v)) != 0
+    assert b"null argument" in L.lddmm_last_error()
+    assert L.lddmm_batch_run_scaled(None, ctypes.byref(v), ctypes.byref(v), ctypes.byref(v), 1, ctypes.c_double(0),
+                                    ctypes.c_double(0), ctypes.c_double(0.5), 5, None, None, None, None) != 0
+    assert b"null argument" in L.lddmm_last_error()
+
+
 def test_option_structs_match_ctypes(tmp_path):
     """lddmm_options / lddmm_result offsets in C equal the ctypes mirror's."""
     import lddmm_amd
