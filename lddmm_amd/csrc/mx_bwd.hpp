@@ -20,7 +20,9 @@
 // k' = exp2(-t) = 2^S exp2(-|x_j - x_i|^2 + |y_i|^2) from v_exp_f32, and the
 // products with the columns from v_mfma_f32_16x16x32_f16 on a split of both
 // factors into FP16 high and low halves (hi.hi + hi.lo + lo.hi: 22 bits of
-// each product, FP32 accumulation); the low half of k' is formed by one
+// each product, FP32 accumulation; columns 0-15 take three MFMAs per row
+// tile, and columns 16-23, whose hi and lo halves share the second tile of
+// 16, two: (hi + lo) of k' against both halves); the low half of k' is formed by one
 // mixed-precision FMA per value (k' minus its FP16 high half, exact).  The
 // wave's scale 2^S keeps k' below 2^15, and power-of-two column scales from
 // each staged tile's maxima keep the columns below 2^14, so both halves stay
@@ -58,7 +60,7 @@ typedef __fp16 mx_h2 __attribute__((ext_vector_type(2)));
 constexpr int kMxTI = 256;          // targets per block
 constexpr int kMxTJ = 256;          // sources per staged tile
 constexpr int kMxRow = kMxTJ + 8;   // halves per column row (16-byte pad: the rows of one read hit distinct banks)
-constexpr int kMxCols = 24;         // columns per source (two tiles of 16; lanes 8-15 of the second repeat 16-23)
+constexpr int kMxCols = 24;         // columns per source (two tiles of 16; the second holds 16-23 as hi and lo halves)
 constexpr int kMxAP = kMxTJ + 4;    // floats per component row of a wave's exponent operand
 
 struct MxSmem {
@@ -274,7 +276,7 @@ __global__ __launch_bounds__(kBlock, 3) void kb_bwd_mx(BatchView v, int t, const
                     nw = keep_or(sw, mx_pow2_below(wb, 14, sw));
         if (nx != sx || na != sa || nw != sw) {
             const float fx = nx / sx, fa = na / sa, fw = nw / sw;
-            const float f0 = colf(nn, fx, fa, fw), f1 = colf(16 + nn, fx, fa, fw);
+            const float f0 = colf(nn, fx, fa, fw), f1 = colf(16 + (nn & 7), fx, fa, fw);
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt) {
                 C[mt][0] *= f0;
@@ -345,7 +347,11 @@ __global__ __launch_bounds__(kBlock, 3) void kb_bwd_mx(BatchView v, int t, const
         for (int kb = 0; kb < 8; ++kb)
             if (((live >> (kb >> 1)) & 1u) && (int)((vpack >> (8 * (kb >> 1))) & 0xFFu) > 32 * (kb & 1))
                 kmask |= 1u << kb;
-        // second column tile: lanes 8-15 repeat columns 16-23 (their sums are not used)
+        // second column tile: lanes 0-7 take the hi halves of columns 16-23 and
+        // lanes 8-15 their lo halves, so that (ah + al) times it gives every
+        // product of the split in two MFMAs; the two lanes of a column are added
+        // at the end
+        const _Float16(*col1)[kMxRow] = nn < 8 ? S.colh : S.coll;
         const int row1 = 16 + (nn & 7);
         while (kmask != 0u) {
             const int kb = __builtin_ctz(kmask);
@@ -355,8 +361,7 @@ __global__ __launch_bounds__(kBlock, 3) void kb_bwd_mx(BatchView v, int t, const
             const float A0 = S.aop[wave][g][kb * 32 + js0], A1 = S.aop[wave][g][kb * 32 + js0 + 4];
             const mx_v8h h0c = *reinterpret_cast<const mx_v8h *>(&S.colh[nn][kb * 32 + co0]);
             const mx_v8h l0c = *reinterpret_cast<const mx_v8h *>(&S.coll[nn][kb * 32 + co0]);
-            const mx_v8h h1c = *reinterpret_cast<const mx_v8h *>(&S.colh[row1][kb * 32 + co0]);
-            const mx_v8h l1c = *reinterpret_cast<const mx_v8h *>(&S.coll[row1][kb * 32 + co0]);
+            const mx_v8h b1c = *reinterpret_cast<const mx_v8h *>(&col1[row1][kb * 32 + co0]);
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt) {
                 const mx_v4f s0 = __builtin_amdgcn_mfma_f32_16x16x4f32(A0, bs[mt], zero4, 0, 0, 0);
@@ -380,11 +385,10 @@ __global__ __launch_bounds__(kBlock, 3) void kb_bwd_mx(BatchView v, int t, const
                     al[5 + r] = (_Float16)mx_sub_h16_hi(f1, hf);
                 }
                 C[mt][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, h0c, C[mt][0], 0, 0, 0);
-                C[mt][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, h1c, C[mt][1], 0, 0, 0);
+                C[mt][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, b1c, C[mt][1], 0, 0, 0);
                 C[mt][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, l0c, C[mt][0], 0, 0, 0);
-                C[mt][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, l1c, C[mt][1], 0, 0, 0);
+                C[mt][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, b1c, C[mt][1], 0, 0, 0);
                 C[mt][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, h0c, C[mt][0], 0, 0, 0);
-                C[mt][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, h1c, C[mt][1], 0, 0, 0);
             }
         }
     };
@@ -430,9 +434,10 @@ __global__ __launch_bounds__(kBlock, 3) void kb_bwd_mx(BatchView v, int t, const
         if (col < 24) return k2 / ((double)sa * sx);
         return 0.0;
     };
-    const double cs0 = cinv(nn), cs1 = cinv(16 + nn);
+    const double cs0 = cinv(nn), cs1 = cinv(16 + (nn & 7));
     // combination with the targets' own vectors, one thread per target:
-    // lane (n, g) holds columns n and 16 + n of targets 16 mt + 4 g + r
+    // lane (n, g) holds column n of targets 16 mt + 4 g + r, and the hi (n < 8)
+    // or lo (n >= 8) part of column 16 + n % 8
     static_assert(kMxEpilogueBytes <= kMxSmemBytes, "epilogue buffer");
     double *E = reinterpret_cast<double *>(smem);   // [wave][target][column], 48 KiB
     __syncthreads();   // the last tile is consumed
@@ -442,7 +447,9 @@ __global__ __launch_bounds__(kBlock, 3) void kb_bwd_mx(BatchView v, int t, const
         for (int r = 0; r < 4; ++r) {
             const int tl = 16 * mt + 4 * g + r;
             E[((size_t)wave * 64 + tl) * kMxCols + nn] = (double)C[mt][0][r] * cs0;
-            if (nn < kMxCols - 16) E[((size_t)wave * 64 + tl) * kMxCols + 16 + nn] = (double)C[mt][1][r] * cs1;
+            const double c1 = (double)C[mt][1][r] * cs1;
+            const double c1t = c1 + __shfl_xor(c1, 8, 64);   // the column's hi and lo parts
+            if (nn < 8) E[((size_t)wave * 64 + tl) * kMxCols + 16 + nn] = c1t;
         }
     __syncthreads();
     if (!real) return;
