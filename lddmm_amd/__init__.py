@@ -542,6 +542,8 @@ class Batch:
     def run(self, itr_max, tol, wolfe1, wolfe2, mem, alp0=None, precision="fp64", rel_energy_tol=0.0,
             fp64_refine=True, refine_rel_tol=0.0, targets=None, progress=0):
         a0 = None if alp0 is None else _f64(np.concatenate([np.ravel(x) for x in alp0]))
+        if a0 is not None and a0.size != int(self.offsets[-1]):
+            raise ValueError("the initial momenta do not match the batch's sizes")
         return self._run(lambda *r: lib().lddmm_batch_run(self._h, _p(a0), *r), itr_max, tol, wolfe1, wolfe2, mem,
                          precision, rel_energy_tol, fp64_refine, refine_rel_tol, targets, progress)
 
@@ -580,6 +582,8 @@ class Batch:
         res = (Result * self.P)()
         rounds = C.c_int()
         tg = None if targets is None else _f64(targets)
+        if tg is not None and tg.shape != (self.P,):
+            raise ValueError("one target energy per problem expected")
         _check(call(_p(out), itr_max, tol, wolfe1, wolfe2, mem, C.byref(opts), _p(tg), res, C.byref(rounds)))
         infos = [dict(fcn=r.fcn, grad_norm=r.grad_norm, seconds=r.seconds, pairs_per_eval=r.pairs_per_eval,
                       iterations=r.iterations, evaluations=r.evaluations, status=r.status,
