@@ -534,6 +534,8 @@ class Batch:
     def objgrd(self, alps):
         """alps: list of (T-1, N_p, D) momenta -> (energies, list of gradients)."""
         a = _f64(np.concatenate([np.ravel(x) for x in alps]))
+        if a.size != int(self.offsets[-1]):
+            raise ValueError("the momenta do not match the batch's sizes")
         f = np.empty(self.P)
         g = np.empty_like(a)
         _check(lib().lddmm_batch_objgrd(self._h, _p(f), _p(g), _p(a)))
