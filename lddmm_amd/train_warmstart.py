@@ -1,7 +1,7 @@
 """Trains the learned warm start of the batched configuration and writes
 its parameters (lddmm_amd/data/warmstart_c4.npz by default):
 
-    python -m lddmm_amd.train_warmstart --problems 512 --seed 1004
+    python -m lddmm_amd.train_warmstart --problems 896 --chunk 64 --seed 1004
 
 Training problems come from the batched-configuration generator
 (geometry.batch_problems: jittered icospheres to seeded superquadrics,
@@ -13,6 +13,11 @@ Wolfe (0, 0.5), at most 300 iterations, stop at a relative decrease below
 for all problems of a chunk; per-problem results equal the single-problem
 path's to rounding).  The network is then fitted on the GPU (BF16 MFMA
 forward and backward, Adam) to the solutions' time-mean momentum density.
+
+A set larger than one session's solves is built in stages: a run with
+--first F --problems M --save-solved part.npz solves problems F .. F+M-1 and
+writes their time-mean momenta without fitting; a later run given
+--add-solved part.npz fits on its own problems and those.
 """
 import argparse
 import os
@@ -43,8 +48,8 @@ def solve(problems, itr=300, precision="fp64"):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--problems", type=int, default=512)
-    ap.add_argument("--chunk", type=int, default=128)
+    ap.add_argument("--problems", type=int, default=896)
+    ap.add_argument("--chunk", type=int, default=64)
     ap.add_argument("--seed", type=int, default=1004)
     ap.add_argument("--nmin", type=int, default=500)
     ap.add_argument("--nmax", type=int, default=20000)
@@ -52,21 +57,51 @@ def main(argv=None):
     ap.add_argument("--batch-rows", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--first", type=int, default=0,
+                    help="generator index of the first problem solved here (problems first .. first + problems - 1)")
+    ap.add_argument("--save-solved", default=None,
+                    help="write the solved problems' time-mean momenta to this .npz and stop (no fit)")
+    ap.add_argument("--add-solved", nargs="*", default=[],
+                    help=".npz files written by --save-solved (same seed and sizes) whose problems join the fit")
     a = ap.parse_args(argv)
     k = G.BATCH_KERNELS
     ws = W.WarmStart(hidden=128, layers=4, seed=1, cen_type=k["cen_type"], cen_width=k["cen_width"],
                      dir_type=k["dir_type"], dir_width=k["dir_width"])
     t0 = time.time()
+    # the fit uses the time mean of each solution only, so a solution is kept
+    # as its mean, one slice (T-1 = 1)
     probs, sols, its = [], [], []
-    for c0 in range(0, a.problems, a.chunk):
-        idx = list(range(c0, min(a.problems, c0 + a.chunk)))
-        ps = G.batch_problems(a.problems, a.seed, a.nmin, a.nmax, indices=idx)
+    last = a.first + a.problems
+    for c0 in range(a.first, last, a.chunk):
+        idx = list(range(c0, min(last, c0 + a.chunk)))
+        ps = G.batch_problems(last, a.seed, a.nmin, a.nmax, indices=idx)
         sol, info = solve(ps)
         probs += ps
-        sols += sol
+        sols += [np.asarray(x).mean(axis=0)[None] for x in sol]
         its += [r["iterations"] for r in info]
         print("solved %d of %d problems (%.0f s)" % (len(probs), a.problems, time.time() - t0), flush=True)
     t_solve = time.time() - t0
+    if a.save_solved:
+        np.savez(a.save_solved, indices=np.arange(a.first, last), sizes=np.array([len(x[0]) for x in sols]),
+                 means=np.concatenate([x[0] for x in sols]).astype(np.float32), iterations=np.array(its),
+                 seed=a.seed, nmin=a.nmin, nmax=a.nmax, solve_seconds=t_solve)
+        print("wrote", a.save_solved, flush=True)
+        return
+    for path in a.add_solved:
+        z = np.load(path, allow_pickle=False)
+        if (int(z["seed"]), int(z["nmin"]), int(z["nmax"])) != (a.seed, a.nmin, a.nmax):
+            raise SystemExit("%s: solved with another generator setting" % path)
+        idx = [int(i) for i in z["indices"]]
+        ps = G.batch_problems(max(idx) + 1, a.seed, a.nmin, a.nmax, indices=idx)
+        offs = np.concatenate([[0], np.cumsum(z["sizes"])])
+        for j, p in enumerate(ps):
+            if len(p["q0"]) != int(z["sizes"][j]):
+                raise SystemExit("%s: problem %d has another size" % (path, idx[j]))
+            sols.append(z["means"][offs[j]:offs[j + 1]].astype(np.float64)[None])
+        probs += ps
+        its += [int(i) for i in z["iterations"]]
+        t_solve += float(z["solve_seconds"])
+        print("added %d solved problems from %s" % (len(ps), path), flush=True)
     t0 = time.time()
     losses = ws.fit(probs, sols, steps=a.steps, batch_rows=a.batch_rows, lr=a.lr)
     t_fit = time.time() - t0
