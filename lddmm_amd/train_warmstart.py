@@ -1,7 +1,11 @@
 """Trains the learned warm start of the batched configuration and writes
 its parameters (lddmm_amd/data/warmstart_c4.npz by default):
 
-    python -m lddmm_amd.train_warmstart --problems 896 --chunk 64 --seed 1004
+    python -m lddmm_amd.train_warmstart --first 896 --problems 896 --save-solved part.npz
+    python -m lddmm_amd.train_warmstart --problems 896 --add-solved part.npz
+
+(1,792 problems in two stages of about 16 GPU-minutes each; seed 1004,
+source chunk 64.)
 
 Training problems come from the batched-configuration generator
 (geometry.batch_problems: jittered icospheres to seeded superquadrics,

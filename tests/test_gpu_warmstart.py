@@ -135,7 +135,7 @@ def test_warm_start_fit_learns_momentum_density(gpu, ws):
 
 def test_trained_warm_start_gain_at_full_time_grid(gpu, ws):
     """The network shipped in lddmm_amd/data/warmstart_c4.npz (trained by
-    lddmm_amd.train_warmstart on 896 problems of the batched configuration's
+    lddmm_amd.train_warmstart on 1,792 problems of the batched configuration's
     generator, solved by the plain FP64 method) on 12 held-out problems of
     another seed at T = 21: the batched plain FP64 run sets each energy; the
     batched FP32 policy reaches it from zero momentum and from the warm start
@@ -148,7 +148,7 @@ def test_trained_warm_start_gain_at_full_time_grid(gpu, ws):
     weights = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lddmm_amd", "data",
                            "warmstart_c4.npz")
     w = ws.WarmStart.load(weights)
-    assert w.meta["problems"] >= 896
+    assert w.meta["problems"] >= 1792
     test = G.batch_problems(12, seed=5, nmin=500, nmax=3000)
     B = gpu.Batch(test, k["cen_type"], k["cen_width"], k["dir_type"], k["dir_width"], k["dis_wgt"], k["knl_order"],
                   k["time_stp"], T)
