@@ -532,22 +532,47 @@ struct BDotArgs {
     VRef x[kMaxBDots], y[kMaxBDots];
 };
 
-__global__ __launch_bounds__(kBlock) void kb_dots1(BDotArgs a, VecTab vt, const int *__restrict__ mask,
-                                                   double *__restrict__ part, int nent)
+// First stage of nk batched dot products: one block per chunk of kVecBlk
+// elements of one problem (VecTab) computes all nk partial sums of that
+// chunk, so that a vector shared by several products (g against every ring
+// slot, d in g.d and d.d) streams from HBM once per chunk and its repeated
+// reads hit the cache.  Thread t sums elements lo + t, lo + t + kBlock, ... in
+// order, and the block sum of each product pairs t with t + w for w = 128, ...,
+// 1 (block_sum256's tree): a fixed order, whatever the launch.  Dynamic LDS:
+// nk x kBlock doubles.
+__global__ __launch_bounds__(kBlock) void kb_dots1_all(BDotArgs a, int nk, VecTab vt, const int *__restrict__ mask,
+                                                       double *__restrict__ part, int nent)
 {
-    __shared__ double red[kBlock];
-    const int k = blockIdx.y;
+    extern __shared__ double redk[];
     const int2 b = vt.blk[blockIdx.x];
     const int p = b.x;
-    const double *x, *y;
-    double s = 0.0;
-    const bool ok = (!mask || mask[p]) && vref_ptr(a.x[k], p, vt.voff[p], x) && vref_ptr(a.y[k], p, vt.voff[p], y);
-    if (ok) {
-        const long long lo = (long long)b.y * kVecBlk, hi = min(vt.vn[p], lo + kVecBlk);
-        for (long long i = lo + threadIdx.x; i < hi; i += kBlock) s += x[i] * y[i];
+    const bool live = !mask || mask[p];
+    const long long lo = (long long)b.y * kVecBlk, hi = min(vt.vn[p], lo + kVecBlk);
+    const double *x[kMaxBDots], *y[kMaxBDots];
+    bool ok[kMaxBDots];
+    double s[kMaxBDots];
+#pragma unroll
+    for (int k = 0; k < kMaxBDots; ++k) {
+        s[k] = 0.0;
+        x[k] = y[k] = nullptr;
+        ok[k] = live && k < nk && vref_ptr(a.x[k], p, vt.voff[p], x[k]) && vref_ptr(a.y[k], p, vt.voff[p], y[k]);
     }
-    const double r = block_sum256(s, red);
-    if (threadIdx.x == 0) part[(size_t)k * nent + blockIdx.x] = r;
+    for (long long i = lo + threadIdx.x; i < hi; i += kBlock) {
+#pragma unroll
+        for (int k = 0; k < kMaxBDots; ++k)
+            if (ok[k]) s[k] += x[k][i] * y[k][i];
+    }
+#pragma unroll
+    for (int k = 0; k < kMaxBDots; ++k)
+        if (k < nk) redk[k * kBlock + threadIdx.x] = s[k];
+    __syncthreads();
+#pragma unroll
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < nk; ++k) redk[k * kBlock + threadIdx.x] += redk[k * kBlock + threadIdx.x + w];
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < nk) part[(size_t)threadIdx.x * nent + blockIdx.x] = redk[threadIdx.x * kBlock];
 }
 
 // one block per problem: its entries are first[p] .. first[p]+count[p]-1
@@ -1612,7 +1637,7 @@ BatchRunResult Batch::run(double *x, const BatchLbfgsOptions &bo)
                 a.y[k] = ys[k0 + k];
             }
             ProfScope ps(kProfLbfgsDots, st_);
-            kb_dots1<<<dim3(nent, nk), kBlock, 0, st_>>>(a, vt, mask, part, nent);
+            kb_dots1_all<<<nent, kBlock, (size_t)nk * kBlock * sizeof(double), st_>>>(a, nk, vt, mask, part, nent);
             LDDMM_LAUNCH_CHECK();
             kb_dots2<<<dim3(P, nk), kBlock, 0, st_>>>(part, nent, first_d, count_d, mask, out + (size_t)k0 * P, P);
             LDDMM_LAUNCH_CHECK();
