@@ -48,3 +48,21 @@ def test_saved_problems_join_a_later_fit(monkeypatch, tmp_path):
         assert np.abs(m - exact).max() <= 1e-6 * np.abs(exact).max()   # FP32 storage of the saved means
     # the iteration counts of both stages: chunks of 2 here, of 3 in the saved run
     assert got["saved"]["median_iterations"] == float(np.median([5, 6, 5, 6, 5] + [5, 6, 7, 5]))
+
+
+def test_vertex_area_is_a_third_of_the_adjacent_triangles():
+    """WarmStart's vertex areas A_i (the density scale of its targets): a
+    third of the areas of the triangles at each vertex, against a plain loop,
+    and they add up to the surface area."""
+    q, tri = G.geodesic_sphere(3)
+    q = q + 0.01 * G.normal(9, q.size).reshape(q.shape)
+    A = W.vertex_area(q, tri)
+    ref = np.zeros(len(q))
+    total = 0.0
+    for t in np.asarray(tri):
+        a = 0.5 * np.linalg.norm(np.cross(q[t[1]] - q[t[0]], q[t[2]] - q[t[0]]))
+        total += a
+        for v in t:
+            ref[v] += a / 3.0
+    assert np.allclose(A, ref, rtol=1e-14, atol=0.0)
+    assert abs(A.sum() - total) <= 1e-12 * total
